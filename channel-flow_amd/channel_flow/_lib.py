@@ -1,0 +1,104 @@
+"""Loader for the in-tree HIP library ``lib/libcfhip.so`` (C-ABI in
+include/channel_flow.h).  There is no CPU fallback: importing a solver class
+when the library is missing raises immediately."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libcfhip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class CfParams(ctypes.Structure):
+    """Mirror of ``cf_params`` (include/channel_flow.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("cfl", ctypes.c_double), ("nu", ctypes.c_double),
+        ("dPdx", ctypes.c_double), ("dPdz", ctypes.c_double),
+        ("omega_z", ctypes.c_double),
+        ("x_mass_cte", ctypes.c_int), ("y_mass_cte", ctypes.c_int),
+        ("les_model", ctypes.c_int), ("iwall_model", ctypes.c_int),
+        ("dirichlet_nu_t", ctypes.c_int), ("jref_eqwm", ctypes.c_int),
+        ("cs", ctypes.c_double),
+        ("eqwm_ncv", ctypes.c_int), ("eqwm_max_iter", ctypes.c_int),
+        ("eqwm_stretch", ctypes.c_double), ("eqwm_tol", ctypes.c_double),
+        ("tau_mean_eqwm", ctypes.c_double),
+        ("wall_surrogate", ctypes.c_int), ("device", ctypes.c_int),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/channel_flow.h
+SIGNATURES = {
+    "cf_last_error": (ctypes.c_char_p, []),
+    "cf_create": (ctypes.c_int, [ctypes.POINTER(CfParams), _dp, _dp, _dp,
+                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "cf_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_set_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "cf_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "cf_get_pressure": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_set_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_get_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_get_wall": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "cf_set_tau_init": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_set_scalars": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_get_scalars": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_init_mass_flow": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_apply_boundary_conditions": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_compute_dt": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_rk3_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_run_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "cf_projection_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "cf_compute_rhs": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "cf_compute_eddy_viscosity": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_compute_wall_model": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
+    "cf_mean_mass_flow_u": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_check_divergence": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "cf_statistics": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_eqwm_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     _dp, _ip]),
+    "cf_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "cf_timing_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "cf_timing_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_device_bytes": (ctypes.c_longlong, [ctypes.c_void_p]),
+    "cf_sync": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def load(path=None):
+    """Load libcfhip.so and bind every C-ABI symbol; raise if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            "libcfhip.so not found at %s: build it with channel-flow_amd/build.sh "
+            "(there is no CPU fallback)" % p)
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load().cf_last_error()
+        raise RuntimeError("%s failed: %s" % (what, msg.decode() if msg else "unknown error"))
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)

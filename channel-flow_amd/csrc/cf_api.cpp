@@ -1,0 +1,934 @@
+// C-ABI of the channel-flow time-step library: a context owns the device
+// state of one channel case (one HIP stream per case) and drives the RK3
+// step of the reference (time_integration.f90:89-173) entirely on the GPU.
+// Host precomputation follows initialization.f90 (grids, operators,
+// weights); device scalars (dt, dPdx, ...) never round-trip through the host
+// inside a step, so steps can be captured in HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/channel_flow.h"
+#include "cf_eqwm.hpp"
+#include "cf_internal.hpp"
+#include "cf_kernels.hpp"
+
+namespace cf {
+
+thread_local std::string g_err;
+
+void set_error(const std::string &m) { g_err = m; }
+
+#define HCK(call)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #call);      \
+            return -1;                                                                         \
+        }                                                                                      \
+    } while (0)
+
+static FFTPlan make_plan(int n, std::string &err) {
+    FFTPlan p{};
+    p.n = n;
+    int m = n;
+    while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
+    while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
+    while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
+    if (m != 1) err = "FFT length " + std::to_string(n) + " has a prime factor other than 2 or 3";
+    return p;
+}
+
+// interpolation.f90:114-252 stencil search, 1-based centre index + weights
+static void interp_2nd(int ny, const double *y, int n, const double *yi, bool ge, int *jjo,
+                       double *w) {
+    auto Y = [&](int j) { return y[j - 1]; };
+    int jj = ny - 1;
+    for (int t = 1; t <= n; t++) {
+        const double yref = yi[t - 1];
+        if (t == 1) jj = 2;
+        if (t == n) jj = ny - 1;
+        for (int j2 = 3; j2 <= ny; j2++) {
+            const bool hit = ge ? (Y(j2) >= yref) : (Y(j2) > yref);
+            if (hit) { jj = j2 - 1; break; }
+        }
+        if (t == 1 && ge && jj == 0) jj = ny;
+        w[3 * (t - 1) + 0] = (yref - Y(jj)) * (yref - Y(jj + 1)) / ((Y(jj - 1) - Y(jj)) * (Y(jj - 1) - Y(jj + 1)));
+        w[3 * (t - 1) + 1] = (yref - Y(jj - 1)) * (yref - Y(jj + 1)) / ((Y(jj) - Y(jj - 1)) * (Y(jj) - Y(jj + 1)));
+        w[3 * (t - 1) + 2] = (yref - Y(jj - 1)) * (yref - Y(jj)) / ((Y(jj + 1) - Y(jj - 1)) * (Y(jj + 1) - Y(jj)));
+        jjo[t - 1] = jj;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side operators (initialization.f90:164-463)
+// ---------------------------------------------------------------------------
+Host::Host(const cf_params &p, const double *x_, const double *y_, const double *z_) : prm(p) {
+    d.nx = p.nx; d.ny = p.ny; d.nz = p.nz;
+    d.nxg = p.nx + 1; d.nyg = p.ny + 1; d.nzg = p.nz + 1;
+    d.nxp = p.nx - 2; d.nzp = p.nz - 2; d.nyp = d.nyg - 2;
+    d.mh = d.nxp / 2 + 1;
+    x.assign(x_, x_ + p.nx);
+    y.assign(y_, y_ + p.ny);
+    z.assign(z_, z_ + p.nz);
+    const int nx = d.nx, ny = d.ny, nz = d.nz, nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
+    xg.assign(nxg, 0); yg.assign(nyg, 0); zg.assign(nzg, 0);
+    auto centers = [](const std::vector<double> &f, std::vector<double> &g) {
+        const int n = (int)f.size(), nm = n - 1;
+        std::vector<double> m(nm);
+        for (int i = 0; i < nm; i++) m[i] = 0.5 * (f[i] + f[i + 1]);
+        for (int i = 0; i < nm; i++) g[i + 1] = m[i];
+        g[0] = m[0] - 2.0 * (m[0] - f[0]);
+        g[nm + 1] = m[nm - 1] + 2.0 * (f[n - 1] - m[nm - 1]);
+    };
+    centers(x, xg);
+    centers(y, yg);
+    centers(z, zg);
+    auto dmin = [](const std::vector<double> &g) {
+        double m = 1e300;
+        for (size_t i = 1; i < g.size(); i++) m = std::min(m, g[i] - g[i - 1]);
+        return m;
+    };
+    dxmin = dmin(xg); dymin = dmin(yg); dzmin = dmin(zg);
+    // 1-based reciprocal metrics
+    rdxf.assign(nx + 2, 0); rdxc.assign(nxg + 2, 0); dxc.assign(nxg + 2, 0);
+    rdyf.assign(ny + 2, 0); rdyc.assign(nyg + 2, 0); dyc.assign(nyg + 2, 0);
+    rdzf.assign(nz + 2, 0); rdzc.assign(nzg + 2, 0); dzc.assign(nzg + 2, 0);
+    dyg.assign(nyg + 2, 0);
+    for (int i = 2; i <= nx; i++) rdxf[i] = 1.0 / (x[i - 1] - x[i - 2]);
+    for (int i = 1; i <= nxg - 1; i++) { dxc[i] = xg[i] - xg[i - 1]; rdxc[i] = 1.0 / dxc[i]; }
+    for (int j = 2; j <= ny; j++) rdyf[j] = 1.0 / (y[j - 1] - y[j - 2]);
+    for (int j = 1; j <= nyg - 1; j++) { dyc[j] = yg[j] - yg[j - 1]; rdyc[j] = 1.0 / dyc[j]; }
+    for (int k = 2; k <= nz; k++) rdzf[k] = 1.0 / (z[k - 1] - z[k - 2]);
+    for (int k = 1; k <= nzg - 1; k++) { dzc[k] = zg[k] - zg[k - 1]; rdzc[k] = 1.0 / dzc[k]; }
+    for (int j = 2; j <= nyg; j++) dyg[j] = yg[j - 1] - yg[j - 2];
+    x1.assign(nx + 1, 0); y1.assign(ny + 1, 0); z1.assign(nz + 1, 0);
+    xg1.assign(nxg + 1, 0); yg1.assign(nyg + 1, 0); zg1.assign(nzg + 1, 0);
+    for (int i = 1; i <= nx; i++) x1[i] = x[i - 1];
+    for (int j = 1; j <= ny; j++) y1[j] = y[j - 1];
+    for (int k = 1; k <= nz; k++) z1[k] = z[k - 1];
+    for (int i = 1; i <= nxg; i++) xg1[i] = xg[i - 1];
+    for (int j = 1; j <= nyg; j++) yg1[j] = yg[j - 1];
+    for (int k = 1; k <= nzg; k++) zg1[k] = zg[k - 1];
+    // initialization.f90:386-388
+    w0.assign(ny + 2, 0); w1.assign(ny + 2, 0);
+    for (int j = 1; j <= ny; j++) {
+        w0[j] = (yg[j] - y[j - 1]) / (yg[j] - yg[j - 1]);
+        w1[j] = 1.0 - w0[j];
+    }
+    // initialization.f90:281-296
+    const double pi = 4.0 * std::atan(1.0);
+    const int nxp = d.nxp, nzp = d.nzp;
+    kxx.assign(nxp, 0); kzz.assign(nzp, 0);
+    const int cx = (nxp + 1) / 2, cz = (nzp + 1) / 2;
+    for (int i = 0; i < nxp; i++) {
+        const double a = (i <= cx) ? (double)i : (double)(-nxp + i);
+        kxx[i] = 2.0 * (std::cos(2.0 * pi * a / (double)nxp) - 1.0) / (dxmin * dxmin);
+    }
+    for (int k = 0; k < nzp; k++) {
+        const double a = (k <= cz) ? (double)k : (double)(-nzp + k);
+        kzz[k] = 2.0 * (std::cos(2.0 * pi * a / (double)nzp) - 1.0) / (dzmin * dzmin);
+    }
+    // initialization.f90:343-373
+    const int nr = d.nyp;
+    dd.assign(nr, 0); dl.assign(std::max(nr - 1, 1), 0); du.assign(std::max(nr - 1, 1), 0);
+    auto Yf = [&](int j) { return y[j - 1]; };
+    auto YG = [&](int j) { return yg[j - 1]; };
+    for (int j = 2; j <= nyg - 1; j++) {
+        const double a = 1.0 / (Yf(j) - Yf(j - 1)) / (YG(j + 1) - YG(j));
+        const double b = 1.0 / (Yf(j) - Yf(j - 1)) * (-1.0 / (YG(j + 1) - YG(j)) - 1.0 / (YG(j) - YG(j - 1)));
+        const double c = 1.0 / (Yf(j) - Yf(j - 1)) / (YG(j) - YG(j - 1));
+        dd[j - 2] = b;
+        if (j < nyg - 1) du[j - 2] = a;
+        if (j > 2) dl[j - 3] = c;
+        if (j == 2) dd[0] = b + c;
+        if (j == nyg - 1) dd[j - 2] = a + b;
+    }
+    // mass_flow.f90:35-44 normalisation, accumulated in the reference's order
+    norm_u = 0.0;
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 3; j <= nyg - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) norm_u = norm_u + (1.0 + 1.0) * 0.5 * (YG(j) - YG(j - 1));
+    norm_v = 0.0;
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= ny; j++)
+            for (int i = 2; i <= nxg - 1; i++) norm_v = norm_v + (1.0 + 1.0) * 0.5 * (Yf(j) - Yf(j - 1));
+    // time_integration.f90:208-212
+    dt_vis = std::min(std::min(p.cfl * dxmin * dxmin / p.nu, p.cfl * dymin * dymin / p.nu),
+                      p.cfl * dzmin * dzmin / p.nu);
+    // subgrid.f90:600 filter width
+    delta.assign(nyg + 2, 0);
+    for (int j = 2; j <= nyg - 1; j++) delta[j] = std::pow(dxmin * dzmin * (Yf(j) - Yf(j - 1)), 1.0 / 3.0);
+    // wallmodel.f90:605-610 stencils
+    jjz.assign(nzg, 0); wz.assign(3 * nzg, 0); jjx.assign(nx, 0); wx.assign(3 * nx, 0);
+    interp_2nd(nz, z.data(), nzg, zg.data(), false, jjz.data(), wz.data());
+    interp_2nd(nxg, xg.data(), nx, x.data(), true, jjx.data(), wx.data());
+}
+
+std::string Host::validate() const {
+    const cf_params &p = prm;
+    if (p.nx < 6 || p.ny < 4 || p.nz < 6) return "grid too small";
+    if (p.nx % 2 || p.nz % 2) return "nx and nz must be even (initialization.f90:54-55)";
+    if ((int64_t)d.nxg * d.nyg * d.nzg > (int64_t)2000000000) return "grid too large";
+    if (d.nxp / 2 > 1024 || d.nzp > 2048) return "FFT length too large";
+    if (p.precision != 0 && p.precision != 1) return "precision must be 0 (fp64) or 1 (mixed)";
+    if (p.les_model != 0 && p.les_model != 1) return "les_model must be 0 or 1";
+    if (p.iwall_model != 0 && p.iwall_model != 3) return "iwall_model must be 0 or 3";
+    if (p.iwall_model == 3) {
+        if (p.jref_eqwm < 2 || p.jref_eqwm > d.nyg - 1) return "jref_eqwm out of range";
+        if (p.eqwm_ncv < 3 || p.eqwm_ncv > EQWM_MAXNCV) return "eqwm_ncv out of range";
+    }
+    if (p.nu <= 0) return "nu must be positive";
+    return "";
+}
+
+// ---------------------------------------------------------------------------
+// Device context
+// ---------------------------------------------------------------------------
+template <typename T>
+static int dalloc(T **p, size_t n, std::vector<void *> &owned, size_t &bytes) {
+    void *q = nullptr;
+    HCK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    owned.push_back(q);
+    bytes += std::max<size_t>(n, 1) * sizeof(T);
+    *p = (T *)q;
+    return 0;
+}
+template <typename T>
+static int dupload(T **p, const std::vector<T> &h, std::vector<void *> &owned, size_t &bytes) {
+    if (dalloc(p, h.size(), owned, bytes)) return -1;
+    HCK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename S>
+Ctx<S>::~Ctx() {
+    if (dev >= 0) hipSetDevice(dev);
+    for (auto &g : graphs) if (g) hipGraphExecDestroy(g);
+    for (auto &e : evpool) hipEventDestroy(e);
+    for (void *p : owned) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+}
+
+template <typename S>
+int Ctx<S>::init() {
+    const Dims &d = h.d;
+    dev = h.prm.device;
+    HCK(hipSetDevice(dev));
+    HCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const size_t nU = (size_t)d.nx * d.nyg * d.nzg, nV = (size_t)d.nxg * d.ny * d.nzg,
+                 nW = (size_t)d.nxg * d.nyg * d.nz, nC = (size_t)d.nxg * d.nyg * d.nzg;
+    n3[0] = nU; n3[1] = nV; n3[2] = nW;
+    for (int b = 0; b < 2; b++)
+        for (int c = 0; c < 3; c++)
+            if (dalloc(&st[b][c], n3[c], owned, bytes)) return -1;
+    for (int c = 0; c < 3; c++)
+        if (dalloc(&G[c], n3[c], owned, bytes)) return -1;
+    if (dalloc(&NT, nC, owned, bytes)) return -1;
+    HCK(hipMemset(NT, 0, nC * sizeof(S)));
+    for (int b = 0; b < 2; b++)
+        for (int c = 0; c < 3; c++) HCK(hipMemset(st[b][c], 0, n3[c] * sizeof(S)));
+    const size_t nspec = (size_t)d.mh * d.nzp * d.nyp;
+    if (dalloc(&spec, nspec, owned, bytes)) return -1;
+    if (dalloc(&cscr, nspec, owned, bytes)) return -1;
+    const size_t nphi = (size_t)d.nxp * d.nyp * d.nzp;
+    if (dalloc(&phi1, nphi, owned, bytes)) return -1;
+    if (dalloc(&phi23, nphi, owned, bytes)) return -1;
+    HCK(hipMemset(phi1, 0, nphi * sizeof(S)));
+    const int npart = std::max(3 * d.nzg, 2 * d.nzg) + 16;
+    if (dalloc(&part, npart, owned, bytes)) return -1;
+    if (dalloc(&sc, 1, owned, bytes)) return -1;
+    if (dalloc(&stats_out, 7 * d.ny + 8, owned, bytes)) return -1;
+    // metrics
+    if (dupload(&m_rdxf, h.rdxf, owned, bytes) || dupload(&m_rdxc, h.rdxc, owned, bytes) ||
+        dupload(&m_rdyf, h.rdyf, owned, bytes) || dupload(&m_rdyc, h.rdyc, owned, bytes) ||
+        dupload(&m_rdzf, h.rdzf, owned, bytes) || dupload(&m_rdzc, h.rdzc, owned, bytes) ||
+        dupload(&m_w0, h.w0, owned, bytes) || dupload(&m_w1, h.w1, owned, bytes) ||
+        dupload(&m_dyg, h.dyg, owned, bytes) || dupload(&m_dxc, h.dxc, owned, bytes) ||
+        dupload(&m_dyc, h.dyc, owned, bytes) || dupload(&m_dzc, h.dzc, owned, bytes) ||
+        dupload(&m_x, h.x1, owned, bytes) || dupload(&m_y, h.y1, owned, bytes) ||
+        dupload(&m_z, h.z1, owned, bytes) || dupload(&m_xg, h.xg1, owned, bytes) ||
+        dupload(&m_yg, h.yg1, owned, bytes) || dupload(&m_zg, h.zg1, owned, bytes) ||
+        dupload(&m_kxx, h.kxx, owned, bytes) || dupload(&m_kzz, h.kzz, owned, bytes) ||
+        dupload(&m_dd, h.dd, owned, bytes) || dupload(&m_dl, h.dl, owned, bytes) ||
+        dupload(&m_du, h.du, owned, bytes) || dupload(&m_delta, h.delta, owned, bytes) ||
+        dupload(&m_jjz, h.jjz, owned, bytes) || dupload(&m_wz, h.wz, owned, bytes) ||
+        dupload(&m_jjx, h.jjx, owned, bytes) || dupload(&m_wx, h.wx, owned, bytes))
+        return -1;
+    met.rdxf = m_rdxf; met.rdxc = m_rdxc; met.rdyf = m_rdyf; met.rdyc = m_rdyc;
+    met.rdzf = m_rdzf; met.rdzc = m_rdzc; met.w0 = m_w0; met.w1 = m_w1; met.dyg = m_dyg;
+    wt.jjz = m_jjz; wt.wz = m_wz; wt.jjx = m_jjx; wt.wx = m_wx;
+    // FFT plans and twiddles: x uses a length-M complex FFT plus the N-point split
+    std::string err;
+    const int M = d.nxp / 2;
+    pM = make_plan(M, err);
+    pZ = make_plan(d.nzp, err);
+    if (!err.empty()) { set_error(err); return -1; }
+    auto twid = [](int n) {
+        std::vector<Cx<S>> t(n + 1);
+        for (int q = 0; q <= n; q++) {
+            long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)n;
+            t[q] = Cx<S>{(S)std::cos(a), (S)(-std::sin(a))};
+        }
+        return t;
+    };
+    if (dupload(&twM, twid(M), owned, bytes) || dupload(&twN, twid(d.nxp), owned, bytes) ||
+        dupload(&twZ, twid(d.nzp), owned, bytes))
+        return -1;
+    // wall model buffers
+    const size_t nwx = (size_t)d.nx * 2 * d.nzg, nwz = (size_t)d.nxg * 2 * d.nz;
+    if (dalloc(&alpha_x, nwx, owned, bytes) || dalloc(&alpha_z, nwz, owned, bytes) ||
+        dalloc(&azt, nwx, owned, bytes) || dalloc(&tau_init, nwx, owned, bytes))
+        return -1;
+    HCK(hipMemset(alpha_x, 0, nwx * sizeof(double)));
+    HCK(hipMemset(alpha_z, 0, nwz * sizeof(double)));
+    HCK(hipMemset(azt, 0, nwx * sizeof(double)));
+    {
+        std::vector<double> ti(nwx, h.prm.tau_mean_eqwm);
+        HCK(hipMemcpy(tau_init, ti.data(), nwx * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // scalars
+    Scalars s0{};
+    s0.dt = h.prm.cfl < 0 ? -h.prm.cfl : 0.0;
+    s0.dPdx = h.prm.dPdx;
+    s0.dPdz = h.prm.dPdz;
+    s0.norm_u = h.norm_u;
+    s0.norm_v = h.norm_v;
+    HCK(hipMemcpy(sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice));
+    HCK(hipDeviceSynchronize());
+    return 0;
+}
+
+// ---- timing helpers ----
+template <typename S>
+void Ctx<S>::tic(const char *name) {
+    if (!timing) return;
+    hipEvent_t a = ev_get(), b = ev_get();
+    hipEventRecord(a, stream);
+    pending.push_back({name, a, b});
+}
+template <typename S>
+void Ctx<S>::toc() {
+    if (!timing || pending.empty()) return;
+    hipEventRecord(pending.back().b, stream);
+    done_ev.push_back(pending.back());
+    pending.pop_back();
+}
+template <typename S>
+hipEvent_t Ctx<S>::ev_get() {
+    if (ev_free.empty()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        evpool.push_back(e);
+        return e;
+    }
+    hipEvent_t e = ev_free.back();
+    ev_free.pop_back();
+    return e;
+}
+template <typename S>
+void Ctx<S>::collect() {
+    if (done_ev.empty()) return;
+    hipStreamSynchronize(stream);
+    for (auto &r : done_ev) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        auto &acc = tstat[r.name];
+        acc.first += ms;
+        acc.second += 1;
+        ev_free.push_back(r.a);
+        ev_free.push_back(r.b);
+    }
+    done_ev.clear();
+}
+
+// ---- building blocks ----
+template <typename S>
+void Ctx<S>::ghosts(S *const *F, int mask) {
+    const Dims &d = h.d;
+    const bool wm = h.prm.iwall_model == 3;
+    const double dgb = h.yg[1] - h.yg[0], dgt = h.yg[d.nyg - 1] - h.yg[d.nyg - 2];
+    tic("ghosts");
+    if (mask & 1) launch_ghosts<S>(d, F[0], 0, wm ? 1 : 0, alpha_x, dgb, dgt, stream);
+    if (mask & 2) launch_ghosts<S>(d, F[1], 1, 0, nullptr, dgb, dgt, stream);
+    if (mask & 4) launch_ghosts<S>(d, F[2], 2, wm ? 1 : 0, alpha_z, dgb, dgt, stream);
+    toc();
+}
+
+template <typename S>
+void Ctx<S>::poisson(S *const *F, S *phi) {
+    const Dims &d = h.d;
+    tic("div_fftx");
+    launch_div_fftx<S, S>(d, met, F[0], F[1], F[2], spec, pM, twM, twN, stream);
+    toc();
+    tic("fftz_fwd");
+    launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
+    toc();
+    tic("thomas");
+    launch_thomas<S>(d, spec, cscr, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
+    toc();
+    tic("fftz_inv");
+    launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
+    toc();
+    tic("ifftx_proju");
+    launch_ifftx_proju<S, S, S>(d, met, spec, phi, F[0], pM, twM, twN, (double)(d.nxp * d.nzp), stream);
+    toc();
+    tic("project_vw");
+    launch_project_vw<S, S>(d, met, phi, F[1], F[2], stream);
+    toc();
+}
+
+template <typename S>
+void Ctx<S>::eddy_viscosity(S *const *F) {
+    const Dims &d = h.d;
+    if (h.prm.les_model != 1) return;  // nu_t stays 0 (subgrid.f90:41-45)
+    tic("smagorinsky");
+    launch_smagorinsky<S>(d, m_x, m_y, m_z, m_xg, m_yg, m_zg, m_delta, F[0], F[1], F[2], NT, sc,
+                          h.prm.cs, h.prm.nu, h.prm.dirichlet_nu_t, stream);
+    toc();
+    tic("ghosts_nut");
+    launch_ghosts<S>(d, NT, 3, h.prm.dirichlet_nu_t == 1 ? 0 : 2, nullptr, 0, 0, stream);
+    toc();
+}
+
+template <typename S>
+void Ctx<S>::wall_model(S *const *F) {
+    const Dims &d = h.d;
+    if (h.prm.iwall_model != 3) return;
+    EqwmParams ep{h.prm.eqwm_ncv, h.prm.eqwm_max_iter, h.prm.eqwm_stretch, h.prm.eqwm_tol, 1.0};
+    const double hw = h.yg[h.prm.jref_eqwm - 1];
+    const double *pred = nullptr;
+    const int *use = nullptr;
+    if (surrogate) surrogate(this, F, &pred, &use);
+    tic("reset_counters");
+    launch_reset_counters(sc, stream);
+    toc();
+    tic("wall_model");
+    launch_wall_model<S>(d, F[0], F[2], h.prm.les_model == 1 ? NT : nullptr, wt, tau_init, alpha_x, azt,
+                         alpha_z, sc, ep, h.prm.nu, hw, h.prm.jref_eqwm, h.prm.tau_mean_eqwm, pred, use,
+                         stream);
+    toc();
+}
+
+template <typename S>
+void Ctx<S>::dt_launch() {
+    const Dims &d = h.d;
+    tic("dt");
+    if (h.prm.cfl < 0) launch_dt_final_only(sc, h.prm.cfl, stream);
+    else
+        launch_dt<S>(d, m_dxc, m_dyc, m_dzc, st[cur][0], st[cur][1], st[cur][2], part, sc, h.prm.cfl,
+                     h.dt_vis, stream);
+    toc();
+}
+
+template <typename S>
+void Ctx<S>::step_launch() {
+    const Dims &d = h.d;
+    const double rk[3][3] = {{8.0 / 15.0, 0, 0}, {1.0 / 4.0, 5.0 / 12.0, 0}, {1.0 / 4.0, 0.0, 3.0 / 4.0}};
+    const bool nut = h.prm.les_model == 1 || nut_user;
+    for (int s = 1; s <= 3; s++) {
+        S *const *Fc = st[cur];
+        S *const *Fn = st[1 - cur];
+        eddy_viscosity(Fc);
+        if (s == 1) wall_model(Fc);
+        tic("rhs_rk");
+        if (prec_mixed())
+            launch_rhs_rk<S, S>(d, met, Fc[0], Fc[1], Fc[2], nut ? NT : nullptr, Fn[0], Fn[1], Fn[2], G[0],
+                                G[1], G[2], sc, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
+                                s == 1 ? 1 : 2, stream);
+        else
+            launch_rhs_rk<S, S>(d, met, Fc[0], Fc[1], Fc[2], nut ? NT : nullptr, Fn[0], Fn[1], Fn[2], G[0],
+                                G[1], G[2], sc, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
+                                s == 1 ? 1 : 2, stream);
+        toc();
+        ghosts(Fn, 7);
+        poisson(Fn, s == 1 ? phi1 : phi23);
+        ghosts(Fn, 7);
+        cur = 1 - cur;
+    }
+    S *const *F = st[cur];
+    if (h.prm.x_mass_cte == 1) {
+        tic("massflow");
+        launch_massflow<S>(d, met, F[0], part, sc, 1, stream);
+        launch_add_corr_u<S>(d, F[0], sc, stream);
+        toc();
+        ghosts(F, 1);
+    }
+    tic("advance_time");
+    launch_advance_time(sc, stream);
+    toc();
+}
+
+template <typename S>
+int Ctx<S>::run_steps(int nsteps, bool use_graph) {
+    HCK(hipSetDevice(dev));
+    for (int n = 0; n < nsteps; n++) {
+        if (use_graph && !timing) {
+            hipGraphExec_t &ge = graphs[cur];
+            const int start = cur;
+            if (!ge) {
+                hipGraph_t g;
+                HCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                dt_launch();
+                step_launch();
+                HCK(hipStreamEndCapture(stream, &g));
+                HCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                HCK(hipGraphDestroy(g));
+                cur = start;  // capture does not execute; replay advances the state
+            }
+            HCK(hipGraphLaunch(ge, stream));
+            cur = 1 - start;  // three substeps flip the ping-pong parity
+        } else {
+            dt_launch();
+            step_launch();
+        }
+    }
+    HCK(hipGetLastError());
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::rk3_step() {
+    HCK(hipSetDevice(dev));
+    step_launch();
+    HCK(hipGetLastError());
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::compute_dt(double *dt) {
+    HCK(hipSetDevice(dev));
+    dt_launch();
+    HCK(hipGetLastError());
+    Scalars s;
+    HCK(hipMemcpyAsync(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
+    HCK(hipStreamSynchronize(stream));
+    if (dt) *dt = s.dt;
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::projection_step(int rk_step) {
+    HCK(hipSetDevice(dev));
+    poisson(st[cur], rk_step == 1 ? phi1 : phi23);
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::apply_bc() {
+    HCK(hipSetDevice(dev));
+    ghosts(st[cur], 7);
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// host double <-> device S
+template <typename S>
+int Ctx<S>::put(S *dst, const double *src, size_t n) {
+    if (std::is_same<S, double>::value) {
+        HCK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<S> t(n);
+        for (size_t q = 0; q < n; q++) t[q] = (S)src[q];
+        HCK(hipMemcpy(dst, t.data(), n * sizeof(S), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+template <typename S>
+int Ctx<S>::get(double *dst, const S *src, size_t n) {
+    HCK(hipStreamSynchronize(stream));
+    if (std::is_same<S, double>::value) {
+        HCK(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<S> t(n);
+        HCK(hipMemcpy(t.data(), src, n * sizeof(S), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < n; q++) dst[q] = (double)t[q];
+    }
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::set_state(const double *U, const double *V, const double *W) {
+    HCK(hipSetDevice(dev));
+    HCK(hipStreamSynchronize(stream));
+    if (put(st[cur][0], U, n3[0]) || put(st[cur][1], V, n3[1]) || put(st[cur][2], W, n3[2])) return -1;
+    return 0;
+}
+template <typename S>
+int Ctx<S>::get_state(double *U, double *V, double *W) {
+    HCK(hipSetDevice(dev));
+    if ((U && get(U, st[cur][0], n3[0])) || (V && get(V, st[cur][1], n3[1])) ||
+        (W && get(W, st[cur][2], n3[2])))
+        return -1;
+    return 0;
+}
+
+// projection.f90:129-137: P = phi/(dt*rk_coef(1,1)) with ghost copies
+template <typename S>
+int Ctx<S>::get_pressure(double *P) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    std::vector<double> ph((size_t)d.nxp * d.nyp * d.nzp);
+    if (get(ph.data(), phi1, ph.size())) return -1;
+    Scalars s;
+    HCK(hipMemcpy(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    const double den = s.dt * (8.0 / 15.0);
+    const int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
+    auto idx = [&](int i, int j, int k) { return i3(nxg, nyg, i, j, k); };
+    auto phv = [&](int i, int j, int k) {  // interior incl. duplicate planes
+        int ii = i == nxg - 1 ? 2 : i, kk = k == nzg - 1 ? 2 : k;
+        return ph[((size_t)(kk - 2) * d.nyp + (j - 2)) * d.nxp + (ii - 2)];
+    };
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++) P[idx(i, j, k)] = phv(i, j, k) / den;
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) { P[idx(1, j, k)] = P[idx(2, j, k)]; P[idx(nxg, j, k)] = P[idx(nxg - 1, j, k)]; }
+    for (int k = 1; k <= nzg; k++)
+        for (int i = 1; i <= nxg; i++) { P[idx(i, 1, k)] = P[idx(i, 2, k)]; P[idx(i, nyg, k)] = P[idx(i, nyg - 1, k)]; }
+    const size_t pl = (size_t)nxg * nyg;
+    std::memcpy(P, P + pl, pl * sizeof(double));
+    std::memcpy(P + pl * (nzg - 1), P + pl * (nzg - 2), pl * sizeof(double));
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::set_nu_t(const double *nt) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    const size_t n = (size_t)d.nxg * d.nyg * d.nzg;
+    HCK(hipStreamSynchronize(stream));
+    bool nz = false;
+    for (size_t q = 0; q < n && !nz; q++) nz = nt[q] != 0.0;
+    if (nz != nut_user) invalidate_graphs();
+    nut_user = nz;
+    return put(NT, nt, n);
+}
+template <typename S>
+int Ctx<S>::get_nu_t(double *nt) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    return get(nt, NT, (size_t)d.nxg * d.nyg * d.nzg);
+}
+template <typename S>
+int Ctx<S>::get_wall(double *ax, double *az, double *ti) {
+    HCK(hipSetDevice(dev));
+    HCK(hipStreamSynchronize(stream));
+    const Dims &d = h.d;
+    const size_t nwx = (size_t)d.nx * 2 * d.nzg, nwz = (size_t)d.nxg * 2 * d.nz;
+    if (ax) HCK(hipMemcpy(ax, alpha_x, nwx * sizeof(double), hipMemcpyDeviceToHost));
+    if (az) HCK(hipMemcpy(az, alpha_z, nwz * sizeof(double), hipMemcpyDeviceToHost));
+    if (ti) HCK(hipMemcpy(ti, tau_init, nwx * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+template <typename S>
+int Ctx<S>::set_tau_init(const double *ti) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    HCK(hipStreamSynchronize(stream));
+    HCK(hipMemcpy(tau_init, ti, (size_t)d.nx * 2 * d.nzg * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::set_scalars(const double *v) {
+    HCK(hipSetDevice(dev));
+    HCK(hipStreamSynchronize(stream));
+    Scalars s;
+    HCK(hipMemcpy(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    s.t = v[0]; s.dt = v[1]; s.dPdx = v[2]; s.dPdz = v[3]; s.qflow_x0 = v[4]; s.dPdy = v[5]; s.qflow_y0 = v[6];
+    HCK(hipMemcpy(sc, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+    return 0;
+}
+template <typename S>
+int Ctx<S>::get_scalars(double *v) {
+    HCK(hipSetDevice(dev));
+    HCK(hipStreamSynchronize(stream));
+    Scalars s;
+    HCK(hipMemcpy(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    v[0] = s.t; v[1] = s.dt; v[2] = s.dPdx; v[3] = s.dPdz; v[4] = s.qflow_x0; v[5] = s.dPdy; v[6] = s.qflow_y0;
+    return 0;
+}
+template <typename S>
+int Ctx<S>::get_raw_scalars(Scalars *out) {
+    HCK(hipSetDevice(dev));
+    HCK(hipStreamSynchronize(stream));
+    HCK(hipMemcpy(out, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::mean_mass_flow_u(double *q) {
+    HCK(hipSetDevice(dev));
+    launch_massflow<S>(h.d, met, st[cur][0], part, sc, 0, stream);
+    HCK(hipGetLastError());
+    Scalars s;
+    if (get_raw_scalars(&s)) return -1;
+    *q = s.pad[0];
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::init_mass_flow() {
+    double q;
+    if (mean_mass_flow_u(&q)) return -1;
+    Scalars s;
+    if (get_raw_scalars(&s)) return -1;
+    s.qflow_x0 = q;
+    HCK(hipMemcpy(sc, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::check_divergence(double *ms, double *ma) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    launch_div<S>(d, m_x, m_y, m_z, st[cur][0], st[cur][1], st[cur][2], part, stream);
+    HCK(hipGetLastError());
+    std::vector<double> hp(2 * (d.nzg - 2));
+    HCK(hipStreamSynchronize(stream));
+    HCK(hipMemcpy(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < d.nzg - 2; k++) { a = std::max(a, hp[2 * k]); b = std::max(b, hp[2 * k + 1]); }
+    if (ms) *ms = a;
+    if (ma) *ma = b;
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::statistics(double *out) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    launch_stats<S>(d, met, st[cur][0], st[cur][1], st[cur][2], stats_out, stream);
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(stream));
+    HCK(hipMemcpy(out, stats_out, 7 * d.ny * sizeof(double), hipMemcpyDeviceToHost));
+    // statistics.f90:107-115 normalisations
+    const double cu = (double)((d.nx - 2) * (d.nzg - 2)), cv = (double)((d.nxg - 2) * (d.nzg - 2)),
+                 cw = (double)((d.nxg - 2) * (d.nz - 2));
+    const double den[7] = {cu, cv, cw, cu, cv, cw, cv};
+    for (int c = 0; c < 7; c++)
+        for (int j = 0; j < d.ny; j++) out[c * d.ny + j] /= den[c];
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::compute_rhs(const double *U, const double *V, const double *W, double *ru, double *rv,
+                        double *rw) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    // stage the inputs in the idle ping-pong buffers
+    S *const *T = st[1 - cur];
+    if (put(T[0], U, n3[0]) || put(T[1], V, n3[1]) || put(T[2], W, n3[2])) return -1;
+    const size_t nru = (size_t)(d.nx - 2) * (d.nyg - 2) * (d.nzg - 2);
+    const size_t nrv = (size_t)(d.nxg - 2) * (d.ny - 2) * (d.nzg - 2);
+    const size_t nrw = (size_t)(d.nxg - 2) * (d.nyg - 2) * (d.nz - 2);
+    double *dr = nullptr;
+    HCK(hipMalloc(&dr, (nru + nrv + nrw) * sizeof(double)));
+    const bool nut = h.prm.les_model == 1 || nut_user;
+    launch_rhs_only<S, S>(d, met, T[0], T[1], T[2], nut ? NT : nullptr, dr, dr + nru, dr + nru + nrv, sc,
+                          h.prm.nu, h.prm.omega_z, stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && ru) e = hipMemcpy(ru, dr, nru * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && rv) e = hipMemcpy(rv, dr + nru, nrv * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && rw) e = hipMemcpy(rw, dr + nru + nrv, nrw * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(dr);
+    if (e != hipSuccess) { set_error(std::string("compute_rhs: ") + hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::compute_eddy_viscosity() {
+    HCK(hipSetDevice(dev));
+    if (h.prm.les_model != 1) {
+        HCK(hipMemsetAsync(NT, 0, (size_t)h.d.nxg * h.d.nyg * h.d.nzg * sizeof(S), stream));
+        if (nut_user) invalidate_graphs();
+        nut_user = false;
+    }
+    eddy_viscosity(st[cur]);
+    HCK(hipGetLastError());
+    HCK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::compute_wall_model(int *n_nan, int *n_iter) {
+    HCK(hipSetDevice(dev));
+    wall_model(st[cur]);
+    HCK(hipGetLastError());
+    Scalars s;
+    if (get_raw_scalars(&s)) return -1;
+    if (n_nan) *n_nan = s.n_nan_eqwm;
+    if (n_iter) *n_iter = s.n_iter_eqwm;
+    return 0;
+}
+
+template <typename S>
+std::string Ctx<S>::timing_report() {
+    collect();
+    std::string out;
+    char line[256];
+    for (auto &kv : tstat) {
+        std::snprintf(line, sizeof line, "%s %.6f %ld\n", kv.first.c_str(),
+                      kv.second.second ? kv.second.first / kv.second.second : 0.0, (long)kv.second.second);
+        out += line;
+    }
+    return out;
+}
+
+template struct Ctx<double>;
+template struct Ctx<float>;
+
+}  // namespace cf
+
+// ===========================================================================
+// extern "C" surface
+// ===========================================================================
+using cf::g_err;
+using cf::set_error;
+
+struct cf_ctx {
+    std::unique_ptr<cf::CtxBase> impl;
+};
+
+#define CTX_CHECK(c)                                           \
+    do {                                                       \
+        if (!(c) || !(c)->impl) {                              \
+            set_error("null context");                         \
+            return -1;                                         \
+        }                                                      \
+    } while (0)
+
+extern "C" {
+
+const char *cf_last_error(void) { return g_err.c_str(); }
+
+int cf_create(const cf_params *p, const double *x, const double *y, const double *z, cf_ctx **out) {
+    if (!p || !x || !y || !z || !out) { set_error("cf_create: null argument"); return -1; }
+    *out = nullptr;
+    cf::Host h(*p, x, y, z);
+    std::string v = h.validate();
+    if (!v.empty()) { set_error("cf_create: " + v); return -1; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("cf_create: no HIP device available");
+        return -1;
+    }
+    if (p->device < 0 || p->device >= ndev) { set_error("cf_create: bad device ordinal"); return -1; }
+    std::unique_ptr<cf::CtxBase> impl;
+    if (p->precision == 0) impl.reset(new cf::Ctx<double>(h));
+    else impl.reset(new cf::Ctx<float>(h));
+    if (impl->init()) return -1;
+    cf_ctx *c = new cf_ctx;
+    c->impl = std::move(impl);
+    *out = c;
+    return 0;
+}
+
+int cf_destroy(cf_ctx *c) {
+    if (!c) return 0;
+    delete c;
+    return 0;
+}
+
+int cf_set_state(cf_ctx *c, const double *U, const double *V, const double *W) {
+    CTX_CHECK(c);
+    return c->impl->set_state(U, V, W);
+}
+int cf_get_state(cf_ctx *c, double *U, double *V, double *W) {
+    CTX_CHECK(c);
+    return c->impl->get_state(U, V, W);
+}
+int cf_get_pressure(cf_ctx *c, double *P) { CTX_CHECK(c); return c->impl->get_pressure(P); }
+int cf_set_nu_t(cf_ctx *c, const double *nt) { CTX_CHECK(c); return c->impl->set_nu_t(nt); }
+int cf_get_nu_t(cf_ctx *c, double *nt) { CTX_CHECK(c); return c->impl->get_nu_t(nt); }
+int cf_get_wall(cf_ctx *c, double *ax, double *az, double *ti) { CTX_CHECK(c); return c->impl->get_wall(ax, az, ti); }
+int cf_set_tau_init(cf_ctx *c, const double *ti) { CTX_CHECK(c); return c->impl->set_tau_init(ti); }
+int cf_set_scalars(cf_ctx *c, const double *s) { CTX_CHECK(c); return c->impl->set_scalars(s); }
+int cf_get_scalars(cf_ctx *c, double *s) { CTX_CHECK(c); return c->impl->get_scalars(s); }
+int cf_init_mass_flow(cf_ctx *c) { CTX_CHECK(c); return c->impl->init_mass_flow(); }
+int cf_apply_boundary_conditions(cf_ctx *c) { CTX_CHECK(c); return c->impl->apply_bc(); }
+int cf_compute_dt(cf_ctx *c, double *dt) { CTX_CHECK(c); return c->impl->compute_dt(dt); }
+int cf_rk3_step(cf_ctx *c) {
+    CTX_CHECK(c);
+    if (c->impl->rk3_step()) return -1;
+    return c->impl->sync();
+}
+int cf_run_steps(cf_ctx *c, int nsteps, int use_graph) {
+    CTX_CHECK(c);
+    if (nsteps < 0) { set_error("cf_run_steps: negative step count"); return -1; }
+    return c->impl->run_steps(nsteps, use_graph != 0);
+}
+int cf_projection_step(cf_ctx *c, int rk_step) { CTX_CHECK(c); return c->impl->projection_step(rk_step); }
+int cf_compute_rhs(cf_ctx *c, const double *U, const double *V, const double *W, double *ru, double *rv,
+                   double *rw) {
+    CTX_CHECK(c);
+    if (!U || !V || !W) { set_error("cf_compute_rhs: null input"); return -1; }
+    return c->impl->compute_rhs(U, V, W, ru, rv, rw);
+}
+int cf_compute_eddy_viscosity(cf_ctx *c) { CTX_CHECK(c); return c->impl->compute_eddy_viscosity(); }
+int cf_compute_wall_model(cf_ctx *c, int *n_nan, int *n_iter) {
+    CTX_CHECK(c);
+    return c->impl->compute_wall_model(n_nan, n_iter);
+}
+int cf_mean_mass_flow_u(cf_ctx *c, double *q) { CTX_CHECK(c); return c->impl->mean_mass_flow_u(q); }
+int cf_check_divergence(cf_ctx *c, double *ms, double *ma) { CTX_CHECK(c); return c->impl->check_divergence(ms, ma); }
+int cf_statistics(cf_ctx *c, double *out) { CTX_CHECK(c); return c->impl->statistics(out); }
+
+int cf_eqwm_batch(int n, const double *umag, const double *nu, const double *h, const double *tau_in,
+                  int ncv, double stretch, double tol, int max_iter, double *tau, int *iter) {
+    if (n < 0 || (n > 0 && (!umag || !nu || !h || !tau_in || !tau || !iter))) {
+        set_error("cf_eqwm_batch: bad arguments");
+        return -1;
+    }
+    if (ncv < 3 || ncv > cf::EQWM_MAXNCV) { set_error("cf_eqwm_batch: ncv out of range [3,128]"); return -1; }
+    if (n == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("cf_eqwm_batch: no HIP device"); return -1; }
+    double *d = nullptr;
+    int *di = nullptr;
+    const size_t b = (size_t)n * sizeof(double);
+    HCK(hipMalloc(&d, 5 * b));
+    if (hipMalloc(&di, (size_t)n * sizeof(int)) != hipSuccess) { hipFree(d); set_error("cf_eqwm_batch: alloc"); return -1; }
+    hipError_t e = hipMemcpy(d, umag, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, nu, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 2 * (size_t)n, h, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 3 * (size_t)n, tau_in, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        cf::EqwmParams p{ncv, max_iter, stretch, tol, 1.0};
+        cf::launch_eqwm_batch(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, d + 4 * (size_t)n, di, p, 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(tau, d + 4 * (size_t)n, b, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(iter, di, (size_t)n * sizeof(int), hipMemcpyDeviceToHost);
+    hipFree(d);
+    hipFree(di);
+    if (e != hipSuccess) { set_error(std::string("cf_eqwm_batch: ") + hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+int cf_timing_enable(cf_ctx *c, int on) { CTX_CHECK(c); c->impl->set_timing(on != 0); return 0; }
+int cf_timing_reset(cf_ctx *c) { CTX_CHECK(c); c->impl->timing_reset(); return 0; }
+int cf_timing_report(cf_ctx *c, char *buf, int len) {
+    CTX_CHECK(c);
+    std::string r = c->impl->timing_report();
+    if (!buf || len <= 0) return (int)r.size() + 1;
+    std::snprintf(buf, (size_t)len, "%s", r.c_str());
+    return 0;
+}
+long long cf_device_bytes(cf_ctx *c) { return c && c->impl ? (long long)c->impl->device_bytes() : -1; }
+int cf_sync(cf_ctx *c) { CTX_CHECK(c); return c->impl->sync(); }
+
+}  // extern "C"

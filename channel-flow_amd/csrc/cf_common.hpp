@@ -1,0 +1,84 @@
+// Shared definitions for the channel-flow HIP kernels (gfx950).
+//
+// Layout in HBM: every velocity-like array keeps the reference's Fortran
+// layout, x fastest, then y, then z, with the reference's ghost extents
+// (global.f90:82-84, initialization.f90:121-124):
+//   U (nx ,nyg,nzg)   V (nxg,ny ,nzg)   W (nxg,nyg,nz )   nu_t (nxg,nyg,nzg)
+// so host mirrors and device buffers share one indexing scheme.  Kernels
+// use 1-based (i,j,k) like the reference.  1-D metric arrays are stored
+// 1-based too (index 0 unused).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cf {
+
+struct Dims {
+    int nx, ny, nz;       // faces incl. periodic duplicates (x, z) and walls (y)
+    int nxg, nyg, nzg;    // centres + ghosts
+    int nxp, nzp, nyp;    // unique periodic points in x, z; interior centres in y
+    int mh;               // x half spectrum length nxp/2+1
+};
+
+// 1-D grid metrics, all 1-based (index 0 unused), double precision.
+struct Metrics {
+    const double *rdxf;   // 1/(x(i)-x(i-1))      i = 2..nx
+    const double *rdxc;   // 1/(xg(i+1)-xg(i))    i = 1..nxg-1
+    const double *rdyf;   // 1/(y(j)-y(j-1))      j = 2..ny
+    const double *rdyc;   // 1/(yg(j+1)-yg(j))    j = 1..nyg-1
+    const double *rdzf;   // 1/(z(k)-z(k-1))      k = 2..nz
+    const double *rdzc;   // 1/(zg(k+1)-zg(k))    k = 1..nzg-1
+    const double *w0, *w1;  // y-face weights (initialization.f90:387-388)
+    const double *dyg;    // yg(j)-yg(j-1)        j = 2..nyg
+};
+
+// Device-resident scalars shared by all kernels of a step (no host sync).
+struct Scalars {
+    double dt;        // current time step
+    double dPdx;      // forcing carried between steps (time_integration.f90:156)
+    double dPdz;
+    double dPdy;
+    double t;
+    double qflow_x0;  // target bulk flow (initialization.f90:460)
+    double qflow_y0;
+    double corr;      // last mass-flow correction (U increment)
+    double norm_u;    // trapezoid normalisation (mass_flow.f90:41), host-computed
+    double norm_v;
+    double dt_conv[3];
+    double pad[4];
+    int n_nan_eqwm;
+    int n_iter_eqwm;
+    int n_fallback_mlp;
+    int pad_i;
+};
+
+__host__ __device__ inline int64_t i3(int n1, int n2, int i, int j, int k) {
+    return (int64_t)(i - 1) + (int64_t)n1 * ((int64_t)(j - 1) + (int64_t)n2 * (int64_t)(k - 1));
+}
+
+template <typename T> struct Cx { T x, y; };
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+
+// Launch geometry helper: blocks are remapped so that consecutive tiles land
+// on the same XCD (workgroups are dispatched round-robin over 8 XCDs), which
+// keeps stencil neighbours inside one XCD's L2.
+// Bijective: XCD x receives blocks x, x+8, x+16, ... and owns the contiguous
+// tile range [x*q + min(x,r), ...) of length q + (x<r), q = n/8, r = n%8.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    const int nxcd = 8;
+    int q = nblocks / nxcd, r = nblocks % nxcd;
+    int xcd = bid % nxcd;
+    int slot = bid / nxcd;
+    return xcd * q + (xcd < r ? xcd : r) + slot;
+}
+
+}  // namespace cf

@@ -1,0 +1,81 @@
+// Kernel launcher declarations shared by the HIP translation units.
+#pragma once
+
+#include "cf_common.hpp"
+
+namespace cf {
+
+// Factorisation of one FFT length into radices 4, 2, 3 (applied in order).
+struct FFTPlan {
+    int n;
+    int nr;
+    int radix[16];
+};
+
+// ---- stencils (cf_stencil.hip) ----
+template <typename S, typename R>
+void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
+                   const S *NT, S *Uo, S *Vo, S *Wo, S *Gu, S *Gv, S *Gw,
+                   const Scalars *sc, double nu, double om, double a_new, double a_g,
+                   int sub, hipStream_t st);
+template <typename S, typename R>
+void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
+                     const S *NT, double *Ru, double *Rv, double *Rw, const Scalars *sc,
+                     double nu, double om, hipStream_t st);
+template <typename S>
+void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *alpha,
+                   double dyg_bot, double dyg_top, hipStream_t st);
+template <typename S, typename P>
+void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st);
+template <typename S>
+void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, Scalars *sc,
+                     int apply, hipStream_t st);
+template <typename S>
+void launch_add_corr_u(const Dims &d, S *U, const Scalars *sc, hipStream_t st);
+template <typename S>
+void launch_dt(const Dims &d, const double *dxc, const double *dyc, const double *dzc,
+               const S *U, const S *V, const S *W, double *part, Scalars *sc, double cfl,
+               double dt_vis, hipStream_t st);
+template <typename S>
+void launch_div(const Dims &d, const double *x, const double *y, const double *z, const S *U,
+                const S *V, const S *W, double *part, hipStream_t st);
+template <typename S>
+void launch_smagorinsky(const Dims &d, const double *x, const double *y, const double *z,
+                        const double *xg, const double *yg, const double *zg, const double *delta,
+                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc, double cs,
+                        double nu, int damping, hipStream_t st);
+template <typename S>
+void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W, double *out,
+                  hipStream_t st);
+
+// ---- Poisson chain (cf_fft.hip) ----
+int fft_rows_per_wg(int M, int elems_per_row, int esize);
+int fft_cols_per_wg(int n, int esize);
+template <typename S, typename T>
+void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
+                     Cx<T> *spec, const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN,
+                     hipStream_t st);
+template <typename T>
+void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ, int sign,
+                 hipStream_t st);
+template <typename T>
+void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
+                   const double *dd, const double *dl, const double *du, hipStream_t st);
+template <typename S, typename T, typename P>
+void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
+                        const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,
+                        hipStream_t st);
+
+// ---- wall model (cf_eqwm.hip) ----
+struct EqwmParams {
+    int ncv;
+    int max_iter;
+    double stretch;
+    double tol;
+    double rho;
+};
+void launch_eqwm_batch(int n, const double *u, const double *nu, const double *h,
+                       const double *tau_in, double *tau, int *iter, EqwmParams p,
+                       hipStream_t st);
+
+}  // namespace cf

@@ -1,0 +1,125 @@
+/*
+ * channel_flow.h -- C-ABI of the MI355X (gfx950) time-step library for the
+ * channel-flow solver.
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success and
+ * a nonzero code on failure; cf_last_error() then returns the message.  The
+ * Fortran bindings (channel-flow_amd/fortran/) turn a nonzero code into
+ * `Stop 'message'`, which is the reference's error convention.
+ *
+ * Arrays exchanged with the host use the reference layout exactly
+ * (global.f90:82-84; Fortran order, x fastest, ghosts included):
+ *   U (nx ,nyg,nzg)   V (nxg,ny ,nzg)   W (nxg,nyg,nz )   P, nu_t (nxg,nyg,nzg)
+ *   rhs_u (2:nx-1,2:nyg-1,2:nzg-1)  rhs_v (2:nxg-1,2:ny-1,2:nzg-1)
+ *   rhs_w (2:nxg-1,2:nyg-1,2:nz-1)
+ *   alpha_x (nx,2,nzg)  alpha_z (nxg,2,nz)  tau_eqwm_initial (nx,2,nzg)
+ * with nx,ny,nz the face counts nx_global.. of the input file and
+ * nxg = nx+1, nyg = ny+1, nzg = nz+1 (one rank owns the whole domain).
+ *
+ * Scalars vector (7 doubles): [t, dt, dPdx, dPdz, Qflow_x_0, dPdy, Qflow_y_0].
+ */
+#ifndef CHANNEL_FLOW_H
+#define CHANNEL_FLOW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cf_ctx cf_ctx;
+
+/* Run configuration: the fields of read_input_parameters
+ * (input_output.f90:21-81) that reach the time step, plus the constants the
+ * reference hard-codes (Smagorinsky Cs subgrid.f90:580; EQWM ncv, stretch
+ * wallmodel.f90:621, tolerance and iteration cap eqwm.f90:109-110). */
+typedef struct cf_params {
+    int nx, ny, nz;          /* nx_global, ny_global, nz_global (faces)          */
+    int precision;           /* 0 = FP64, 1 = mixed (FP32 stencils/FFT storage) */
+    double cfl;              /* CFL; < 0 means fixed dt = -CFL                   */
+    double nu;
+    double dPdx, dPdz;       /* initial forcing                                   */
+    double omega_z;          /* Rossby_plus*utau/2 (input_output.f90:84)         */
+    int x_mass_cte, y_mass_cte;
+    int les_model;           /* 0 none, 1 constant-coefficient Smagorinsky       */
+    int iwall_model;         /* 0 no-slip, 3 equilibrium wall model               */
+    int dirichlet_nu_t;
+    int jref_eqwm;
+    double cs;               /* Smagorinsky coefficient                           */
+    int eqwm_ncv;            /* wall-model ODE cells (reference: 100)             */
+    int eqwm_max_iter;       /* Picard iteration cap (reference: 100)            */
+    double eqwm_stretch;     /* wall-model grid stretching (reference: 1.10)     */
+    double eqwm_tol;         /* Picard tolerance (reference: 1e-7)               */
+    double tau_mean_eqwm;    /* fallback stress = dPdx_ref                        */
+    int wall_surrogate;      /* 0 ODE only, 1 learned MLP with ODE fallback       */
+    int device;              /* HIP device ordinal                                */
+} cf_params;
+
+const char *cf_last_error(void);
+
+/* initialize (initialization.f90:23-536): grids, operators, device buffers.
+ * x (nx), y (ny), z (nz) are the face coordinates of the input file. */
+int cf_create(const cf_params *p, const double *x, const double *y, const double *z,
+              cf_ctx **out);
+int cf_destroy(cf_ctx *c);
+
+/* host mirrors <-> device state */
+int cf_set_state(cf_ctx *c, const double *U, const double *V, const double *W);
+int cf_get_state(cf_ctx *c, double *U, double *V, double *W);
+int cf_get_pressure(cf_ctx *c, double *P);      /* projection.f90:129-137 */
+int cf_set_nu_t(cf_ctx *c, const double *nu_t);
+int cf_get_nu_t(cf_ctx *c, double *nu_t);
+int cf_get_wall(cf_ctx *c, double *alpha_x, double *alpha_z, double *tau_init);
+int cf_set_tau_init(cf_ctx *c, const double *tau_init);
+int cf_set_scalars(cf_ctx *c, const double *s);
+int cf_get_scalars(cf_ctx *c, double *s);
+
+/* initialization.f90:460: Qflow_x_0 from the current U */
+int cf_init_mass_flow(cf_ctx *c);
+
+/* apply_boundary_conditions (boundary_conditions.f90:20-75) */
+int cf_apply_boundary_conditions(cf_ctx *c);
+/* compute_dt (time_integration.f90:179-227) */
+int cf_compute_dt(cf_ctx *c, double *dt);
+/* compute_time_step_RK3 (time_integration.f90:89-173) */
+int cf_rk3_step(cf_ctx *c);
+/* main.f90:67-74 for nsteps: compute_dt + compute_time_step_RK3, all on the
+ * device with no host synchronisation inside (HIP graph when use_graph). */
+int cf_run_steps(cf_ctx *c, int nsteps, int use_graph);
+/* compute_projection_step (projection.f90:20-28); rk_step selects whether
+ * the pressure is saved (rk_step == 1) */
+int cf_projection_step(cf_ctx *c, int rk_step);
+/* compute_rhs_u/v/w (equations.f90:24,186,351) of host fields, with the
+ * context's nu_t, dPdx, dPdz */
+int cf_compute_rhs(cf_ctx *c, const double *U, const double *V, const double *W, double *rhs_u,
+                   double *rhs_v, double *rhs_w);
+/* compute_eddy_viscosity (subgrid.f90:23-47) of the current state */
+int cf_compute_eddy_viscosity(cf_ctx *c);
+/* compute_wall_model (wallmodel.f90:24-82) of the current state */
+int cf_compute_wall_model(cf_ctx *c, int *n_nan, int *n_iter);
+/* compute_mean_mass_flow_U (mass_flow.f90:25-52) of the current state */
+int cf_mean_mass_flow_u(cf_ctx *c, double *qflow);
+/* check_divergence (projection.f90:234-256) plus max |div| */
+int cf_check_divergence(cf_ctx *c, double *max_signed, double *max_abs);
+/* compute_statistics plane means (statistics.f90:43-118), FP64:
+ * out = Umean,Vmean,Wmean,U2mean,V2mean,W2mean,UVmean (ny each) */
+int cf_statistics(cf_ctx *c, double *out);
+
+/* solve_u_eqwm (eqwm.f90:90-97) for a batch of n wall points on the device.
+ * Host arrays; rho = 1.  Returns tau and the iteration count per point. */
+int cf_eqwm_batch(int n, const double *umag, const double *nu, const double *h,
+                  const double *tau_in, int ncv, double stretch, double tol, int max_iter,
+                  double *tau, int *iter);
+
+/* Per-kernel timing with HIP events recorded on the context's stream. */
+int cf_timing_enable(cf_ctx *c, int on);
+/* writes "name avg_ms count\n" lines; returns 0 */
+int cf_timing_report(cf_ctx *c, char *buf, int len);
+int cf_timing_reset(cf_ctx *c);
+/* bytes of device memory held by the context */
+long long cf_device_bytes(cf_ctx *c);
+/* synchronise the context's stream */
+int cf_sync(cf_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,199 @@
+"""GPU parity: the HIP library (through the C-ABI) against the CPU oracle on
+the same seeded inputs.  FP64 path tolerance: max relative field difference
+1e-10 (steps) and round-off (single operators)."""
+import numpy as np
+import pytest
+
+from channel_flow import cases
+from oracle_lib import Oracle, eqwm as orc_eqwm
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver():
+    from channel_flow.solver import ChannelFlow
+    return ChannelFlow
+
+
+def small_cfg(**kw):
+    cfg = cases.case_c1()
+    cfg.cells = kw.pop("cells", (32, 24, 16))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def pair(cfg):
+    CF = _solver()
+    g = CF(cfg)
+    o = Oracle(cfg, jref=g.jref_eqwm)
+    return g, o
+
+
+def interior_u(g, A):
+    return A[1:g.nx - 1, 1:g.nyg - 1, 1:g.nzg - 2]
+
+
+def test_rhs_random_fields():
+    cfg = small_cfg()
+    g, o = pair(cfg)
+    rng = np.random.default_rng(1)
+    U = np.asfortranarray(rng.standard_normal(g.shape_u))
+    V = np.asfortranarray(rng.standard_normal(g.shape_v))
+    W = np.asfortranarray(rng.standard_normal(g.shape_w))
+    ru, rv, rw = g.compute_rhs(U, V, W)
+    su, sv, sw = o.compute_rhs(U, V, W)
+    for a, b in ((ru, su), (rv, sv), (rw, sw)):
+        assert relerr(a, b) < 1e-12
+
+
+def test_rhs_with_eddy_viscosity_and_rotation():
+    cfg = small_cfg(omega_z=0.05)
+    g, o = pair(cfg)
+    rng = np.random.default_rng(2)
+    U = np.asfortranarray(rng.standard_normal(g.shape_u))
+    V = np.asfortranarray(rng.standard_normal(g.shape_v))
+    W = np.asfortranarray(rng.standard_normal(g.shape_w))
+    nt = np.asfortranarray(np.abs(rng.standard_normal(g.shape_c)) * 1e-3)
+    g.set_nu_t(nt)
+    o.set_nu_t(nt)
+    ru, rv, rw = g.compute_rhs(U, V, W)
+    su, sv, sw = o.compute_rhs(U, V, W)
+    for a, b in ((ru, su), (rv, sv), (rw, sw)):
+        assert relerr(a, b) < 1e-12
+
+
+def test_boundary_conditions_bitexact():
+    cfg = small_cfg()
+    g, o = pair(cfg)
+    rng = np.random.default_rng(3)
+    U = np.asfortranarray(rng.standard_normal(g.shape_u))
+    V = np.asfortranarray(rng.standard_normal(g.shape_v))
+    W = np.asfortranarray(rng.standard_normal(g.shape_w))
+    g.set_state(U, V, W)
+    o.set_state(U, V, W)
+    g.apply_boundary_conditions()
+    o.apply_boundary_conditions()
+    for a, b in zip(g.get_state(), o.get_state()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_projection_divergence_free():
+    cfg = small_cfg()
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    _, gmax = g.check_divergence()
+    assert gmax < 1e-12
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert relerr(a, b) < 1e-12
+    assert abs(g.scalars["Qflow_x_0"] - o.scalars["Qflow_x_0"]) < 1e-14
+
+
+@pytest.mark.parametrize("cells", [(32, 24, 16), (48, 32, 24)])
+def test_ten_rk3_steps(cells):
+    cfg = small_cfg(cells=cells)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(10, graph=False)
+    o.run(10)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert relerr(a, b) < 1e-10
+    _, dmax = g.check_divergence()
+    assert dmax < 1e-10
+    assert abs(g.scalars["dPdx"] - o.scalars["dPdx"]) < 1e-9 * abs(o.scalars["dPdx"]) + 1e-12
+
+
+def test_graph_replay_matches_eager():
+    cfg = small_cfg()
+    CF = _solver()
+    a, b = CF(cfg), CF(cfg)
+    U, V, W = cases.initial_field(cfg)
+    a.initialise(U, V, W)
+    b.initialise(U, V, W)
+    for _ in range(3):  # graph replays with eager work and syncs in between
+        a.run(2, graph=True)
+        b.run(2, graph=False)
+        a.check_divergence()
+        for x, y in zip(a.get_state(), b.get_state()):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_cfl_dt_matches():
+    cfg = small_cfg(cfl=0.5)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    assert g.compute_dt() == o.compute_dt()
+
+
+def test_statistics_match():
+    cfg = small_cfg()
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    gs, os_ = g.compute_statistics(), o.compute_statistics()
+    for k in gs:
+        np.testing.assert_allclose(gs[k], os_[k], rtol=1e-12, atol=1e-14)
+
+
+def test_smagorinsky_matches():
+    cfg = small_cfg(les_model=1, cs=0.1, stretch=0.0)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.compute_eddy_viscosity()
+    o.compute_eddy_viscosity()
+    assert relerr(g.nu_t, o.nu_t) < 1e-12
+
+
+def test_wall_model_matches():
+    cfg = small_cfg(les_model=1, iwall_model=3, stretch=0.0, eqwm_ncv=30, eqwm_tol=1e-8, nu=1e-4)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.compute_eddy_viscosity()
+    o.compute_eddy_viscosity()
+    g.compute_wall_model()
+    o.compute_wall_model()
+    for a, b in zip(g.wall_state(), o.wall_state()):
+        assert relerr(a, b) < 1e-6
+
+
+def test_wmles_steps_match():
+    cfg = small_cfg(les_model=1, iwall_model=3, stretch=0.0, eqwm_ncv=30, eqwm_tol=1e-8,
+                    nu=1e-4, cfl=-1e-3)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(3, graph=False)
+    o.run(3)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert relerr(a, b) < 1e-7
+
+
+def test_eqwm_batch_matches_oracle():
+    from channel_flow.solver import eqwm_batch
+    rng = np.random.default_rng(5)
+    n = 257
+    u = rng.uniform(0.2, 2.0, n)
+    nu = 10 ** rng.uniform(-5, -3, n)
+    h = rng.uniform(0.05, 0.2, n)
+    tin = rng.uniform(1e-4, 1e-2, n)
+    tau, it = eqwm_batch(u, nu, h, tin, ncv=30, stretch=1.1, tol=1e-8, max_iter=100)
+    for q in range(n):
+        t_o, i_o = orc_eqwm(u[q], nu[q], h[q], tin[q], ncv=30, stretch=1.1, tol=1e-8, max_iter=100)
+        assert abs(tau[q] - t_o) <= 1e-6 * abs(t_o)
+        assert abs(int(it[q]) - i_o) <= 1
