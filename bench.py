@@ -1,0 +1,188 @@
+"""Benchmark of the RK3 time step (compute_dt + compute_time_step_RK3,
+main.f90:70-73) on one MI355X.
+
+Workload: DNS at Re_b = 10000, 384x256x384 unique cells (faces
+386x257x386), tanh y-stretching 2.2, CFL 0.5, seeded Poiseuille + 10%
+perturbation projected divergence-free.  One "step" = one full RK3 step
+(3 substeps with projections, boundary conditions, mass-flow correction and
+the CFL time step).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: value = FP64 Mcell-updates/s; the mixed path
+(FP32 stencil/FFT storage, FP64 Thomas/reductions) is reported beside it.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "channel-flow_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(name, n_cells, esz, mh_ratio):
+    """Unique input+output bytes per launch of each step kernel at storage
+    precision (halo re-reads excluded).  mh_ratio = (nxp/2+1)*2/nxp: the
+    half spectrum holds mh complex values per nxp reals."""
+    spec = mh_ratio * n_cells * esz  # one pass over the half spectrum
+    f = n_cells * esz                # one pass over a cell field
+    table = {
+        "rhs_rk": 9 * f,                 # read U,V,W (+G), write U',V',W' (+G)
+        "div_fftx": 3 * f + spec,        # read U,V,W, write spectrum
+        "fftz_fwd": 2 * spec,
+        "fftz_inv": 2 * spec,
+        "thomas": 2 * spec,
+        "ifftx_proju": spec + f + 2 * f,  # read spectrum, write phi, read+write U
+        "project_vw": 5 * f,             # read phi,V,W, write V,W
+    }
+    return table.get(name)
+
+
+def run_precision(cfg, steps, warmup, cases):
+    from channel_flow.solver import ChannelFlow
+    g = ChannelFlow(cfg, device=0)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    del U, V, W
+    log("%s: initialised, %.2f GB on device" % (cfg.precision, g.device_bytes() / 1e9))
+    g.run(warmup, graph=True)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(steps, graph=True)
+    g.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    log("%s: %.3f ms/step (graph)" % (cfg.precision, ms))
+    # per-kernel durations from HIP events on the solver's stream, same steps
+    g.timing(True)
+    g.timing_reset()
+    t1 = time.perf_counter()
+    g.run(steps, graph=False)
+    g.sync()
+    ms_ev = (time.perf_counter() - t1) * 1e3 / steps
+    rep = g.timing_report()
+    g.timing(False)
+    _, div = g.check_divergence()
+    return g, ms, ms_ev, rep, div
+
+
+def cpu_baseline(cells, steps_cap_s=30.0):
+    """The oracle (FP64 C restatement of the reference step) on the host,
+    OpenMP over all cores, on a bounded sample: one RK3 step of the same
+    workload (plus one more if time allows)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    from channel_flow import cases
+    cfg = cases.case_c2()
+    cfg.cells = cells
+    nth = int(os.environ.get("CF_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    o = Oracle(cfg, nthreads=nth)
+    U, V, W = cases.initial_field(cfg)
+    o.initialise(U, V, W)
+    del U, V, W
+    log("cpu baseline: oracle initialised with %d threads" % nth)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.compute_dt()
+        o.compute_time_step_RK3()
+        n += 1
+        el = time.perf_counter() - t0
+        log("cpu baseline: step %d done, %.1f s" % (n, el))
+        if el > steps_cap_s / 2 or n >= 3:
+            break
+    sps = el / n
+    ncell = cells[0] * cells[1] * cells[2]
+    o.close()
+    return {"value": ncell / sps / 1e6, "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
+            "sample": "%d RK3 step(s) of the %dx%dx%d workload, %.2f s/step" % (n, *cells, sps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=str, default="384,256,384")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs a single case per GPU and has no "
+              "multi-GPU path; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from channel_flow import cases
+    cells = tuple(int(c) for c in a.cells.split(","))
+    ncell = cells[0] * cells[1] * cells[2]
+    mh_ratio = (cells[0] // 2 + 1) * 2.0 / cells[0]
+
+    cfg = cases.case_c2("fp64")
+    cfg.cells = cells
+    g64, ms64, ms64_ev, rep64, div64 = run_precision(cfg, a.steps, a.warmup, cases)
+    value = ncell / (ms64 / 1e3) / 1e6
+    # dominant kernel and its roofline
+    tot = {k: v[0] * v[1] for k, v in rep64.items()}
+    dom = max(tot, key=tot.get)
+    dom_ms = rep64[dom][0]
+    ab = algorithmic_bytes(dom, ncell, 8, mh_ratio)
+    achieved = ab / (dom_ms / 1e3) / 1e9 if ab else None
+    kernels = {k: {"avg_ms": round(v[0], 4), "launches": v[1],
+                   "GBps": (round(algorithmic_bytes(k, ncell, 8, mh_ratio) / (v[0] / 1e3) / 1e9, 1)
+                            if algorithmic_bytes(k, ncell, 8, mh_ratio) else None)}
+               for k, v in rep64.items()}
+    out = {
+        "metric": "Mcell-updates/s per RK3 step, FP64 vs mixed, 384x256x384; ensemble p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms64, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded Poiseuille + 10% perturbation, projected)",
+        "config": {"workload": "DNS Re_b=10000 %dx%dx%d tanh2.2 CFL0.5" % cells,
+                   "cells": list(cells), "precision": "fp64", "graph": True},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None},
+        "kernels_fp64": kernels,
+        "ms_per_step_event_timed": round(ms64_ev, 4),
+        "max_div_fp64": div64,
+    }
+    if not a.no_mixed:
+        U64, V64, W64 = g64.get_state()
+        s64 = g64.compute_statistics()
+        g64.close()
+        cfgm = cases.case_c2("mixed")
+        cfgm.cells = cells
+        gm, msm, msm_ev, repm, divm = run_precision(cfgm, a.steps, a.warmup, cases)
+        Um, Vm, Wm = gm.get_state()
+        sm = gm.compute_statistics()
+        err = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in
+                  ((Um, U64), (Vm, V64), (Wm, W64)))
+        prof = float(np.max(np.abs(sm["Umean"] - s64["Umean"])) / np.max(np.abs(s64["Umean"])))
+        out["mixed"] = {"value": round(ncell / (msm / 1e3) / 1e6, 2), "ms_per_step": round(msm, 4),
+                        "speedup_vs_fp64": round(ms64 / msm, 3), "max_div": divm,
+                        "field_max_rel_err_vs_fp64": err, "mean_profile_rel_err": prof,
+                        "steps_compared": a.steps * 2 + a.warmup,
+                        "kernels": {k: {"avg_ms": round(v[0], 4), "launches": v[1]} for k, v in repm.items()}}
+        gm.close()
+    else:
+        g64.close()
+    if not a.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(cells)
+        except MemoryError:
+            out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -108,7 +108,10 @@ def test_ten_rk3_steps(cells):
         assert relerr(a, b) < 1e-10
     _, dmax = g.check_divergence()
     assert dmax < 1e-10
-    assert abs(g.scalars["dPdx"] - o.scalars["dPdx"]) < 1e-9 * abs(o.scalars["dPdx"]) + 1e-12
+    # dPdx = correction/dt (time_integration.f90:156): compare the correction,
+    # a difference of two O(1) bulk flows summed in different orders
+    dt = o.scalars["dt"]
+    assert abs(g.scalars["dPdx"] - o.scalars["dPdx"]) * dt < 1e-13
 
 
 def test_graph_replay_matches_eager():
