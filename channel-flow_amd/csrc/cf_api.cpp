@@ -242,9 +242,14 @@ int Ctx<S>::init() {
     if (dalloc(&phi1, nphi, owned, bytes)) return -1;
     if (dalloc(&phi23, nphi, owned, bytes)) return -1;
     HCK(hipMemset(phi1, 0, nphi * sizeof(S)));
-    const int npart = std::max(3 * d.nzg, 2 * d.nzg) + 16;
+    const int npart = std::max(std::max(3 * d.nzg, 2 * d.nzg), massflow_parts(d)) + 16;
     if (dalloc(&part, npart, owned, bytes)) return -1;
+    if (dalloc(&dtscr, dt_scratch_doubles(d), owned, bytes)) return -1;
     if (dalloc(&sc, 1, owned, bytes)) return -1;
+    if (prec_mixed()) {
+        if (dalloc(&ubar, d.nyg + 2, owned, bytes)) return -1;
+        HCK(hipMemset(ubar, 0, (d.nyg + 2) * sizeof(double)));
+    }
     if (dalloc(&stats_out, 7 * d.ny + 8, owned, bytes)) return -1;
     // metrics
     if (dupload(&m_rdxf, h.rdxf, owned, bytes) || dupload(&m_rdxc, h.rdxc, owned, bytes) ||
@@ -390,7 +395,7 @@ void Ctx<S>::eddy_viscosity(S *const *F) {
     const Dims &d = h.d;
     if (h.prm.les_model != 1) return;  // nu_t stays 0 (subgrid.f90:41-45)
     tic("smagorinsky");
-    launch_smagorinsky<S>(d, m_x, m_y, m_z, m_xg, m_yg, m_zg, m_delta, F[0], F[1], F[2], NT, sc,
+    launch_smagorinsky<S>(d, m_x, m_y, m_z, m_xg, m_yg, m_zg, m_delta, F[0], F[1], F[2], NT, sc, ubar,
                           h.prm.cs, h.prm.nu, h.prm.dirichlet_nu_t, stream);
     toc();
     tic("ghosts_nut");
@@ -413,7 +418,7 @@ void Ctx<S>::wall_model(S *const *F) {
     tic("wall_model");
     launch_wall_model<S>(d, F[0], F[2], h.prm.les_model == 1 ? NT : nullptr, wt, tau_init, alpha_x, azt,
                          alpha_z, sc, ep, h.prm.nu, hw, h.prm.jref_eqwm, h.prm.tau_mean_eqwm, pred, use,
-                         stream);
+                         ubar, stream);
     toc();
 }
 
@@ -423,8 +428,8 @@ void Ctx<S>::dt_launch() {
     tic("dt");
     if (h.prm.cfl < 0) launch_dt_final_only(sc, h.prm.cfl, stream);
     else
-        launch_dt<S>(d, m_dxc, m_dyc, m_dzc, st[cur][0], st[cur][1], st[cur][2], part, sc, h.prm.cfl,
-                     h.dt_vis, stream);
+        launch_dt<S>(d, m_dxc, m_dyc, m_dzc, st[cur][0], st[cur][1], st[cur][2], ubar, dtscr, sc,
+                     h.prm.cfl, h.dt_vis, stream);
     toc();
 }
 
@@ -439,14 +444,9 @@ void Ctx<S>::step_launch() {
         eddy_viscosity(Fc);
         if (s == 1) wall_model(Fc);
         tic("rhs_rk");
-        if (prec_mixed())
-            launch_rhs_rk<S, S>(d, met, Fc[0], Fc[1], Fc[2], nut ? NT : nullptr, Fn[0], Fn[1], Fn[2], G[0],
-                                G[1], G[2], sc, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
-                                s == 1 ? 1 : 2, stream);
-        else
-            launch_rhs_rk<S, S>(d, met, Fc[0], Fc[1], Fc[2], nut ? NT : nullptr, Fn[0], Fn[1], Fn[2], G[0],
-                                G[1], G[2], sc, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
-                                s == 1 ? 1 : 2, stream);
+        launch_rhs_rk<S, S>(d, met, Fc[0], Fc[1], Fc[2], nut ? NT : nullptr, Fn[0], Fn[1], Fn[2], G[0], G[1],
+                            G[2], sc, ubar, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
+                            s == 1 ? 1 : 2, stream);
         toc();
         ghosts(Fn, 7);
         poisson(Fn, s == 1 ? phi1 : phi23);
@@ -457,9 +457,15 @@ void Ctx<S>::step_launch() {
     if (h.prm.x_mass_cte == 1) {
         tic("massflow");
         launch_massflow<S>(d, met, F[0], part, sc, 1, stream);
-        launch_add_corr_u<S>(d, F[0], sc, stream);
-        toc();
-        ghosts(F, 1);
+        if (ubar) {
+            // deviation storage: the uniform correction shifts the profile
+            launch_ubar_shift(sc, ubar, d.nyg, h.prm.iwall_model == 3 ? 1 : 0, stream);
+            toc();
+        } else {
+            launch_add_corr_u<S>(d, F[0], sc, stream);
+            toc();
+            ghosts(F, 1);
+        }
     }
     tic("advance_time");
     launch_advance_time(sc, stream);
@@ -557,11 +563,60 @@ int Ctx<S>::get(double *dst, const S *src, size_t n) {
     return 0;
 }
 
+// Mixed path: U is stored as an FP32 deviation from an FP64 profile ubar(j),
+// the plane mean of the incoming U on the interior rows, with ghost rows set
+// by the wall condition (so the ghost formulas hold for the deviation too).
+template <typename S>
+int Ctx<S>::set_profile(const double *U) {
+    const Dims &d = h.d;
+    std::vector<double> ub(d.nyg + 2, 0.0);
+    for (int j = 2; j <= d.nyg - 1; j++) {
+        double acc = 0.0;
+        for (int k = 2; k <= d.nzg - 2; k++)
+            for (int i = 2; i <= d.nx - 1; i++) acc += U[i3(d.nx, d.nyg, i, j, k)];
+        ub[j] = acc / ((double)(d.nx - 2) * (d.nzg - 3));
+    }
+    const bool neu = h.prm.iwall_model == 3;
+    ub[1] = neu ? ub[2] : -ub[2];
+    ub[d.nyg] = neu ? ub[d.nyg - 1] : -ub[d.nyg - 1];
+    HCK(hipMemcpy(ubar, ub.data(), ub.size() * sizeof(double), hipMemcpyHostToDevice));
+    // bulk-flow contribution of the profile, in the order of mass_flow.f90:37-44
+    double q = 0.0;
+    for (int j = 3; j <= d.nyg - 1; j++) q += (ub[j] + ub[j - 1]) * 0.5 * h.dyg[j];
+    Scalars s;
+    HCK(hipMemcpy(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    s.qbar_u = q * (double)(d.nx - 2) * (double)(d.nzg - 2);
+    HCK(hipMemcpy(sc, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+    ubar_h = ub;
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::sync_profile() {
+    if (!ubar) return 0;
+    HCK(hipStreamSynchronize(stream));
+    ubar_h.resize(h.d.nyg + 2);
+    HCK(hipMemcpy(ubar_h.data(), ubar, ubar_h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 template <typename S>
 int Ctx<S>::set_state(const double *U, const double *V, const double *W) {
     HCK(hipSetDevice(dev));
     HCK(hipStreamSynchronize(stream));
-    if (put(st[cur][0], U, n3[0]) || put(st[cur][1], V, n3[1]) || put(st[cur][2], W, n3[2])) return -1;
+    if (ubar) {
+        if (set_profile(U)) return -1;
+        const Dims &d = h.d;
+        std::vector<double> Ud(U, U + n3[0]);
+        for (int k = 1; k <= d.nzg; k++)
+            for (int j = 1; j <= d.nyg; j++)
+                for (int i = 1; i <= d.nx; i++) Ud[i3(d.nx, d.nyg, i, j, k)] -= ubar_h[j];
+        if (put(st[cur][0], Ud.data(), n3[0])) return -1;
+    } else if (put(st[cur][0], U, n3[0])) {
+        return -1;
+    }
+    if (put(st[cur][1], V, n3[1]) || put(st[cur][2], W, n3[2])) return -1;
+    invalidate_graphs();
     return 0;
 }
 template <typename S>
@@ -570,6 +625,13 @@ int Ctx<S>::get_state(double *U, double *V, double *W) {
     if ((U && get(U, st[cur][0], n3[0])) || (V && get(V, st[cur][1], n3[1])) ||
         (W && get(W, st[cur][2], n3[2])))
         return -1;
+    if (U && ubar) {
+        if (sync_profile()) return -1;
+        const Dims &d = h.d;
+        for (int k = 1; k <= d.nzg; k++)
+            for (int j = 1; j <= d.nyg; j++)
+                for (int i = 1; i <= d.nx; i++) U[i3(d.nx, d.nyg, i, j, k)] += ubar_h[j];
+    }
     return 0;
 }
 
@@ -709,7 +771,7 @@ template <typename S>
 int Ctx<S>::statistics(double *out) {
     HCK(hipSetDevice(dev));
     const Dims &d = h.d;
-    launch_stats<S>(d, met, st[cur][0], st[cur][1], st[cur][2], stats_out, stream);
+    launch_stats<S>(d, met, st[cur][0], st[cur][1], st[cur][2], ubar, stats_out, stream);
     HCK(hipGetLastError());
     HCK(hipStreamSynchronize(stream));
     HCK(hipMemcpy(out, stats_out, 7 * d.ny * sizeof(double), hipMemcpyDeviceToHost));
@@ -729,7 +791,17 @@ int Ctx<S>::compute_rhs(const double *U, const double *V, const double *W, doubl
     const Dims &d = h.d;
     // stage the inputs in the idle ping-pong buffers
     S *const *T = st[1 - cur];
-    if (put(T[0], U, n3[0]) || put(T[1], V, n3[1]) || put(T[2], W, n3[2])) return -1;
+    if (ubar) {
+        if (sync_profile()) return -1;
+        std::vector<double> Ud(U, U + n3[0]);
+        for (int k = 1; k <= d.nzg; k++)
+            for (int j = 1; j <= d.nyg; j++)
+                for (int i = 1; i <= d.nx; i++) Ud[i3(d.nx, d.nyg, i, j, k)] -= ubar_h[j];
+        if (put(T[0], Ud.data(), n3[0])) return -1;
+    } else if (put(T[0], U, n3[0])) {
+        return -1;
+    }
+    if (put(T[1], V, n3[1]) || put(T[2], W, n3[2])) return -1;
     const size_t nru = (size_t)(d.nx - 2) * (d.nyg - 2) * (d.nzg - 2);
     const size_t nrv = (size_t)(d.nxg - 2) * (d.ny - 2) * (d.nzg - 2);
     const size_t nrw = (size_t)(d.nxg - 2) * (d.nyg - 2) * (d.nz - 2);
@@ -737,7 +809,7 @@ int Ctx<S>::compute_rhs(const double *U, const double *V, const double *W, doubl
     HCK(hipMalloc(&dr, (nru + nrv + nrw) * sizeof(double)));
     const bool nut = h.prm.les_model == 1 || nut_user;
     launch_rhs_only<S, S>(d, met, T[0], T[1], T[2], nut ? NT : nullptr, dr, dr + nru, dr + nru + nrv, sc,
-                          h.prm.nu, h.prm.omega_z, stream);
+                          ubar, h.prm.nu, h.prm.omega_z, stream);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e == hipSuccess && ru) e = hipMemcpy(ru, dr, nru * sizeof(double), hipMemcpyDeviceToHost);

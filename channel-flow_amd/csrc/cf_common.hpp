@@ -46,7 +46,8 @@ struct Scalars {
     double norm_u;    // trapezoid normalisation (mass_flow.f90:41), host-computed
     double norm_v;
     double dt_conv[3];
-    double pad[4];
+    double qbar_u;    // bulk-flow contribution of the stored profile ubar (mixed path)
+    double pad[3];
     int n_nan_eqwm;
     int n_iter_eqwm;
     int n_fallback_mlp;

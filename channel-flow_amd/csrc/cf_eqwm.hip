@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(EQWM_WG) k_wall_model(Dims d, const S *__restr
                              double *__restrict__ tau_init, double *__restrict__ alpha_x,
                              double *__restrict__ azt, Scalars *sc, EqwmParams p, double nu,
                              double h, int jref, double tau_mean, const double *__restrict__ tau_pred,
-                             const int *__restrict__ use_pred) {
+                             const int *__restrict__ use_pred, const double *__restrict__ ubar) {
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
     EqwmLds s = eqwm_lds(p.ncv);
     eqwm_grid(p.ncv, p.stretch, s.g_fa, s.g_cv);
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(EQWM_WG) k_wall_model(Dims d, const S *__restr
                 wz[2] * (double)W[i3(nxg, nyg, ii, jw, jz + 1)];
     }
     const double wt = wx[0] * t1[0] + wx[1] * t1[1] + wx[2] * t1[2];
-    const double uu = (double)U[i3(nx, nyg, i, jw, k)];
+    const double uu = (double)U[i3(nx, nyg, i, jw, k)] + (ubar ? ubar[jw] : 0.0);
     const double ul = sqrt(uu * uu + wt * wt);
     const int64_t qa = i3(nx, 2, i, wall + 1, k);
     double tau;
@@ -228,12 +228,12 @@ template <typename S>
 void launch_wall_model(const Dims &d, const S *U, const S *W, const S *NT, const WallTables &t,
                        double *tau_init, double *alpha_x, double *azt, double *az, Scalars *sc,
                        EqwmParams p, double nu, double h, int jref, double tau_mean,
-                       const double *tau_pred, const int *use_pred, hipStream_t st) {
+                       const double *tau_pred, const int *use_pred, const double *ubar, hipStream_t st) {
     const int n = d.nx * d.nzg * 2;
     const int wg = eqwm_block(p.ncv);
     hipLaunchKernelGGL((k_wall_model<S>), dim3((n + wg - 1) / wg), dim3(wg), eqwm_lds_bytes(p.ncv, wg), st, d,
                        U, W, NT, t, tau_init,
-                       alpha_x, azt, sc, p, nu, h, jref, tau_mean, tau_pred, use_pred);
+                       alpha_x, azt, sc, p, nu, h, jref, tau_mean, tau_pred, use_pred, ubar);
     const int n2 = d.nxg * 2 * d.nz;
     hipLaunchKernelGGL(k_alpha_z, dim3((n2 + 255) / 256), dim3(256), 0, st, d, azt, az);
 }
@@ -242,7 +242,8 @@ void launch_wall_model(const Dims &d, const S *U, const S *W, const S *NT, const
 // out[q] = (u_les, u_x, w_interp) with q as in k_wall_model
 template <typename S>
 __global__ void k_wall_inputs(Dims d, const S *__restrict__ U, const S *__restrict__ W,
-                              const WallTables t, int jref, double *__restrict__ out) {
+                              const WallTables t, int jref, const double *__restrict__ ubar,
+                              double *__restrict__ out) {
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nx * nzg * 2) return;
@@ -261,28 +262,29 @@ __global__ void k_wall_inputs(Dims d, const S *__restrict__ U, const S *__restri
                 wz[2] * (double)W[i3(nxg, nyg, ii, jw, jz + 1)];
     }
     const double wt = wx[0] * t1[0] + wx[1] * t1[1] + wx[2] * t1[2];
-    const double uu = (double)U[i3(nx, nyg, i, jw, k)];
+    const double uu = (double)U[i3(nx, nyg, i, jw, k)] + (ubar ? ubar[jw] : 0.0);
     out[q] = sqrt(uu * uu + wt * wt);
 }
 
 template <typename S>
 void launch_wall_inputs(const Dims &d, const S *U, const S *W, const WallTables &t, int jref,
-                        double *out, hipStream_t st) {
+                        const double *ubar, double *out, hipStream_t st) {
     const int n = d.nx * d.nzg * 2;
-    hipLaunchKernelGGL((k_wall_inputs<S>), dim3((n + 255) / 256), dim3(256), 0, st, d, U, W, t, jref, out);
+    hipLaunchKernelGGL((k_wall_inputs<S>), dim3((n + 255) / 256), dim3(256), 0, st, d, U, W, t, jref, ubar,
+                       out);
 }
 
 template void launch_wall_model<double>(const Dims &, const double *, const double *, const double *,
                                         const WallTables &, double *, double *, double *, double *,
                                         Scalars *, EqwmParams, double, double, int, double,
-                                        const double *, const int *, hipStream_t);
+                                        const double *, const int *, const double *, hipStream_t);
 template void launch_wall_model<float>(const Dims &, const float *, const float *, const float *,
                                        const WallTables &, double *, double *, double *, double *,
                                        Scalars *, EqwmParams, double, double, int, double,
-                                       const double *, const int *, hipStream_t);
+                                       const double *, const int *, const double *, hipStream_t);
 template void launch_wall_inputs<double>(const Dims &, const double *, const double *, const WallTables &,
-                                         int, double *, hipStream_t);
+                                         int, const double *, double *, hipStream_t);
 template void launch_wall_inputs<float>(const Dims &, const float *, const float *, const WallTables &, int,
-                                        double *, hipStream_t);
+                                        const double *, double *, hipStream_t);
 
 }  // namespace cf

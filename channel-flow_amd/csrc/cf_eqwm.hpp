@@ -25,9 +25,10 @@ template <typename S>
 void launch_wall_model(const Dims &d, const S *U, const S *W, const S *NT, const WallTables &t,
                        double *tau_init, double *alpha_x, double *azt, double *az, Scalars *sc,
                        EqwmParams p, double nu, double h, int jref, double tau_mean,
-                       const double *tau_pred, const int *use_pred, hipStream_t st);
+                       const double *tau_pred, const int *use_pred, const double *ubar,
+                       hipStream_t st);
 template <typename S>
 void launch_wall_inputs(const Dims &d, const S *U, const S *W, const WallTables &t, int jref,
-                        double *out, hipStream_t st);
+                        const double *ubar, double *out, hipStream_t st);
 
 }  // namespace cf

@@ -15,6 +15,8 @@
 // The FFT engine is an LDS Stockham autosort transform (radix 4, 2, 3) over a
 // batch of lines per workgroup; each pass stages its butterflies through
 // registers so a single LDS buffer suffices.
+#include <cstdlib>
+
 #include "cf_kernels.hpp"
 
 namespace cf {
@@ -299,9 +301,20 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
 // rows per workgroup for the x transforms: at most 2048 complex elements (so
 // a radix-2 pass needs <= 4 butterflies per thread of 256) and at most 64 KiB
 // of LDS for `elems_per_row` complex values of `esize` bytes per row.
+static int env_int(const char *name, int dflt) {
+    const char *v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
 int fft_rows_per_wg(int M, int elems_per_row, int esize) {
     int rb = 2048 / M;
     if (rb > 16) rb = 16;
+    // tuning knob: cap on rows per workgroup (more workgroups per CU)
+    // (defaults from a sweep at 384x256x384: fewer rows per workgroup raise
+    // occupancy, especially for the inverse with its larger LDS footprint)
+    const bool inv = elems_per_row > M;
+    const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", inv ? 6 : 8);
+    if (cap > 0 && rb > cap) rb = cap;
     const int lim = 65536 / (elems_per_row * esize);
     if (rb > lim) rb = lim;
     if (rb < 1) rb = 1;
@@ -312,6 +325,8 @@ int fft_rows_per_wg(int M, int elems_per_row, int esize) {
 int fft_cols_per_wg(int n, int esize) {
     int b = 4096 / n;
     if (b > 16) b = 16;
+    const int cap = env_int("CF_FFT_COLS", 0);
+    if (cap > 0 && b > cap) b = cap;
     const int lim = 65536 / ((n + 1) * esize);
     if (b > lim) b = lim;
     if (b < 1) b = 1;

@@ -21,6 +21,7 @@ void set_error(const std::string &m);
 void launch_reset_counters(Scalars *sc, hipStream_t st);
 void launch_dt_final_only(Scalars *sc, double cfl, hipStream_t st);
 void launch_advance_time(Scalars *sc, hipStream_t st);
+void launch_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann, hipStream_t st);
 
 // Host copy of the grids and operators of initialization.f90.
 struct Host {
@@ -93,6 +94,11 @@ struct Ctx : CtxBase {
     double *cscr = nullptr;
     S *phi1 = nullptr, *phi23 = nullptr;
     double *part = nullptr, *stats_out = nullptr;
+    double *dtscr = nullptr;
+    double *ubar = nullptr;            // mixed path: FP64 profile U = ubar(j) + U'
+    std::vector<double> ubar_h;
+    int set_profile(const double *U);
+    int sync_profile();
     Scalars *sc = nullptr;
     double *m_rdxf, *m_rdxc, *m_rdyf, *m_rdyc, *m_rdzf, *m_rdzc, *m_w0, *m_w1, *m_dyg;
     double *m_dxc, *m_dyc, *m_dzc, *m_x, *m_y, *m_z, *m_xg, *m_yg, *m_zg;

@@ -16,12 +16,12 @@ struct FFTPlan {
 template <typename S, typename R>
 void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                    const S *NT, S *Uo, S *Vo, S *Wo, S *Gu, S *Gv, S *Gw,
-                   const Scalars *sc, double nu, double om, double a_new, double a_g,
-                   int sub, hipStream_t st);
+                   const Scalars *sc, const double *ubar, double nu, double om, double a_new,
+                   double a_g, int sub, hipStream_t st);
 template <typename S, typename R>
 void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      const S *NT, double *Ru, double *Rv, double *Rw, const Scalars *sc,
-                     double nu, double om, hipStream_t st);
+                     const double *ubar, double nu, double om, hipStream_t st);
 template <typename S>
 void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *alpha,
                    double dyg_bot, double dyg_top, hipStream_t st);
@@ -34,19 +34,21 @@ template <typename S>
 void launch_add_corr_u(const Dims &d, S *U, const Scalars *sc, hipStream_t st);
 template <typename S>
 void launch_dt(const Dims &d, const double *dxc, const double *dyc, const double *dzc,
-               const S *U, const S *V, const S *W, double *part, Scalars *sc, double cfl,
-               double dt_vis, hipStream_t st);
+               const S *U, const S *V, const S *W, const double *ubar, double *scratch,
+               Scalars *sc, double cfl, double dt_vis, hipStream_t st);
+size_t dt_scratch_doubles(const Dims &d);
+int massflow_parts(const Dims &d);
 template <typename S>
 void launch_div(const Dims &d, const double *x, const double *y, const double *z, const S *U,
                 const S *V, const S *W, double *part, hipStream_t st);
 template <typename S>
 void launch_smagorinsky(const Dims &d, const double *x, const double *y, const double *z,
                         const double *xg, const double *yg, const double *zg, const double *delta,
-                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc, double cs,
-                        double nu, int damping, hipStream_t st);
+                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc,
+                        const double *ubar, double cs, double nu, int damping, hipStream_t st);
 template <typename S>
-void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W, double *out,
-                  hipStream_t st);
+void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
+                  const double *ubar, double *out, hipStream_t st);
 
 // ---- Poisson chain (cf_fft.hip) ----
 int fft_rows_per_wg(int M, int elems_per_row, int esize);

@@ -8,174 +8,136 @@
 namespace cf {
 
 // ---------------------------------------------------------------------------
-// Fused RHS (equations.f90:24-498) + RK3 combine (time_integration.f90:104-145)
-//
-// One thread per unique cell (i in 2..nx-1, j in 2..nyg-1, k in 2..nzg-2);
-// duplicate periodic planes are filled afterwards by the ghost kernel, as the
-// reference's apply_boundary_conditions overwrites them.  Low-storage form:
-//   substep 1:  U1 = U0 + dt*a11*F1,   G = U0 + dt*a21*F1
-//   substep s:  Us = G  + dt*ass*Fs    (s = 2, 3; a31 = a21, a32 = 0)
-// so the step keeps G instead of Uo and F1 (3 fields instead of 6).
+// Right-hand sides at one point (equations.f90:24-498), shared by the fused
+// step kernel (LDS accessor) and the module-level RHS kernel (global
+// accessor).  A.u(di,dj,dk) returns U(i+di, j+dj, k+dk) etc.  Per term the
+// reference's association order is kept:
+//   (((conv_1 + conv_2) + conv_3) + visc) + dP  [+ rotation]
+// with divisions by grid spacings replaced by multiplications with
+// precomputed reciprocals.
 // ---------------------------------------------------------------------------
-template <typename S, typename R, bool NUT, bool ROT, int SUB>
-__global__ void __launch_bounds__(256)
-k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
-         const S *__restrict__ W, const S *__restrict__ NT, S *__restrict__ Uo,
-         S *__restrict__ Vo, S *__restrict__ Wo, S *__restrict__ Gu, S *__restrict__ Gv,
-         S *__restrict__ Gw, const Scalars *__restrict__ sc, double nu_d, double om_d,
-         double a_new, double a_g, int tiles_x, int tiles_y, int ntiles) {
-    const int t = xcd_remap(blockIdx.x, ntiles);
-    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
-    const int i = 2 + tx * 64 + (threadIdx.x & 63);
-    const int j = 2 + ty * 4 + (threadIdx.x >> 6);
-    const int k = 2 + tz;
-    if (i > d.nx - 1 || j > d.nyg - 1) return;
-
-    const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
-    const R nu = (R)nu_d, half = (R)0.5, quart = (R)0.25, two = (R)2;
-#define u(I, J, K) ((R)U[i3(nx, nyg, I, J, K)])
-#define v(I, J, K) ((R)V[i3(nxg, ny, I, J, K)])
-#define w(I, J, K) ((R)W[i3(nxg, nyg, I, J, K)])
-#define nt(I, J, K) ((R)NT[i3(nxg, nyg, I, J, K)])
+template <typename R, bool NUT, bool ROT, class A>
+__device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, int j, int k, R nu, R om,
+                                          R dPdx, R dPdz, bool do_u, bool do_v, bool do_w, R &ru,
+                                          R &rv, R &rw) {
+    const R half = (R)0.5, quart = (R)0.25, two = (R)2;
     const R w0j = (R)m.w0[j], w1j = (R)m.w1[j];
     const R w0m = (R)m.w0[j - 1], w1m = (R)m.w1[j - 1];
-    const double dt = sc->dt;
-
-    // ---------------- u ----------------
-    {
-        R c1 = half * (u(i, j, k) + u(i + 1, j, k));
-        R c0 = half * (u(i - 1, j, k) + u(i, j, k));
+#define u(I, J, K) a.u(I, J, K)
+#define v(I, J, K) a.v(I, J, K)
+#define w(I, J, K) a.w(I, J, K)
+#define nt(I, J, K) a.nt(I, J, K)
+    if (do_u) {
+        R c1 = half * (u(0, 0, 0) + u(1, 0, 0));
+        R c0 = half * (u(-1, 0, 0) + u(0, 0, 0));
         R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdxc[i];
-        R uv1 = (w0j * u(i, j, k) + w1j * u(i, j + 1, k)) * (half * (v(i, j, k) + v(i + 1, j, k)));
-        R uv0 = (w0m * u(i, j - 1, k) + w1m * u(i, j, k)) * (half * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
+        R uv1 = (w0j * u(0, 0, 0) + w1j * u(0, 1, 0)) * (half * (v(0, 0, 0) + v(1, 0, 0)));
+        R uv0 = (w0m * u(0, -1, 0) + w1m * u(0, 0, 0)) * (half * (v(0, -1, 0) + v(1, -1, 0)));
         R t2 = -(uv1 - uv0) * (R)m.rdyf[j];
-        R uw1 = (half * (u(i, j, k) + u(i, j, k + 1))) * (half * (w(i, j, k) + w(i + 1, j, k)));
-        R uw0 = (half * (u(i, j, k - 1) + u(i, j, k))) * (half * (w(i, j, k - 1) + w(i + 1, j, k - 1)));
+        R uw1 = (half * (u(0, 0, 0) + u(0, 0, 1))) * (half * (w(0, 0, 0) + w(1, 0, 0)));
+        R uw0 = (half * (u(0, 0, -1) + u(0, 0, 0))) * (half * (w(0, 0, -1) + w(1, 0, -1)));
         R t3 = -(uw1 - uw0) * (R)m.rdzf[k];
         R rdz1 = (R)m.rdzc[k - 1], rdz2 = (R)m.rdzc[k], rdz3 = (R)m.rdzf[k];
         R rdy3 = (R)m.rdyf[j];
         R rdx1 = (R)m.rdxf[i], rdx2 = (R)m.rdxf[i + 1], rdx3 = (R)m.rdxc[i];
         R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
         if (NUT) {
-            nux1 = nu + nt(i, j, k);
-            nux2 = nu + nt(i + 1, j, k);
-            nuy1 = nu + half * (w0m * nt(i, j - 1, k) + w1m * nt(i, j, k) + w0m * nt(i + 1, j - 1, k) + w1m * nt(i + 1, j, k));
-            nuy2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i + 1, j, k) + w1j * nt(i + 1, j + 1, k));
-            nuz1 = nu + quart * (nt(i, j, k) + nt(i, j, k - 1) + nt(i + 1, j, k) + nt(i + 1, j, k - 1));
-            nuz2 = nu + quart * (nt(i, j, k) + nt(i, j, k + 1) + nt(i + 1, j, k) + nt(i + 1, j, k + 1));
+            nux1 = nu + nt(0, 0, 0);
+            nux2 = nu + nt(1, 0, 0);
+            nuy1 = nu + half * (w0m * nt(0, -1, 0) + w1m * nt(0, 0, 0) + w0m * nt(1, -1, 0) + w1m * nt(1, 0, 0));
+            nuy2 = nu + half * (w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0) + w0j * nt(1, 0, 0) + w1j * nt(1, 1, 0));
+            nuz1 = nu + quart * (nt(0, 0, 0) + nt(0, 0, -1) + nt(1, 0, 0) + nt(1, 0, -1));
+            nuz2 = nu + quart * (nt(0, 0, 0) + nt(0, 0, 1) + nt(1, 0, 0) + nt(1, 0, 1));
         }
-        R dudy1 = (u(i, j + 1, k) - u(i, j, k)) * (R)m.rdyc[j];
-        R dudy0 = (u(i, j, k) - u(i, j - 1, k)) * (R)m.rdyc[j - 1];
-        R visc = two * rdx3 * (nux2 * (u(i + 1, j, k) - u(i, j, k)) * rdx2 - nux1 * (u(i, j, k) - u(i - 1, j, k)) * rdx1) +
-                 rdy3 * (nuy2 * (dudy1 + (v(i + 1, j, k) - v(i, j, k)) * rdx2) -
-                         nuy1 * (dudy0 + (v(i + 1, j - 1, k) - v(i, j - 1, k)) * rdx1)) +
-                 rdz3 * (nuz2 * ((u(i, j, k + 1) - u(i, j, k)) * rdz2 + (w(i + 1, j, k) - w(i, j, k)) * rdx2) -
-                         nuz1 * ((u(i, j, k) - u(i, j, k - 1)) * rdz1 + (w(i + 1, j, k - 1) - w(i, j, k - 1)) * rdx1));
+        R dudy1 = (u(0, 1, 0) - u(0, 0, 0)) * (R)m.rdyc[j];
+        R dudy0 = (u(0, 0, 0) - u(0, -1, 0)) * (R)m.rdyc[j - 1];
+        R visc = two * rdx3 * (nux2 * (u(1, 0, 0) - u(0, 0, 0)) * rdx2 - nux1 * (u(0, 0, 0) - u(-1, 0, 0)) * rdx1) +
+                 rdy3 * (nuy2 * (dudy1 + (v(1, 0, 0) - v(0, 0, 0)) * rdx2) -
+                         nuy1 * (dudy0 + (v(1, -1, 0) - v(0, -1, 0)) * rdx1)) +
+                 rdz3 * (nuz2 * ((u(0, 0, 1) - u(0, 0, 0)) * rdz2 + (w(1, 0, 0) - w(0, 0, 0)) * rdx2) -
+                         nuz1 * ((u(0, 0, 0) - u(0, 0, -1)) * rdz1 + (w(1, 0, -1) - w(0, 0, -1)) * rdx1));
         R r = t1 + t2;
         r = r + t3;
         r = r + visc;
-        r = r + (R)sc->dPdx;
+        r = r + dPdx;
         if (ROT) {
-            R a = half * (v(i, j - 1, k) + v(i, j, k));
-            R b = half * (v(i + 1, j - 1, k) + v(i + 1, j, k));
-            r = r + two * (R)om_d * (half * (a + b));
+            R aa = half * (v(0, -1, 0) + v(0, 0, 0));
+            R bb = half * (v(1, -1, 0) + v(1, 0, 0));
+            r = r + two * om * (half * (aa + bb));
         }
-        const int64_t q = i3(nx, nyg, i, j, k);
-        if (SUB == 1) {
-            R u0 = u(i, j, k);
-            Uo[q] = (S)(u0 + (R)(dt * a_new) * r);
-            Gu[q] = (S)(u0 + (R)(dt * a_g) * r);
-        } else {
-            Uo[q] = (S)((R)Gu[q] + (R)(dt * a_new) * r);
-        }
+        ru = r;
     }
-    // ---------------- v (j <= ny-1) ----------------
-    if (j <= ny - 1) {
-        R c1 = half * (v(i, j, k) + v(i, j + 1, k));
-        R c0 = half * (v(i, j - 1, k) + v(i, j, k));
+    if (do_v) {
+        R c1 = half * (v(0, 0, 0) + v(0, 1, 0));
+        R c0 = half * (v(0, -1, 0) + v(0, 0, 0));
         R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdyc[j];
-        R uv1 = (w0j * u(i, j, k) + w1j * u(i, j + 1, k)) * (half * (v(i, j, k) + v(i + 1, j, k)));
-        R uv0 = (w0j * u(i - 1, j, k) + w1j * u(i - 1, j + 1, k)) * (half * (v(i - 1, j, k) + v(i, j, k)));
+        R uv1 = (w0j * u(0, 0, 0) + w1j * u(0, 1, 0)) * (half * (v(0, 0, 0) + v(1, 0, 0)));
+        R uv0 = (w0j * u(-1, 0, 0) + w1j * u(-1, 1, 0)) * (half * (v(-1, 0, 0) + v(0, 0, 0)));
         R t2 = -(uv1 - uv0) * (R)m.rdxf[i];
-        R vw1 = (half * (v(i, j, k) + v(i, j, k + 1))) * (w0j * w(i, j, k) + w1j * w(i, j + 1, k));
-        R vw0 = (half * (v(i, j, k - 1) + v(i, j, k))) * (w0j * w(i, j, k - 1) + w1j * w(i, j + 1, k - 1));
+        R vw1 = (half * (v(0, 0, 0) + v(0, 0, 1))) * (w0j * w(0, 0, 0) + w1j * w(0, 1, 0));
+        R vw0 = (half * (v(0, 0, -1) + v(0, 0, 0))) * (w0j * w(0, 0, -1) + w1j * w(0, 1, -1));
         R t3 = -(vw1 - vw0) * (R)m.rdzf[k];
         R rdz1 = (R)m.rdzc[k - 1], rdz2 = (R)m.rdzc[k], rdz3 = (R)m.rdzf[k];
         R rdy1 = (R)m.rdyf[j], rdy2 = (R)m.rdyf[j + 1], rdy3 = (R)m.rdyc[j];
         R rdx1 = (R)m.rdxc[i - 1], rdx2 = (R)m.rdxc[i], rdx3 = (R)m.rdxf[i];
         R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
         if (NUT) {
-            nux1 = nu + half * (w0j * nt(i - 1, j, k) + w1j * nt(i - 1, j + 1, k) + w0j * nt(i, j, k) + w1j * nt(i, j + 1, k));
-            nux2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i + 1, j, k) + w1j * nt(i + 1, j + 1, k));
-            nuy1 = nu + nt(i, j, k);
-            nuy2 = nu + nt(i, j + 1, k);
-            nuz1 = nu + half * (w0j * nt(i, j, k - 1) + w1j * nt(i, j + 1, k - 1) + w0j * nt(i, j, k) + w1j * nt(i, j + 1, k));
-            nuz2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i, j, k + 1) + w1j * nt(i, j + 1, k + 1));
+            nux1 = nu + half * (w0j * nt(-1, 0, 0) + w1j * nt(-1, 1, 0) + w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0));
+            nux2 = nu + half * (w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0) + w0j * nt(1, 0, 0) + w1j * nt(1, 1, 0));
+            nuy1 = nu + nt(0, 0, 0);
+            nuy2 = nu + nt(0, 1, 0);
+            nuz1 = nu + half * (w0j * nt(0, 0, -1) + w1j * nt(0, 1, -1) + w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0));
+            nuz2 = nu + half * (w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0) + w0j * nt(0, 0, 1) + w1j * nt(0, 1, 1));
         }
-        R visc = rdx3 * (nux2 * ((v(i + 1, j, k) - v(i, j, k)) * rdx2 + (u(i, j + 1, k) - u(i, j, k)) * rdy3) -
-                         nux1 * ((v(i, j, k) - v(i - 1, j, k)) * rdx1 + (u(i - 1, j + 1, k) - u(i - 1, j, k)) * rdy3)) +
-                 two * rdy3 * (nuy2 * (v(i, j + 1, k) - v(i, j, k)) * rdy2 - nuy1 * (v(i, j, k) - v(i, j - 1, k)) * rdy1) +
-                 rdz3 * (nuz2 * ((v(i, j, k + 1) - v(i, j, k)) * rdz2 + (w(i, j + 1, k) - w(i, j, k)) * rdy3) -
-                         nuz1 * ((v(i, j, k) - v(i, j, k - 1)) * rdz1 + (w(i, j + 1, k - 1) - w(i, j, k - 1)) * rdy3));
+        R visc = rdx3 * (nux2 * ((v(1, 0, 0) - v(0, 0, 0)) * rdx2 + (u(0, 1, 0) - u(0, 0, 0)) * rdy3) -
+                         nux1 * ((v(0, 0, 0) - v(-1, 0, 0)) * rdx1 + (u(-1, 1, 0) - u(-1, 0, 0)) * rdy3)) +
+                 two * rdy3 * (nuy2 * (v(0, 1, 0) - v(0, 0, 0)) * rdy2 - nuy1 * (v(0, 0, 0) - v(0, -1, 0)) * rdy1) +
+                 rdz3 * (nuz2 * ((v(0, 0, 1) - v(0, 0, 0)) * rdz2 + (w(0, 1, 0) - w(0, 0, 0)) * rdy3) -
+                         nuz1 * ((v(0, 0, 0) - v(0, 0, -1)) * rdz1 + (w(0, 1, -1) - w(0, 0, -1)) * rdy3));
         R r = t1 + t2;
         r = r + t3;
         r = r + visc;
         if (ROT) {
-            R a = half * (u(i - 1, j, k) + u(i, j, k));
-            R b = half * (u(i - 1, j + 1, k) + u(i, j + 1, k));
-            r = r - two * (R)om_d * (w0j * a + w1j * b);
+            R aa = half * (u(-1, 0, 0) + u(0, 0, 0));
+            R bb = half * (u(-1, 1, 0) + u(0, 1, 0));
+            r = r - two * om * (w0j * aa + w1j * bb);
         }
-        const int64_t q = i3(nxg, ny, i, j, k);
-        if (SUB == 1) {
-            R v0 = v(i, j, k);
-            Vo[q] = (S)(v0 + (R)(dt * a_new) * r);
-            Gv[q] = (S)(v0 + (R)(dt * a_g) * r);
-        } else {
-            Vo[q] = (S)((R)Gv[q] + (R)(dt * a_new) * r);
-        }
+        rv = r;
     }
-    // ---------------- w ----------------
-    {
-        R c1 = half * (w(i, j, k) + w(i, j, k + 1));
-        R c0 = half * (w(i, j, k - 1) + w(i, j, k));
+    if (do_w) {
+        R c1 = half * (w(0, 0, 0) + w(0, 0, 1));
+        R c0 = half * (w(0, 0, -1) + w(0, 0, 0));
         R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdzc[k];
-        R uw1 = (half * (u(i, j, k) + u(i, j, k + 1))) * (half * (w(i, j, k) + w(i + 1, j, k)));
-        R uw0 = (half * (u(i - 1, j, k) + u(i - 1, j, k + 1))) * (half * (w(i - 1, j, k) + w(i, j, k)));
+        R uw1 = (half * (u(0, 0, 0) + u(0, 0, 1))) * (half * (w(0, 0, 0) + w(1, 0, 0)));
+        R uw0 = (half * (u(-1, 0, 0) + u(-1, 0, 1))) * (half * (w(-1, 0, 0) + w(0, 0, 0)));
         R t2 = -(uw1 - uw0) * (R)m.rdxf[i];
-        R vw1 = (half * (v(i, j, k) + v(i, j, k + 1))) * (w0j * w(i, j, k) + w1j * w(i, j + 1, k));
-        R vw0 = (half * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (w0m * w(i, j - 1, k) + w1m * w(i, j, k));
+        R vw1 = (half * (v(0, 0, 0) + v(0, 0, 1))) * (w0j * w(0, 0, 0) + w1j * w(0, 1, 0));
+        R vw0 = (half * (v(0, -1, 0) + v(0, -1, 1))) * (w0m * w(0, -1, 0) + w1m * w(0, 0, 0));
         R t3 = -(vw1 - vw0) * (R)m.rdyf[j];
         R rdz1 = (R)m.rdzf[k], rdz2 = (R)m.rdzf[k + 1], rdz3 = (R)m.rdzc[k];
         R rdy3 = (R)m.rdyf[j];
         R rdx1 = (R)m.rdxc[i - 1], rdx2 = (R)m.rdxc[i], rdx3 = (R)m.rdxf[i];
         R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
         if (NUT) {
-            nux1 = nu + quart * (nt(i, j, k) + nt(i, j, k + 1) + nt(i - 1, j, k) + nt(i - 1, j, k + 1));
-            nux2 = nu + quart * (nt(i, j, k) + nt(i, j, k + 1) + nt(i + 1, j, k) + nt(i + 1, j, k + 1));
-            nuy1 = nu + half * (w0m * nt(i, j - 1, k) + w1m * nt(i, j, k) + w0m * nt(i, j - 1, k + 1) + w1m * nt(i, j, k + 1));
-            nuy2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i, j, k + 1) + w1j * nt(i, j + 1, k + 1));
-            nuz1 = nu + nt(i, j, k);
-            nuz2 = nu + nt(i, j, k + 1);
+            nux1 = nu + quart * (nt(0, 0, 0) + nt(0, 0, 1) + nt(-1, 0, 0) + nt(-1, 0, 1));
+            nux2 = nu + quart * (nt(0, 0, 0) + nt(0, 0, 1) + nt(1, 0, 0) + nt(1, 0, 1));
+            nuy1 = nu + half * (w0m * nt(0, -1, 0) + w1m * nt(0, 0, 0) + w0m * nt(0, -1, 1) + w1m * nt(0, 0, 1));
+            nuy2 = nu + half * (w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0) + w0j * nt(0, 0, 1) + w1j * nt(0, 1, 1));
+            nuz1 = nu + nt(0, 0, 0);
+            nuz2 = nu + nt(0, 0, 1);
         }
-        R dwdy1 = (w(i, j + 1, k) - w(i, j, k)) * (R)m.rdyc[j];
-        R dwdy0 = (w(i, j, k) - w(i, j - 1, k)) * (R)m.rdyc[j - 1];
-        R visc = rdx3 * (nux2 * ((w(i + 1, j, k) - w(i, j, k)) * rdx2 + (u(i, j, k + 1) - u(i, j, k)) * rdz2) -
-                         nux1 * ((w(i, j, k) - w(i - 1, j, k)) * rdx1 + (u(i - 1, j, k + 1) - u(i - 1, j, k)) * rdz1)) +
-                 rdy3 * (nuy2 * (dwdy1 + (v(i, j, k + 1) - v(i, j, k)) * rdz2) -
-                         nuy1 * (dwdy0 + (v(i, j - 1, k + 1) - v(i, j - 1, k)) * rdz1)) +
-                 two * rdz3 * (nuz2 * (w(i, j, k + 1) - w(i, j, k)) * rdz2 - nuz1 * (w(i, j, k) - w(i, j, k - 1)) * rdz1);
+        R dwdy1 = (w(0, 1, 0) - w(0, 0, 0)) * (R)m.rdyc[j];
+        R dwdy0 = (w(0, 0, 0) - w(0, -1, 0)) * (R)m.rdyc[j - 1];
+        R visc = rdx3 * (nux2 * ((w(1, 0, 0) - w(0, 0, 0)) * rdx2 + (u(0, 0, 1) - u(0, 0, 0)) * rdz2) -
+                         nux1 * ((w(0, 0, 0) - w(-1, 0, 0)) * rdx1 + (u(-1, 0, 1) - u(-1, 0, 0)) * rdz1)) +
+                 rdy3 * (nuy2 * (dwdy1 + (v(0, 0, 1) - v(0, 0, 0)) * rdz2) -
+                         nuy1 * (dwdy0 + (v(0, -1, 1) - v(0, -1, 0)) * rdz1)) +
+                 two * rdz3 * (nuz2 * (w(0, 0, 1) - w(0, 0, 0)) * rdz2 - nuz1 * (w(0, 0, 0) - w(0, 0, -1)) * rdz1);
         R r = t1 + t2;
         r = r + t3;
         r = r + visc;
-        r = r + (R)sc->dPdz;
-        const int64_t q = i3(nxg, nyg, i, j, k);
-        if (SUB == 1) {
-            R w0v = w(i, j, k);
-            Wo[q] = (S)(w0v + (R)(dt * a_new) * r);
-            Gw[q] = (S)(w0v + (R)(dt * a_g) * r);
-        } else {
-            Wo[q] = (S)((R)Gw[q] + (R)(dt * a_new) * r);
-        }
+        r = r + dPdz;
+        rw = r;
     }
 #undef u
 #undef v
@@ -183,148 +145,189 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
 #undef nt
 }
 
+// global-memory accessor (module-level RHS entry point)
+template <typename S, typename R>
+struct GAcc {
+    const S *U, *V, *W, *NT;
+    const double *ubar;
+    int i, j, k, nx, nxg, nyg, ny;
+    __device__ R u(int a, int b, int c) const {
+        R x = (R)U[i3(nx, nyg, i + a, j + b, k + c)];
+        return ubar ? x + (R)ubar[j + b] : x;
+    }
+    __device__ R v(int a, int b, int c) const { return (R)V[i3(nxg, ny, i + a, j + b, k + c)]; }
+    __device__ R w(int a, int b, int c) const { return (R)W[i3(nxg, nyg, i + a, j + b, k + c)]; }
+    __device__ R nt(int a, int b, int c) const { return (R)NT[i3(nxg, nyg, i + a, j + b, k + c)]; }
+};
+
+// ---------------------------------------------------------------------------
+// Fused RHS + RK3 combine (time_integration.f90:104-145), 2.5-D blocked.
+//
+// A workgroup owns a 64 x TY tile of (i,j) and marches over a chunk of KC
+// z-planes.  U, V, W (and nu_t) live in a 3-plane ring in LDS covering the
+// tile plus a one-cell halo, so each value is read from HBM about once per
+// chunk; all stencil reads are LDS reads.  One thread per unique cell
+// (i in 2..nx-1, j in 2..nyg-1, k in 2..nzg-2); duplicate periodic planes are
+// filled by the ghost kernel afterwards, as the reference's
+// apply_boundary_conditions overwrites them.  Low-storage form:
+//   substep 1:  U1 = U0 + dt*a11*F1,   G = U0 + dt*a21*F1
+//   substep s:  Us = G  + dt*ass*Fs    (s = 2, 3; a31 = a21, a32 = 0)
+// so the step keeps G instead of Uo and F1 (3 fields instead of 6).
+// In the mixed path U is stored as a deviation from a per-plane FP64 profile
+// ubar(j) (the RK combine is linear, so deviations update in place).
+// ---------------------------------------------------------------------------
+constexpr int RK_TX = 64, RK_KC = 16, RK_NSLOT = 4;
+
+template <typename R, int TY>
+struct LAcc {
+    static constexpr int LX = RK_TX + 2, PL = (RK_TX + 2) * (TY + 2);
+    const R *lu, *lv, *lw, *ln;  // ring base per field: [RK_NSLOT][TY+2][LX]
+    int li, lj, slot[3];         // local (i,j) in the halo frame; ring slots of k-1, k, k+1
+    R ub[3];                     // ubar(j-1), ubar(j), ubar(j+1)
+    __device__ int at(int a, int b, int c) const { return slot[c + 1] * PL + (lj + b) * LX + (li + a); }
+    __device__ R u(int a, int b, int c) const { return lu[at(a, b, c)] + ub[b + 1]; }
+    __device__ R v(int a, int b, int c) const { return lv[at(a, b, c)]; }
+    __device__ R w(int a, int b, int c) const { return lw[at(a, b, c)]; }
+    __device__ R nt(int a, int b, int c) const { return ln[at(a, b, c)]; }
+};
+
+// The ring has four slots so one barrier per plane suffices: during plane k
+// the workgroup reads slots (k-1, k, k+1) and writes plane k+2 (prefetched
+// into registers one iteration earlier) into the slot of plane k-2.
+template <typename S, typename R, bool NUT, bool ROT, int SUB, int TY>
+__global__ void __launch_bounds__(RK_TX *TY)
+k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
+         const S *__restrict__ W, const S *__restrict__ NT, S *__restrict__ Uo,
+         S *__restrict__ Vo, S *__restrict__ Wo, S *__restrict__ Gu, S *__restrict__ Gv,
+         S *__restrict__ Gw, const Scalars *__restrict__ sc, const double *__restrict__ ubar,
+         double nu_d, double om_d, double a_new, double a_g, int tiles_x, int tiles_y,
+         int nchunks, int ntiles) {
+    constexpr int LX = RK_TX + 2, PL = LX * (TY + 2), NT_ = RK_TX * TY;
+    constexpr int NPF = (PL + NT_ - 1) / NT_;  // plane elements per thread
+    extern __shared__ unsigned char smem_raw[];
+    R *ring = reinterpret_cast<R *>(smem_raw);
+    R *lu = ring, *lv = ring + RK_NSLOT * PL, *lw = ring + 2 * RK_NSLOT * PL, *ln = ring + 3 * RK_NSLOT * PL;
+    const int t = xcd_remap(blockIdx.x, ntiles);
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tc = t / (tiles_x * tiles_y);
+    const int i0 = 2 + tx * RK_TX, j0 = 2 + ty * TY;
+    const int kbeg = 2 + tc * RK_KC;
+    const int kend = min(kbeg + RK_KC, d.nzg - 1);  // exclusive; unique planes 2..nzg-2
+    const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nz = d.nz, nzg = d.nzg;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int tid = ly * RK_TX + lx;
+    const int i = i0 + lx;
+    const int j = __builtin_amdgcn_readfirstlane(j0 + ly);
+    const bool active = i <= nx - 1 && j <= nyg - 1;
+
+    R pu[NPF], pv[NPF], pw[NPF], pn[NPF];
+    auto fetch = [&](int kk) {  // global -> registers (tile + halo of plane kk)
+#pragma unroll
+        for (int q = 0; q < NPF; q++) {
+            const int e = tid + q * NT_;
+            const int ex = e % LX, ey = e / LX;
+            const int gi = i0 - 1 + ex, gj = j0 - 1 + ey;
+            R vu = 0, vv = 0, vw = 0, vn = 0;
+            if (e < PL) {
+                if (gj <= nyg) {
+                    if (gi <= nx && kk <= nzg) vu = (R)U[i3(nx, nyg, gi, gj, kk)];
+                    if (gi <= nxg && kk <= nz) vw = (R)W[i3(nxg, nyg, gi, gj, kk)];
+                    if (NUT && gi <= nxg && kk <= nzg) vn = (R)NT[i3(nxg, nyg, gi, gj, kk)];
+                }
+                if (gj <= ny && gi <= nxg && kk <= nzg) vv = (R)V[i3(nxg, ny, gi, gj, kk)];
+            }
+            pu[q] = vu; pv[q] = vv; pw[q] = vw; pn[q] = vn;
+        }
+    };
+    auto store = [&](int sl) {  // registers -> ring slot
+#pragma unroll
+        for (int q = 0; q < NPF; q++) {
+            const int e = tid + q * NT_;
+            if (e < PL) {
+                lu[sl * PL + e] = pu[q];
+                lv[sl * PL + e] = pv[q];
+                lw[sl * PL + e] = pw[q];
+                if (NUT) ln[sl * PL + e] = pn[q];
+            }
+        }
+    };
+    fetch(kbeg - 1);
+    store(0);
+    fetch(kbeg);
+    store(1);
+    fetch(kbeg + 1);
+    store(2);
+    if (kbeg + 2 <= kend) fetch(kbeg + 2);
+    __syncthreads();
+
+    LAcc<R, TY> a;
+    a.lu = lu; a.lv = lv; a.lw = lw; a.ln = ln;
+    a.li = lx + 1; a.lj = ly + 1;
+    if (ubar) {
+        a.ub[0] = (R)ubar[j - 1]; a.ub[1] = (R)ubar[j]; a.ub[2] = (R)ubar[min(j + 1, nyg)];
+    } else {
+        a.ub[0] = a.ub[1] = a.ub[2] = (R)0;
+    }
+    const R nu = (R)nu_d, om = (R)om_d, dPdx = (R)sc->dPdx, dPdz = (R)sc->dPdz;
+    const double dt = sc->dt;
+    const R cnew = (R)(dt * a_new), cg = (R)(dt * a_g);
+    for (int k = kbeg; k < kend; k++) {
+        const int r0 = k - kbeg;
+        if (active) {
+            a.slot[0] = r0 & 3; a.slot[1] = (r0 + 1) & 3; a.slot[2] = (r0 + 2) & 3;
+            R ru = 0, rv = 0, rw = 0;
+            rhs_point<R, NUT, ROT>(a, m, i, j, k, nu, om, dPdx, dPdz, true, j <= ny - 1, true, ru, rv, rw);
+            const int c = a.slot[1] * PL + a.lj * LX + a.li;
+            const R cu = lu[c], cv = lv[c], cw = lw[c];
+            const int64_t qu = i3(nx, nyg, i, j, k), qv = i3(nxg, ny, i, j, k), qw = i3(nxg, nyg, i, j, k);
+            if (SUB == 1) {
+                Uo[qu] = (S)(cu + cnew * ru);
+                Gu[qu] = (S)(cu + cg * ru);
+                if (j <= ny - 1) {
+                    Vo[qv] = (S)(cv + cnew * rv);
+                    Gv[qv] = (S)(cv + cg * rv);
+                }
+                Wo[qw] = (S)(cw + cnew * rw);
+                Gw[qw] = (S)(cw + cg * rw);
+            } else {
+                Uo[qu] = (S)((R)Gu[qu] + cnew * ru);
+                if (j <= ny - 1) Vo[qv] = (S)((R)Gv[qv] + cnew * rv);
+                Wo[qw] = (S)((R)Gw[qw] + cnew * rw);
+            }
+        }
+        // plane k+2 into the slot of plane k-2, then prefetch plane k+3
+        if (k + 2 <= kend) {
+            store((r0 + 3) & 3);
+            if (k + 3 <= kend) fetch(k + 3);
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // RHS only (compute_rhs_u/v/w as separate outputs on the interior ranges of
 // the reference, equations.f90:29,191,356), used by the module-level entry
-// points.  Same arithmetic as the fused kernel with dt = 0 paths removed.
+// points.  Same point arithmetic as the fused kernel.
 // ---------------------------------------------------------------------------
 template <typename S, typename R, bool NUT, bool ROT>
 __global__ void __launch_bounds__(256)
 k_rhs_only(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const S *__restrict__ W, const S *__restrict__ NT, double *__restrict__ Ru,
            double *__restrict__ Rv, double *__restrict__ Rw, const Scalars *__restrict__ sc,
-           double nu_d, double om_d) {
+           const double *__restrict__ ubar, double nu_d, double om_d) {
     // covers i in 2..nxg-1, j in 2..nyg-1, k in 2..nzg-1 (union of the three
     // interior boxes, duplicates included, as the reference computes them)
     const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
     const int k = 2 + blockIdx.z;
     if (i > d.nxg - 1 || j > d.nyg - 1) return;
-    const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nz = d.nz, nzg = d.nzg;
-    const R nu = (R)nu_d, half = (R)0.5, quart = (R)0.25, two = (R)2;
-#define u(I, J, K) ((R)U[i3(nx, nyg, I, J, K)])
-#define v(I, J, K) ((R)V[i3(nxg, ny, I, J, K)])
-#define w(I, J, K) ((R)W[i3(nxg, nyg, I, J, K)])
-#define nt(I, J, K) ((R)NT[i3(nxg, nyg, I, J, K)])
-    const R w0j = (R)m.w0[j], w1j = (R)m.w1[j];
-    const R w0m = (R)m.w0[j - 1], w1m = (R)m.w1[j - 1];
-    if (i <= nx - 1 && Ru) {
-        R c1 = half * (u(i, j, k) + u(i + 1, j, k));
-        R c0 = half * (u(i - 1, j, k) + u(i, j, k));
-        R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdxc[i];
-        R uv1 = (w0j * u(i, j, k) + w1j * u(i, j + 1, k)) * (half * (v(i, j, k) + v(i + 1, j, k)));
-        R uv0 = (w0m * u(i, j - 1, k) + w1m * u(i, j, k)) * (half * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
-        R t2 = -(uv1 - uv0) * (R)m.rdyf[j];
-        R uw1 = (half * (u(i, j, k) + u(i, j, k + 1))) * (half * (w(i, j, k) + w(i + 1, j, k)));
-        R uw0 = (half * (u(i, j, k - 1) + u(i, j, k))) * (half * (w(i, j, k - 1) + w(i + 1, j, k - 1)));
-        R t3 = -(uw1 - uw0) * (R)m.rdzf[k];
-        R rdz1 = (R)m.rdzc[k - 1], rdz2 = (R)m.rdzc[k], rdz3 = (R)m.rdzf[k];
-        R rdy3 = (R)m.rdyf[j];
-        R rdx1 = (R)m.rdxf[i], rdx2 = (R)m.rdxf[i + 1], rdx3 = (R)m.rdxc[i];
-        R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
-        if (NUT) {
-            nux1 = nu + nt(i, j, k);
-            nux2 = nu + nt(i + 1, j, k);
-            nuy1 = nu + half * (w0m * nt(i, j - 1, k) + w1m * nt(i, j, k) + w0m * nt(i + 1, j - 1, k) + w1m * nt(i + 1, j, k));
-            nuy2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i + 1, j, k) + w1j * nt(i + 1, j + 1, k));
-            nuz1 = nu + quart * (nt(i, j, k) + nt(i, j, k - 1) + nt(i + 1, j, k) + nt(i + 1, j, k - 1));
-            nuz2 = nu + quart * (nt(i, j, k) + nt(i, j, k + 1) + nt(i + 1, j, k) + nt(i + 1, j, k + 1));
-        }
-        R dudy1 = (u(i, j + 1, k) - u(i, j, k)) * (R)m.rdyc[j];
-        R dudy0 = (u(i, j, k) - u(i, j - 1, k)) * (R)m.rdyc[j - 1];
-        R visc = two * rdx3 * (nux2 * (u(i + 1, j, k) - u(i, j, k)) * rdx2 - nux1 * (u(i, j, k) - u(i - 1, j, k)) * rdx1) +
-                 rdy3 * (nuy2 * (dudy1 + (v(i + 1, j, k) - v(i, j, k)) * rdx2) -
-                         nuy1 * (dudy0 + (v(i + 1, j - 1, k) - v(i, j - 1, k)) * rdx1)) +
-                 rdz3 * (nuz2 * ((u(i, j, k + 1) - u(i, j, k)) * rdz2 + (w(i + 1, j, k) - w(i, j, k)) * rdx2) -
-                         nuz1 * ((u(i, j, k) - u(i, j, k - 1)) * rdz1 + (w(i + 1, j, k - 1) - w(i, j, k - 1)) * rdx1));
-        R r = t1 + t2;
-        r = r + t3;
-        r = r + visc;
-        r = r + (R)sc->dPdx;
-        if (ROT) {
-            R a = half * (v(i, j - 1, k) + v(i, j, k));
-            R b = half * (v(i + 1, j - 1, k) + v(i + 1, j, k));
-            r = r + two * (R)om_d * (half * (a + b));
-        }
-        Ru[i3(nx - 2, nyg - 2, i - 1, j - 1, k - 1)] = (double)r;
-    }
-    if (j <= ny - 1 && Rv) {
-        R c1 = half * (v(i, j, k) + v(i, j + 1, k));
-        R c0 = half * (v(i, j - 1, k) + v(i, j, k));
-        R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdyc[j];
-        R uv1 = (w0j * u(i, j, k) + w1j * u(i, j + 1, k)) * (half * (v(i, j, k) + v(i + 1, j, k)));
-        R uv0 = (w0j * u(i - 1, j, k) + w1j * u(i - 1, j + 1, k)) * (half * (v(i - 1, j, k) + v(i, j, k)));
-        R t2 = -(uv1 - uv0) * (R)m.rdxf[i];
-        R vw1 = (half * (v(i, j, k) + v(i, j, k + 1))) * (w0j * w(i, j, k) + w1j * w(i, j + 1, k));
-        R vw0 = (half * (v(i, j, k - 1) + v(i, j, k))) * (w0j * w(i, j, k - 1) + w1j * w(i, j + 1, k - 1));
-        R t3 = -(vw1 - vw0) * (R)m.rdzf[k];
-        R rdz1 = (R)m.rdzc[k - 1], rdz2 = (R)m.rdzc[k], rdz3 = (R)m.rdzf[k];
-        R rdy1 = (R)m.rdyf[j], rdy2 = (R)m.rdyf[j + 1], rdy3 = (R)m.rdyc[j];
-        R rdx1 = (R)m.rdxc[i - 1], rdx2 = (R)m.rdxc[i], rdx3 = (R)m.rdxf[i];
-        R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
-        if (NUT) {
-            nux1 = nu + half * (w0j * nt(i - 1, j, k) + w1j * nt(i - 1, j + 1, k) + w0j * nt(i, j, k) + w1j * nt(i, j + 1, k));
-            nux2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i + 1, j, k) + w1j * nt(i + 1, j + 1, k));
-            nuy1 = nu + nt(i, j, k);
-            nuy2 = nu + nt(i, j + 1, k);
-            nuz1 = nu + half * (w0j * nt(i, j, k - 1) + w1j * nt(i, j + 1, k - 1) + w0j * nt(i, j, k) + w1j * nt(i, j + 1, k));
-            nuz2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i, j, k + 1) + w1j * nt(i, j + 1, k + 1));
-        }
-        R visc = rdx3 * (nux2 * ((v(i + 1, j, k) - v(i, j, k)) * rdx2 + (u(i, j + 1, k) - u(i, j, k)) * rdy3) -
-                         nux1 * ((v(i, j, k) - v(i - 1, j, k)) * rdx1 + (u(i - 1, j + 1, k) - u(i - 1, j, k)) * rdy3)) +
-                 two * rdy3 * (nuy2 * (v(i, j + 1, k) - v(i, j, k)) * rdy2 - nuy1 * (v(i, j, k) - v(i, j - 1, k)) * rdy1) +
-                 rdz3 * (nuz2 * ((v(i, j, k + 1) - v(i, j, k)) * rdz2 + (w(i, j + 1, k) - w(i, j, k)) * rdy3) -
-                         nuz1 * ((v(i, j, k) - v(i, j, k - 1)) * rdz1 + (w(i, j + 1, k - 1) - w(i, j, k - 1)) * rdy3));
-        R r = t1 + t2;
-        r = r + t3;
-        r = r + visc;
-        if (ROT) {
-            R a = half * (u(i - 1, j, k) + u(i, j, k));
-            R b = half * (u(i - 1, j + 1, k) + u(i, j + 1, k));
-            r = r - two * (R)om_d * (w0j * a + w1j * b);
-        }
-        Rv[i3(nxg - 2, ny - 2, i - 1, j - 1, k - 1)] = (double)r;
-    }
-    if (k <= nz - 1 && Rw) {
-        R c1 = half * (w(i, j, k) + w(i, j, k + 1));
-        R c0 = half * (w(i, j, k - 1) + w(i, j, k));
-        R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdzc[k];
-        R uw1 = (half * (u(i, j, k) + u(i, j, k + 1))) * (half * (w(i, j, k) + w(i + 1, j, k)));
-        R uw0 = (half * (u(i - 1, j, k) + u(i - 1, j, k + 1))) * (half * (w(i - 1, j, k) + w(i, j, k)));
-        R t2 = -(uw1 - uw0) * (R)m.rdxf[i];
-        R vw1 = (half * (v(i, j, k) + v(i, j, k + 1))) * (w0j * w(i, j, k) + w1j * w(i, j + 1, k));
-        R vw0 = (half * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (w0m * w(i, j - 1, k) + w1m * w(i, j, k));
-        R t3 = -(vw1 - vw0) * (R)m.rdyf[j];
-        R rdz1 = (R)m.rdzf[k], rdz2 = (R)m.rdzf[k + 1], rdz3 = (R)m.rdzc[k];
-        R rdy3 = (R)m.rdyf[j];
-        R rdx1 = (R)m.rdxc[i - 1], rdx2 = (R)m.rdxc[i], rdx3 = (R)m.rdxf[i];
-        R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
-        if (NUT) {
-            nux1 = nu + quart * (nt(i, j, k) + nt(i, j, k + 1) + nt(i - 1, j, k) + nt(i - 1, j, k + 1));
-            nux2 = nu + quart * (nt(i, j, k) + nt(i, j, k + 1) + nt(i + 1, j, k) + nt(i + 1, j, k + 1));
-            nuy1 = nu + half * (w0m * nt(i, j - 1, k) + w1m * nt(i, j, k) + w0m * nt(i, j - 1, k + 1) + w1m * nt(i, j, k + 1));
-            nuy2 = nu + half * (w0j * nt(i, j, k) + w1j * nt(i, j + 1, k) + w0j * nt(i, j, k + 1) + w1j * nt(i, j + 1, k + 1));
-            nuz1 = nu + nt(i, j, k);
-            nuz2 = nu + nt(i, j, k + 1);
-        }
-        R dwdy1 = (w(i, j + 1, k) - w(i, j, k)) * (R)m.rdyc[j];
-        R dwdy0 = (w(i, j, k) - w(i, j - 1, k)) * (R)m.rdyc[j - 1];
-        R visc = rdx3 * (nux2 * ((w(i + 1, j, k) - w(i, j, k)) * rdx2 + (u(i, j, k + 1) - u(i, j, k)) * rdz2) -
-                         nux1 * ((w(i, j, k) - w(i - 1, j, k)) * rdx1 + (u(i - 1, j, k + 1) - u(i - 1, j, k)) * rdz1)) +
-                 rdy3 * (nuy2 * (dwdy1 + (v(i, j, k + 1) - v(i, j, k)) * rdz2) -
-                         nuy1 * (dwdy0 + (v(i, j - 1, k + 1) - v(i, j - 1, k)) * rdz1)) +
-                 two * rdz3 * (nuz2 * (w(i, j, k + 1) - w(i, j, k)) * rdz2 - nuz1 * (w(i, j, k) - w(i, j, k - 1)) * rdz1);
-        R r = t1 + t2;
-        r = r + t3;
-        r = r + visc;
-        r = r + (R)sc->dPdz;
-        Rw[i3(nxg - 2, nyg - 2, i - 1, j - 1, k - 1)] = (double)r;
-    }
-    (void)nzg;
-#undef u
-#undef v
-#undef w
-#undef nt
+    GAcc<S, R> a{U, V, W, NT, ubar, i, j, k, d.nx, d.nxg, d.nyg, d.ny};
+    const bool du = i <= d.nx - 1 && Ru, dv = j <= d.ny - 1 && Rv, dw = k <= d.nz - 1 && Rw;
+    R ru = 0, rv = 0, rw = 0;
+    rhs_point<R, NUT, ROT>(a, m, i, j, k, (R)nu_d, (R)om_d, (R)sc->dPdx, (R)sc->dPdz, du, dv, dw, ru, rv,
+                           rw);
+    if (du) Ru[i3(d.nx - 2, d.nyg - 2, i - 1, j - 1, k - 1)] = (double)ru;
+    if (dv) Rv[i3(d.nxg - 2, d.ny - 2, i - 1, j - 1, k - 1)] = (double)rv;
+    if (dw) Rw[i3(d.nxg - 2, d.nyg - 2, i - 1, j - 1, k - 1)] = (double)rw;
 }
 
 // ---------------------------------------------------------------------------
@@ -442,36 +445,59 @@ k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
 // ---------------------------------------------------------------------------
 template <typename T, typename Op>
 __device__ T block_reduce(T v, Op op, T *sh) {
+    // linear thread id over 1-D or 2-D blocks; result valid in thread (0,0)
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nthr = blockDim.x * blockDim.y;
     for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_down(v, o, 64));
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = tid & 63, wid = tid >> 6;
     if (lane == 0) sh[wid] = v;
     __syncthreads();
-    const int nw = (blockDim.x + 63) >> 6;
+    const int nw = (nthr + 63) >> 6;
     if (wid == 0) {
         v = lane < nw ? sh[lane] : sh[0];
-        if (lane >= nw) v = sh[0];
         for (int o = 32; o > 0; o >>= 1) {
             T other = __shfl_down(v, o, 64);
             if (lane + o < nw) v = op(v, other);
         }
     }
     __syncthreads();
-    return v;  // valid in thread 0
+    return v;
 }
 struct OpAdd { __device__ double operator()(double a, double b) const { return a + b; } };
 struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
 
-// mass_flow.f90:116-123: per-plane partial sums of the U trapezoid
+// mass_flow.f90:116-123: partial sums of the U trapezoid over k = 2..nzg-1
+// (the unique planes plus the duplicate plane nzg-1, which the reference
+// includes), one deterministic partial per workgroup tile (64 x 8 x 16).
+constexpr int MF_TX = 64, MF_TY = 8, MF_KC = 16;
+// one thread per (i, j) column segment of MF_KC planes (i = 2..nx-1,
+// j = 3..nyg-1, k = 2..nzg-1), loads of the segment issued together; one
+// deterministic partial per 256-thread block
 template <typename S>
 __global__ void __launch_bounds__(256)
-k_massflow_u_partial(Dims d, Metrics m, const S *__restrict__ U, double *__restrict__ part) {
+k_massflow_u_partial(Dims d, Metrics m, const S *__restrict__ U, const double *__restrict__ ubar,
+                     double *__restrict__ part) {
     __shared__ double sh[8];
-    const int k = 2 + blockIdx.x;  // unique planes 2..nzg-2
+    const int ni = d.nx - 2, nj = d.nyg - 3, nk = d.nzg - 2;
+    const int nch = (nk + MF_KC - 1) / MF_KC;
+    const int64_t n = (int64_t)ni * nj * nch;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
-    const int ni = d.nx - 2, nj = d.nyg - 3;
-    for (int q = threadIdx.x; q < ni * nj; q += blockDim.x) {
-        const int i = 2 + q % ni, j = 3 + q / ni;
-        acc += ((double)U[i3(d.nx, d.nyg, i, j, k)] + (double)U[i3(d.nx, d.nyg, i, j - 1, k)]) * 0.5 * m.dyg[j];
+    if (g < n) {
+        const int i = 2 + (int)(g % ni);
+        const int j = 3 + (int)((g / ni) % nj);
+        const int kb = 2 + (int)(g / ((int64_t)ni * nj)) * MF_KC;
+        const double wj = 0.5 * m.dyg[j];
+        double v[MF_KC];
+#pragma unroll
+        for (int q = 0; q < MF_KC; q++) {
+            const int k = kb + q;
+            v[q] = k <= d.nzg - 1 ? (double)U[i3(d.nx, d.nyg, i, j, k)] + (double)U[i3(d.nx, d.nyg, i, j - 1, k)]
+                                  : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < MF_KC; q++) acc += v[q] * wj;
     }
     acc = block_reduce(acc, OpAdd(), sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -479,14 +505,14 @@ k_massflow_u_partial(Dims d, Metrics m, const S *__restrict__ U, double *__restr
 
 // final: Int_U over k = 2..nzg-1, i.e. the unique planes plus the duplicate
 // plane nzg-1 (= plane 2 after the periodic copy, mass_flow.f90:116)
-__global__ void k_massflow_u_final(const double *__restrict__ part, int nplanes,
+__global__ void k_massflow_u_final(const double *__restrict__ part, int nparts,
                                    Scalars *sc, int apply) {
     __shared__ double sh[8];
     double acc = 0.0;
-    for (int q = threadIdx.x; q < nplanes; q += blockDim.x) acc += part[q];
+    for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += part[q];
     acc = block_reduce(acc, OpAdd(), sh);
     if (threadIdx.x == 0) {
-        const double total = acc + part[0];
+        const double total = acc + sc->qbar_u;
         const double mean = total / sc->norm_u;
         sc->pad[0] = mean;
         if (apply) {
@@ -510,52 +536,115 @@ k_add_corr_u(Dims d, S *__restrict__ U, const Scalars *__restrict__ sc) {
     U[q] = (S)((double)U[q] + sc->corr);
 }
 
-// time_integration.f90:188-199: maxima of spacing/|u| over the reference's
-// ranges (ghost duplicates included); max is order independent.
+// time_integration.f90:188-199.  max over cells of spacing/|u| equals, per
+// line of constant spacing, spacing / min|u| (correctly rounded division is
+// monotone), so the kernel only takes minima: of |U| per i, |V| per j, |W|
+// per k, over the reference's ranges, reduced per tile into partial arrays and
+// then per line (no atomics, order independent).  Duplicate periodic planes
+// hold copies of unique ones; the wall row V(:,ny,:) is read.
 template <typename S>
-__global__ void __launch_bounds__(256)
-k_dt_partial(Dims d, const double *__restrict__ dxc, const double *__restrict__ dyc,
-             const double *__restrict__ dzc, const S *__restrict__ U, const S *__restrict__ V,
-             const S *__restrict__ W, double *__restrict__ part) {
-    __shared__ double sh[8];
-    const int k = 2 + blockIdx.x;  // 2..nzg-1
-    double mu = 0.0, mv = 0.0, mw = 0.0;
-    const int ni = d.nxg - 2, nj = d.nyg - 2;
-    for (int q = threadIdx.x; q < ni * nj; q += blockDim.x) {
-        const int i = 2 + q % ni, j = 2 + q / ni;
-        mu = fmax(mu, dxc[i] / fabs((double)U[i3(d.nx, d.nyg, i, j, k)]));
-        mv = fmax(mv, dyc[j] / fabs((double)V[i3(d.nxg, d.ny, i, j, k)]));
-        mw = fmax(mw, dzc[k] / fabs((double)W[i3(d.nxg, d.nyg, i, j, k)]));
+__global__ void __launch_bounds__(MF_TX *MF_TY)
+k_dt_min(Dims d, const S *__restrict__ U, const S *__restrict__ V, const S *__restrict__ W,
+         const double *__restrict__ ubar, double *__restrict__ pmin, int tiles_x, int tiles_y,
+         int nch) {
+    __shared__ double shu[MF_TY][MF_TX];
+    __shared__ double shv[MF_TY][MF_TX];
+    __shared__ double shw[MF_KC][MF_TY];
+    const int t = blockIdx.x;
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tc = t / (tiles_x * tiles_y);
+    const int i = 2 + tx * MF_TX + threadIdx.x;
+    const int j = 2 + ty * MF_TY + threadIdx.y;
+    const int kb = 2 + tc * MF_KC;
+    const bool act = i <= d.nx - 1 && j <= d.nyg - 1;
+    const double ub = (act && ubar) ? ubar[j] : 0.0;
+    double mu = INFINITY, mv = INFINITY;
+#pragma unroll 4
+    for (int kk = 0; kk < MF_KC; kk++) {
+        const int k = kb + kk;
+        double mwk = INFINITY;
+        if (act && k <= d.nzg - 2) {
+            mu = fmin(mu, fabs((double)U[i3(d.nx, d.nyg, i, j, k)] + ub));
+            mv = fmin(mv, fabs((double)V[i3(d.nxg, d.ny, i, j, k)]));  // j <= nyg-1 = ny
+            mwk = fabs((double)W[i3(d.nxg, d.nyg, i, j, k)]);
+        }
+        for (int o = 32; o > 0; o >>= 1) mwk = fmin(mwk, __shfl_down(mwk, o, 64));
+        if (threadIdx.x == 0) shw[kk][threadIdx.y] = mwk;
     }
-    mu = block_reduce(mu, OpMax(), sh);
-    mv = block_reduce(mv, OpMax(), sh);
-    mw = block_reduce(mw, OpMax(), sh);
-    if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = mu;
-        part[3 * blockIdx.x + 1] = mv;
-        part[3 * blockIdx.x + 2] = mw;
+    shu[threadIdx.y][threadIdx.x] = mu;
+    shv[threadIdx.y][threadIdx.x] = mv;
+    __syncthreads();
+    // per-line minima of this tile -> partial arrays (no atomics)
+    const int nyz = tiles_y * nch, nxz = tiles_x * nch, nxy = tiles_x * tiles_y;
+    double *pu = pmin;                                   // [nyz][nxp]
+    double *pv = pu + (size_t)nyz * d.nxp;               // [nxz][nyp]
+    double *pw = pv + (size_t)nxz * d.nyp;               // [nxy][nzp]
+    const int tid = threadIdx.x + MF_TX * threadIdx.y;
+    if (tid < MF_TX) {
+        double r = INFINITY;
+        for (int q = 0; q < MF_TY; q++) r = fmin(r, shu[q][tid]);
+        const int ii = tx * MF_TX + tid;
+        if (ii < d.nxp) pu[(size_t)(ty + tiles_y * tc) * d.nxp + ii] = r;
+    } else if (tid < MF_TX + MF_TY) {
+        const int q = tid - MF_TX;
+        double r = INFINITY;
+        for (int c = 0; c < MF_TX; c++) r = fmin(r, shv[q][c]);
+        const int jj = ty * MF_TY + q;
+        if (jj < d.nyp) pv[(size_t)(tx + tiles_x * tc) * d.nyp + jj] = r;
+    } else if (tid < MF_TX + MF_TY + MF_KC) {
+        const int q = tid - MF_TX - MF_TY;
+        double r = INFINITY;
+        for (int c = 0; c < MF_TY; c++) r = fmin(r, shw[q][c]);
+        const int kk = tc * MF_KC + q;
+        if (kk < d.nzp) pw[(size_t)(tx + tiles_x * ty) * d.nzp + kk] = r;
     }
 }
 
-__global__ void k_dt_final(const double *__restrict__ part, int nplanes, Scalars *sc,
-                           double cfl, double dt_vis) {
+// one workgroup per line: minimum over the tiles of the partial arrays
+__global__ void __launch_bounds__(256)
+k_dt_lines(Dims d, const double *__restrict__ pmin, double *__restrict__ mins, int tiles_x,
+           int tiles_y, int nch) {
     __shared__ double sh[8];
-    double mu = 0.0, mv = 0.0, mw = 0.0;
-    for (int q = threadIdx.x; q < nplanes; q += blockDim.x) {
-        mu = fmax(mu, part[3 * q]);
-        mv = fmax(mv, part[3 * q + 1]);
-        mw = fmax(mw, part[3 * q + 2]);
+    const int nyz = tiles_y * nch, nxz = tiles_x * nch, nxy = tiles_x * tiles_y;
+    const double *pu = pmin, *pv = pu + (size_t)nyz * d.nxp, *pw = pv + (size_t)nxz * d.nyp;
+    const int q = blockIdx.x;
+    double r = INFINITY;
+    if (q < d.nxp) {
+        for (int t = threadIdx.x; t < nyz; t += 256) r = fmin(r, pu[(size_t)t * d.nxp + q]);
+    } else if (q < d.nxp + d.nyp) {
+        const int jj = q - d.nxp;
+        for (int t = threadIdx.x; t < nxz; t += 256) r = fmin(r, pv[(size_t)t * d.nyp + jj]);
+    } else {
+        const int kk = q - d.nxp - d.nyp;
+        for (int t = threadIdx.x; t < nxy; t += 256) r = fmin(r, pw[(size_t)t * d.nzp + kk]);
     }
-    mu = block_reduce(mu, OpMax(), sh);
-    mv = block_reduce(mv, OpMax(), sh);
-    mw = block_reduce(mw, OpMax(), sh);
+    r = block_reduce(r, OpMin(), sh);
+    if (threadIdx.x == 0) mins[q] = r;
+}
+
+// dt = min(CFL*max(spacing/min|u|) over the three components, viscous, 5e-3)
+__global__ void k_dt_final(Dims d, const double *__restrict__ dxc, const double *__restrict__ dyc,
+                           const double *__restrict__ dzc, const double *__restrict__ mins,
+                           Scalars *sc, double cfl, double dt_vis) {
+    __shared__ double sh[8];
+    const double *mu = mins, *mv = mins + d.nxp, *mw = mv + d.nyp;  // index i-2, j-2, k-2
+    double lu = 0.0, lv = 0.0, lw = 0.0;
+    // U: i = 2..nxg-1 (= nx); column nx is the periodic copy of column 2
+    for (int i = 2 + threadIdx.x; i <= d.nxg - 1; i += blockDim.x)
+        lu = fmax(lu, dxc[i] / (i == d.nx ? mu[0] : mu[i - 2]));
+    for (int j = 2 + threadIdx.x; j <= d.nyg - 1; j += blockDim.x) lv = fmax(lv, dyc[j] / mv[j - 2]);
+    // W: k = 2..nzg-1 (= nz); plane nz is the periodic copy of plane 2
+    for (int k = 2 + threadIdx.x; k <= d.nzg - 1; k += blockDim.x)
+        lw = fmax(lw, dzc[k] / (k == d.nz ? mw[0] : mw[k - 2]));
+    lu = block_reduce(lu, OpMax(), sh);
+    lv = block_reduce(lv, OpMax(), sh);
+    lw = block_reduce(lw, OpMax(), sh);
     if (threadIdx.x == 0) {
-        const double dc = fmin(fmin(cfl * mu, cfl * mv), cfl * mw);
+        const double dc = fmin(fmin(cfl * lu, cfl * lv), cfl * lw);
         double dt = fmin(dc, dt_vis);
         dt = fmin(dt, 5e-3);
         if (cfl < 0) dt = -cfl;
         sc->dt = dt;
-        sc->dt_conv[0] = mu; sc->dt_conv[1] = mv; sc->dt_conv[2] = mw;
+        sc->dt_conv[0] = lu; sc->dt_conv[1] = lv; sc->dt_conv[2] = lw;
     }
 }
 
@@ -595,7 +684,8 @@ k_smagorinsky(Dims d, const double *__restrict__ x, const double *__restrict__ y
               const double *__restrict__ yg, const double *__restrict__ zg,
               const double *__restrict__ delta, const S *__restrict__ U, const S *__restrict__ V,
               const S *__restrict__ W, S *__restrict__ NT, const Scalars *__restrict__ sc,
-              double cs, double nu, int damping, int tiles_x, int tiles_y, int ntiles) {
+              const double *__restrict__ ubar, double cs, double nu, int damping, int tiles_x,
+              int tiles_y, int ntiles) {
     const int t = xcd_remap(blockIdx.x, ntiles);
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
     const int i = 2 + tx * 64 + (threadIdx.x & 63);
@@ -603,7 +693,7 @@ k_smagorinsky(Dims d, const double *__restrict__ x, const double *__restrict__ y
     const int k = 2 + tz;
     if (i > d.nx - 1 || j > d.nyg - 1) return;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
-#define u(I, J, K) ((double)U[i3(nx, nyg, I, J, K)])
+#define u(I, J, K) ((double)U[i3(nx, nyg, I, J, K)] + (ubar ? ubar[J] : 0.0))
 #define v(I, J, K) ((double)V[i3(nxg, ny, I, J, K)])
 #define w(I, J, K) ((double)W[i3(nxg, nyg, I, J, K)])
     const double s11 = (u(i, j, k) - u(i - 1, j, k)) / (x[i] - x[i - 1]);
@@ -652,15 +742,16 @@ k_smagorinsky(Dims d, const double *__restrict__ x, const double *__restrict__ y
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_stats_planes(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
-               const S *__restrict__ W, double *__restrict__ out) {
+               const S *__restrict__ W, const double *__restrict__ ubar, double *__restrict__ out) {
     __shared__ double sh[8];
     const int jj = 1 + blockIdx.x;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nz = d.nz, nzg = d.nzg;
     const double a = m.w0[jj], b = m.w1[jj];
+    const double ua = ubar ? ubar[jj] : 0.0, ubb = ubar ? ubar[jj + 1] : 0.0;
     double su = 0, su2 = 0, sw = 0, sw2 = 0, sv = 0, sv2 = 0, suv = 0;
     for (int q = threadIdx.x; q < (nx - 2) * (nzg - 2); q += blockDim.x) {
         const int i = 1 + q % (nx - 2), k = 1 + q / (nx - 2);
-        const double t = a * (double)U[i3(nx, nyg, i, jj, k)] + b * (double)U[i3(nx, nyg, i, jj + 1, k)];
+        const double t = a * ((double)U[i3(nx, nyg, i, jj, k)] + ua) + b * ((double)U[i3(nx, nyg, i, jj + 1, k)] + ubb);
         su += t; su2 += t * t;
     }
     for (int q = threadIdx.x; q < (nxg - 2) * (nz - 2); q += blockDim.x) {
@@ -673,8 +764,8 @@ k_stats_planes(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__
         const double vv = (double)V[i3(nxg, ny, i, jj, k)];
         sv += vv; sv2 += vv * vv;
         const int ii = i - 1;
-        const double ta = a * (double)U[i3(nx, nyg, ii, jj, k)] + b * (double)U[i3(nx, nyg, ii, jj + 1, k)];
-        const double tb = a * (double)U[i3(nx, nyg, ii + 1, jj, k)] + b * (double)U[i3(nx, nyg, ii + 1, jj + 1, k)];
+        const double ta = a * ((double)U[i3(nx, nyg, ii, jj, k)] + ua) + b * ((double)U[i3(nx, nyg, ii, jj + 1, k)] + ubb);
+        const double tb = a * ((double)U[i3(nx, nyg, ii + 1, jj, k)] + ua) + b * ((double)U[i3(nx, nyg, ii + 1, jj + 1, k)] + ubb);
         suv += vv * (0.5 * (ta + tb));
     }
     double vals[7] = {su, sv, sw, su2, sv2, sw2, suv};
@@ -693,19 +784,24 @@ static inline int tiles_y(const Dims &d) { return (d.nyp + 3) / 4; }
 template <typename S, typename R>
 void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                    const S *NT, S *Uo, S *Vo, S *Wo, S *Gu, S *Gv, S *Gw,
-                   const Scalars *sc, double nu, double om, double a_new, double a_g,
-                   int sub, hipStream_t st) {
-    const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * d.nzp;
+                   const Scalars *sc, const double *ubar, double nu, double om, double a_new,
+                   double a_g, int sub, hipStream_t st) {
     const bool nut = NT != nullptr, rot = fabs(om) > 1e-6;
-#define L(NUTB, ROTB, SUBV)                                                                       \
-    hipLaunchKernelGGL((k_rhs_rk<S, R, NUTB, ROTB, SUBV>), dim3(nt), dim3(256), 0, st, d, m, U, V, W, \
-                       NT, Uo, Vo, Wo, Gu, Gv, Gw, sc, nu, om, a_new, a_g, tx, ty, nt)
+    // 64x8 tiles without eddy viscosity, 64x4 with it (the ring of four
+    // fields then stays below 64 KiB of LDS in FP64)
+    const int TYv = nut ? 4 : 8;
+    const int tx = (d.nxp + RK_TX - 1) / RK_TX, ty = (d.nyp + TYv - 1) / TYv;
+    const int nch = (d.nzp + RK_KC - 1) / RK_KC, nt = tx * ty * nch;
+    const size_t shm = (size_t)(nut ? 4 : 3) * RK_NSLOT * (RK_TX + 2) * (TYv + 2) * sizeof(R);
+#define L(NUTB, ROTB, SUBV, TYB)                                                                           \
+    hipLaunchKernelGGL((k_rhs_rk<S, R, NUTB, ROTB, SUBV, TYB>), dim3(nt), dim3(RK_TX, TYB), shm, st, d, m, U, V, \
+                       W, NT, Uo, Vo, Wo, Gu, Gv, Gw, sc, ubar, nu, om, a_new, a_g, tx, ty, nch, nt)
     if (sub == 1) {
-        if (nut) { if (rot) L(true, true, 1); else L(true, false, 1); }
-        else { if (rot) L(false, true, 1); else L(false, false, 1); }
+        if (nut) { if (rot) L(true, true, 1, 4); else L(true, false, 1, 4); }
+        else { if (rot) L(false, true, 1, 8); else L(false, false, 1, 8); }
     } else {
-        if (nut) { if (rot) L(true, true, 2); else L(true, false, 2); }
-        else { if (rot) L(false, true, 2); else L(false, false, 2); }
+        if (nut) { if (rot) L(true, true, 2, 4); else L(true, false, 2, 4); }
+        else { if (rot) L(false, true, 2, 8); else L(false, false, 2, 8); }
     }
 #undef L
 }
@@ -713,12 +809,12 @@ void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, cons
 template <typename S, typename R>
 void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      const S *NT, double *Ru, double *Rv, double *Rw, const Scalars *sc,
-                     double nu, double om, hipStream_t st) {
+                     const double *ubar, double nu, double om, hipStream_t st) {
     dim3 grid((d.nxg - 2 + 63) / 64, (d.nyg - 2 + 3) / 4, d.nzg - 2);
     const bool nut = NT != nullptr, rot = fabs(om) > 1e-6;
 #define L(NUTB, ROTB)                                                                              \
     hipLaunchKernelGGL((k_rhs_only<S, R, NUTB, ROTB>), grid, dim3(256), 0, st, d, m, U, V, W, NT, Ru, Rv, \
-                       Rw, sc, nu, om)
+                       Rw, sc, ubar, nu, om)
     if (nut) { if (rot) L(true, true); else L(true, false); }
     else { if (rot) L(false, true); else L(false, false); }
 #undef L
@@ -748,8 +844,14 @@ void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W
 template <typename S>
 void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, Scalars *sc,
                      int apply, hipStream_t st) {
-    hipLaunchKernelGGL((k_massflow_u_partial<S>), dim3(d.nzp), dim3(256), 0, st, d, m, U, part);
-    hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(256), 0, st, part, d.nzp, sc, apply);
+    const int nb = massflow_parts(d);
+    hipLaunchKernelGGL((k_massflow_u_partial<S>), dim3(nb), dim3(256), 0, st, d, m, U, nullptr, part);
+    hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, part, nb, sc, apply);
+}
+int massflow_parts(const Dims &d) {
+    const int nch = (d.nzg - 2 + MF_KC - 1) / MF_KC;
+    const int64_t n = (int64_t)(d.nx - 2) * (d.nyg - 3) * nch;
+    return (int)((n + 255) / 256);
 }
 
 template <typename S>
@@ -758,13 +860,25 @@ void launch_add_corr_u(const Dims &d, S *U, const Scalars *sc, hipStream_t st) {
     hipLaunchKernelGGL((k_add_corr_u<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, U, sc);
 }
 
+size_t dt_scratch_doubles(const Dims &d) {
+    const int tx = (d.nxp + MF_TX - 1) / MF_TX, ty = (d.nyp + MF_TY - 1) / MF_TY;
+    const int nch = (d.nzp + MF_KC - 1) / MF_KC;
+    return (size_t)ty * nch * d.nxp + (size_t)tx * nch * d.nyp + (size_t)tx * ty * d.nzp +
+           (size_t)(d.nxp + d.nyp + d.nzp);
+}
+
 template <typename S>
 void launch_dt(const Dims &d, const double *dxc, const double *dyc, const double *dzc,
-               const S *U, const S *V, const S *W, double *part, Scalars *sc, double cfl,
-               double dt_vis, hipStream_t st) {
-    const int np = d.nzg - 2;
-    hipLaunchKernelGGL((k_dt_partial<S>), dim3(np), dim3(256), 0, st, d, dxc, dyc, dzc, U, V, W, part);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, part, np, sc, cfl, dt_vis);
+               const S *U, const S *V, const S *W, const double *ubar, double *scratch,
+               Scalars *sc, double cfl, double dt_vis, hipStream_t st) {
+    const int tx = (d.nxp + MF_TX - 1) / MF_TX, ty = (d.nyp + MF_TY - 1) / MF_TY;
+    const int nch = (d.nzp + MF_KC - 1) / MF_KC;
+    double *mins = scratch + (dt_scratch_doubles(d) - (size_t)(d.nxp + d.nyp + d.nzp));
+    hipLaunchKernelGGL((k_dt_min<S>), dim3(tx * ty * nch), dim3(MF_TX, MF_TY), 0, st, d, U, V, W, ubar,
+                       scratch, tx, ty, nch);
+    const int nl = d.nxp + d.nyp + d.nzp;
+    hipLaunchKernelGGL(k_dt_lines, dim3(nl), dim3(256), 0, st, d, scratch, mins, tx, ty, nch);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, st, d, dxc, dyc, dzc, mins, sc, cfl, dt_vis);
 }
 
 template <typename S>
@@ -776,26 +890,27 @@ void launch_div(const Dims &d, const double *x, const double *y, const double *z
 template <typename S>
 void launch_smagorinsky(const Dims &d, const double *x, const double *y, const double *z,
                         const double *xg, const double *yg, const double *zg, const double *delta,
-                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc, double cs,
-                        double nu, int damping, hipStream_t st) {
+                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc,
+                        const double *ubar, double cs, double nu, int damping, hipStream_t st) {
     const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * d.nzp;
     hipLaunchKernelGGL((k_smagorinsky<S>), dim3(nt), dim3(256), 0, st, d, x, y, z, xg, yg, zg, delta,
-                       U, V, W, NT, sc, cs, nu, damping, tx, ty, nt);
+                       U, V, W, NT, sc, ubar, cs, nu, damping, tx, ty, nt);
 }
 
 template <typename S>
-void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W, double *out,
-                  hipStream_t st) {
-    hipLaunchKernelGGL((k_stats_planes<S>), dim3(d.ny), dim3(256), 0, st, d, m, U, V, W, out);
+void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
+                  const double *ubar, double *out, hipStream_t st) {
+    hipLaunchKernelGGL((k_stats_planes<S>), dim3(d.ny), dim3(256), 0, st, d, m, U, V, W, ubar, out);
 }
 
 #define INST(S, R)                                                                                    \
     template void launch_rhs_rk<S, R>(const Dims &, const Metrics &, const S *, const S *, const S *, \
                                       const S *, S *, S *, S *, S *, S *, S *, const Scalars *,       \
-                                      double, double, double, double, int, hipStream_t);              \
+                                      const double *, double, double, double, double, int,           \
+                                      hipStream_t);                                                   \
     template void launch_rhs_only<S, R>(const Dims &, const Metrics &, const S *, const S *,          \
                                         const S *, const S *, double *, double *, double *,          \
-                                        const Scalars *, double, double, hipStream_t);
+                                        const Scalars *, const double *, double, double, hipStream_t);
 INST(double, double)
 INST(float, float)
 #undef INST
@@ -806,16 +921,17 @@ INST(float, float)
                                      int, hipStream_t);                                                \
     template void launch_add_corr_u<S>(const Dims &, S *, const Scalars *, hipStream_t);              \
     template void launch_dt<S>(const Dims &, const double *, const double *, const double *,          \
-                               const S *, const S *, const S *, double *, Scalars *, double, double,  \
-                               hipStream_t);                                                           \
+                               const S *, const S *, const S *, const double *, double *,             \
+                               Scalars *, double, double, hipStream_t);                                \
     template void launch_div<S>(const Dims &, const double *, const double *, const double *,         \
                                 const S *, const S *, const S *, double *, hipStream_t);              \
     template void launch_smagorinsky<S>(const Dims &, const double *, const double *, const double *, \
                                         const double *, const double *, const double *,               \
                                         const double *, const S *, const S *, const S *, S *,         \
-                                        const Scalars *, double, double, int, hipStream_t);           \
+                                        const Scalars *, const double *, double, double, int,         \
+                                        hipStream_t);                                                  \
     template void launch_stats<S>(const Dims &, const Metrics &, const S *, const S *, const S *,     \
-                                  double *, hipStream_t);
+                                  const double *, double *, hipStream_t);
 INST1(double)
 INST1(float)
 #undef INST1
@@ -838,6 +954,21 @@ __global__ void k_reset_counters(Scalars *sc) {
 __global__ void k_dt_fixed(Scalars *sc, double cfl) { sc->dt = -cfl; }
 // t = to + rk_t(3)*dt with rk_t(3) = 1 (time_integration.f90:146)
 __global__ void k_advance_time(Scalars *sc) { sc->t = sc->t + 1.0 * sc->dt; }
+
+// mass-flow correction in the deviation storage: U = ubar(j) + U', and the
+// correction adds corr to U on the interior (time_integration.f90:155), i.e.
+// to ubar(2..nyg-1); the ghost entries follow the wall condition the ghost
+// kernel applies to U' (Dirichlet: antisymmetric, Neumann: symmetric).
+__global__ void k_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann) {
+    const double c = sc->corr;
+    for (int j = 2; j <= nyg - 1; j++) ubar[j] += c;
+    ubar[1] = neumann ? ubar[2] : -ubar[2];
+    ubar[nyg] = neumann ? ubar[nyg - 1] : -ubar[nyg - 1];
+    sc->qbar_u = sc->qbar_u + c * sc->norm_u;
+}
+void launch_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann, hipStream_t st) {
+    hipLaunchKernelGGL(k_ubar_shift, dim3(1), dim3(1), 0, st, sc, ubar, nyg, neumann);
+}
 
 void launch_reset_counters(Scalars *sc, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, sc);
