@@ -63,8 +63,10 @@ def test_fortran_driver_matches_python_host(tmp_path):
     U, V, W = cases.initial_field(cfg)
     g.initialise(U, V, W)
     g.run(6, graph=True)
+    # the driver builds its own tanh grid (flang's libm vs numpy: 1 ulp)
+    np.testing.assert_allclose(snap["y"], g.y, rtol=0, atol=4e-16)
     for name, a in zip("UVW", g.get_state()):
-        np.testing.assert_array_equal(snap[name], a)
+        assert np.max(np.abs(snap[name] - a)) <= 1e-12 * np.max(np.abs(a))
     t, dPdx, q0 = snap["trailer"]
     assert t == pytest.approx(6 * 2e-3, rel=1e-14)
     # restart from the snapshot for 4 more steps == 10 steps straight
@@ -73,6 +75,6 @@ def test_fortran_driver_matches_python_host(tmp_path):
     snap2 = read_snapshot(tmp_path / "r2" / "cont.4")
     g.run(4, graph=True)
     for name, a in zip("UVW", g.get_state()):
-        np.testing.assert_array_equal(snap2[name], a)
+        assert np.max(np.abs(snap2[name] - a)) <= 1e-12 * np.max(np.abs(a))
     _, dmax = g.check_divergence()
     assert dmax < 1e-10
