@@ -9,6 +9,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libcfhip.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class CfParams(ctypes.Structure):
@@ -60,6 +61,19 @@ SIGNATURES = {
     "cf_eqwm_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      _dp, _ip]),
+    "cf_mlp_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]),
+    "cf_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_mlp_nparams": (ctypes.c_int, []),
+    "cf_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "cf_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "cf_mlp_get_norm": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp]),
+    "cf_mlp_set_norm": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp]),
+    "cf_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_float, ctypes.c_float, _fp]),
+    "cf_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _fp, _fp]),
+    "cf_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _ip]),
+    "cf_set_wall_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "cf_wall_fallbacks": (ctypes.c_int, [ctypes.c_void_p]),
     "cf_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "cf_timing_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     "cf_timing_reset": (ctypes.c_int, [ctypes.c_void_p]),
@@ -102,3 +116,7 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(_ip)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp)

@@ -188,6 +188,20 @@ class ChannelFlow:
         _lib.check(self.lib.cf_compute_wall_model(self.h, ctypes.byref(a), ctypes.byref(b)), "compute_wall_model")
         return a.value, b.value
 
+    def set_wall_surrogate(self, model):
+        """Use a trained WallMLP in the wall model (None detaches it); points
+        outside its training hull are solved by the EQWM ODE."""
+        _lib.check(self.lib.cf_set_wall_surrogate(self.h, model.h if model is not None else None),
+                   "cf_set_wall_surrogate")
+        self._surrogate = model  # keep the model alive while attached
+
+    def wall_fallbacks(self):
+        """Wall points that the ODE solved in the last wall-model call."""
+        n = self.lib.cf_wall_fallbacks(self.h)
+        if n < 0:
+            _lib.check(n, "cf_wall_fallbacks")
+        return n
+
     def compute_mean_mass_flow_U(self):
         q = ctypes.c_double()
         _lib.check(self.lib.cf_mean_mass_flow_u(self.h, ctypes.byref(q)), "compute_mean_mass_flow_U")

@@ -846,6 +846,40 @@ int Ctx<S>::compute_wall_model(int *n_nan, int *n_iter) {
     return 0;
 }
 
+// learned wall model: predicted stresses where the inputs lie inside the
+// training hull; the ODE solver handles the rest (wallmodel kernel)
+template <typename S>
+int Ctx<S>::attach_mlp(Mlp *m) {
+    HCK(hipSetDevice(dev));
+    invalidate_graphs();
+    if (!m) { surrogate = nullptr; return 0; }
+    const Dims &d = h.d;
+    const int n = d.nx * d.nzg * 2;
+    if (!wall_u) {
+        if (dalloc(&wall_u, n, owned, bytes) || dalloc(&tau_pred, n, owned, bytes) ||
+            dalloc(&use_pred, n, owned, bytes))
+            return -1;
+    }
+    if (m->ensure_wall(n)) return -1;
+    const double hw = h.yg[h.prm.jref_eqwm - 1];
+    surrogate = [this, m, n, hw](Ctx<S> *, S *const *F, const double **pred, const int **use) {
+        tic("mlp_wall");
+        launch_wall_inputs<S>(h.d, F[0], F[2], wt, h.prm.jref_eqwm, ubar, wall_u, stream);
+        m->wall_predict(n, wall_u, hw, h.prm.nu, &sc->dPdx, tau_pred, use_pred, sc, stream);
+        toc();
+        *pred = tau_pred;
+        *use = use_pred;
+    };
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::wall_fallbacks() {
+    Scalars s;
+    if (get_raw_scalars(&s)) return -1;
+    return s.n_fallback_mlp;
+}
+
 template <typename S>
 std::string Ctx<S>::timing_report() {
     collect();
@@ -990,6 +1024,12 @@ int cf_eqwm_batch(int n, const double *umag, const double *nu, const double *h, 
     if (e != hipSuccess) { set_error(std::string("cf_eqwm_batch: ") + hipGetErrorString(e)); return -1; }
     return 0;
 }
+
+int cf_set_wall_surrogate(cf_ctx *c, cf_mlp *m) {
+    CTX_CHECK(c);
+    return c->impl->attach_mlp(m ? cf::mlp_of(m) : nullptr);
+}
+int cf_wall_fallbacks(cf_ctx *c) { CTX_CHECK(c); return c->impl->wall_fallbacks(); }
 
 int cf_timing_enable(cf_ctx *c, int on) { CTX_CHECK(c); c->impl->set_timing(on != 0); return 0; }
 int cf_timing_reset(cf_ctx *c) { CTX_CHECK(c); c->impl->timing_reset(); return 0; }

@@ -238,8 +238,9 @@ void launch_wall_model(const Dims &d, const S *U, const S *W, const S *NT, const
     hipLaunchKernelGGL(k_alpha_z, dim3((n2 + 255) / 256), dim3(256), 0, st, d, azt, az);
 }
 
-// inputs of the wall model at every wall point, for the learned surrogate:
-// out[q] = (u_les, u_x, w_interp) with q as in k_wall_model
+// input of the wall model at every wall point, for the learned surrogate:
+// the wall-parallel speed at the matching height, stored at the index
+// (i, wall, k) of the (nx, 2, nzg) wall arrays that k_wall_model reads
 template <typename S>
 __global__ void k_wall_inputs(Dims d, const S *__restrict__ U, const S *__restrict__ W,
                               const WallTables t, int jref, const double *__restrict__ ubar,
@@ -263,7 +264,7 @@ __global__ void k_wall_inputs(Dims d, const S *__restrict__ U, const S *__restri
     }
     const double wt = wx[0] * t1[0] + wx[1] * t1[1] + wx[2] * t1[2];
     const double uu = (double)U[i3(nx, nyg, i, jw, k)] + (ubar ? ubar[jw] : 0.0);
-    out[q] = sqrt(uu * uu + wt * wt);
+    out[i3(nx, 2, i, wall + 1, k)] = sqrt(uu * uu + wt * wt);
 }
 
 template <typename S>

@@ -12,6 +12,7 @@
 #include "../../include/channel_flow.h"
 #include "cf_eqwm.hpp"
 #include "cf_kernels.hpp"
+#include "cf_mlp.hpp"
 
 namespace cf {
 
@@ -73,6 +74,8 @@ struct CtxBase {
     virtual size_t device_bytes() const = 0;
     virtual int sync() = 0;
     virtual hipStream_t get_stream() const = 0;
+    virtual int attach_mlp(Mlp *m) = 0;
+    virtual int wall_fallbacks() = 0;
 };
 
 template <typename S>
@@ -171,6 +174,10 @@ struct Ctx : CtxBase {
         return 0;
     }
     hipStream_t get_stream() const override { return stream; }
+    double *wall_u = nullptr, *tau_pred = nullptr;
+    int *use_pred = nullptr;
+    int attach_mlp(Mlp *m) override;
+    int wall_fallbacks() override;
 };
 
 }  // namespace cf

@@ -109,6 +109,43 @@ int cf_eqwm_batch(int n, const double *umag, const double *nu, const double *h,
                   const double *tau_in, int ncv, double stretch, double tol, int max_iter,
                   double *tau, int *iter);
 
+/* ---- learned wall model ----
+ * A 4 -> 64 -> 64 -> 64 -> 1 tanh MLP that maps the wall-model inputs
+ * (u_par, h, nu, dP/dx) to the wall stress tau_w.  It stands in for the
+ * per-point ODE solve solve_u_eqwm (eqwm.f90:90-97, called from
+ * wallmodel.f90:629-650) and has no counterpart in the reference itself:
+ * the reference always solves the ODE.  The two 64x64 layers run on BF16
+ * matrix cores with FP32 accumulation; master weights are FP32.  Inputs
+ * outside the box of the training features fall back to the ODE solver. */
+typedef struct cf_mlp cf_mlp;
+int cf_mlp_create(unsigned seed, cf_mlp **out);
+int cf_mlp_destroy(cf_mlp *m);
+/* number of FP32 parameters: W1[4][64] b1[64] W2[64][64] b2 W3[64][64] b3 W4[64] b4
+ * (weights stored [input][output]) */
+int cf_mlp_nparams(void);
+int cf_mlp_get_params(cf_mlp *m, float *w);
+int cf_mlp_set_params(cf_mlp *m, const float *w);
+/* nrm[16]: feature means [0..3], feature std [4..7], ln tau mean [8], std [9];
+ * hull[8]: per-feature min [0..3] and max [4..7] in standardised units */
+int cf_mlp_get_norm(cf_mlp *m, float *nrm, float *hull);
+int cf_mlp_set_norm(cf_mlp *m, const float *nrm, const float *hull);
+/* Fit the normalisation and hull to the data, then `epochs` passes of Adam
+ * over mini-batches of `batch` samples (mean squared error of standardised
+ * ln tau), learning rate decaying exponentially from lr to lr_final.
+ * feat: n x 4 raw (u_par, h, nu, dP/dx); tau: n.  loss_history: epochs
+ * values (loss of the last batch of each epoch), or null. */
+int cf_mlp_train(cf_mlp *m, long long n, const double *feat, const double *tau, int epochs, int batch,
+                 float lr, float lr_final, float *loss_history);
+/* gradient of the batch loss with respect to every parameter (no update) */
+int cf_mlp_gradients(cf_mlp *m, int n, const double *feat, const double *tau, float *grad, float *loss);
+/* tau for n raw feature rows; inside[q] = 1 when row q lies in the hull (may be null) */
+int cf_mlp_predict(cf_mlp *m, long long n, const double *feat, double *tau, int *inside);
+/* Use the model inside cf_compute_wall_model / the RK3 step of a context
+ * (iwall_model = 3); null detaches it.  The model must outlive the use. */
+int cf_set_wall_surrogate(cf_ctx *c, cf_mlp *m);
+/* wall points that fell back to the ODE solver in the last wall-model call */
+int cf_wall_fallbacks(cf_ctx *c);
+
 /* Per-kernel timing with HIP events recorded on the context's stream. */
 int cf_timing_enable(cf_ctx *c, int on);
 /* writes "name avg_ms count\n" lines; returns 0 */
