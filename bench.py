@@ -106,6 +106,35 @@ def cpu_baseline(cells, steps_cap_s=30.0):
             "sample": "%d RK3 step(s) of the %dx%dx%d workload, %.2f s/step" % (n, *cells, sps)}
 
 
+def serving_bench(scale, max_active, budget_gb, n_cases=64, seed=5):
+    """Ensemble of channel cases (seeded Poisson arrivals; grids 64^3,
+    96x64x96, 128x96x128, 192x128x192; Re_tau ~ U[180, 2000]; DNS/WMLES
+    50/50; 200-2000 steps times `scale`) through the serving loop with
+    continuous batching, and the same cases one at a time (a server with a
+    single slot, same arrivals)."""
+    from channel_flow import serving
+    # steps and arrival gaps scale together: the offered load is the full ensemble's
+    wl = serving.ensemble_workload(n=n_cases, seed=seed, steps_scale=scale, mean_gap=0.05 * scale)
+    fields = serving._FieldCache()
+    for cfg, _, _ in wl:
+        fields(cfg)
+    log("serving: %d cases, %d steps in total, fields ready" % (len(wl), sum(n for _, n, _ in wl)))
+    budget = int(budget_gb * 1e9)
+    t0 = time.perf_counter()
+    recs, summ = serving.serve(wl, budget, max_active=max_active, fields=fields)
+    mix = serving.latency_report(recs, summ)
+    log("serving: batched run done in %.1f s: %s" % (time.perf_counter() - t0, mix))
+    t0 = time.perf_counter()
+    recs1, summ1 = serving.serve(wl, budget, max_active=1, fields=fields)
+    seq = serving.latency_report(recs1, summ1)
+    log("serving: one-at-a-time run done in %.1f s: %s" % (time.perf_counter() - t0, seq))
+    return {"workload": "%d cases, Poisson arrivals mean gap 50 ms, steps 200-2000, both x %g" % (n_cases, scale),
+            "max_active": max_active, "hbm_budget_gb": budget_gb,
+            "batched": mix, "one_at_a_time": seq,
+            "throughput_gain": round(mix["mcell_per_s"] / seq["mcell_per_s"], 3),
+            "case_p99_gain": round(seq["case_s_p99"] / mix["case_s_p99"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +143,11 @@ def main():
     ap.add_argument("--cells", type=str, default="384,256,384")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-scale", type=float, default=0.1,
+                    help="steps of the serving ensemble relative to 200-2000 per case")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs a single case per GPU and has no "
@@ -176,6 +210,8 @@ def main():
         gm.close()
     else:
         g64.close()
+    if not a.no_serve:
+        out["serving"] = serving_bench(a.serve_scale, a.serve_slots, a.serve_budget_gb)
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cells)

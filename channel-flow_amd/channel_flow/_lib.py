@@ -74,6 +74,24 @@ SIGNATURES = {
     "cf_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _ip]),
     "cf_set_wall_surrogate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cf_wall_fallbacks": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_stats_accumulate": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_stats_window": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "cf_stats_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_footprint": (ctypes.c_longlong, [ctypes.POINTER(CfParams), _dp, _dp, _dp]),
+    "cf_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "cf_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CfParams), _dp, _dp, _dp, _dp, _dp,
+                                        _dp, ctypes.c_int, ctypes.c_double,
+                                        ctypes.POINTER(ctypes.c_longlong)]),
+    "cf_server_start": (ctypes.c_int, [ctypes.c_void_p]),
+    "cf_server_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "cf_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "cf_server_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _ip, _ip]),
+    "cf_server_result": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _dp, _dp]),
+    "cf_server_case_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, ctypes.c_int]),
+    "cf_server_step_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _fp, ctypes.c_int]),
+    "cf_server_summary": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
     "cf_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "cf_timing_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     "cf_timing_reset": (ctypes.c_int, [ctypes.c_void_p]),

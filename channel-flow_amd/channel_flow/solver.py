@@ -221,6 +221,28 @@ class ChannelFlow:
         names = ["Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean"]
         return {n: out[i * self.ny:(i + 1) * self.ny].copy() for i, n in enumerate(names)}
 
+    def stats_accumulate(self):
+        """Add the current plane statistics to the device window (async)."""
+        _lib.check(self.lib.cf_stats_accumulate(self.h), "cf_stats_accumulate")
+
+    def stats_reset(self):
+        _lib.check(self.lib.cf_stats_reset(self.h), "cf_stats_reset")
+
+    def stats_window(self):
+        """Window averages of the plane statistics, plus the Reynolds stresses
+        uu, vv, ww, uv about the window mean, and the sample count."""
+        out = np.zeros(7 * self.ny)
+        n = ctypes.c_double()
+        _lib.check(self.lib.cf_stats_window(self.h, _lib.dptr(out), ctypes.byref(n)), "cf_stats_window")
+        names = ["Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean"]
+        d = {k: out[i * self.ny:(i + 1) * self.ny].copy() for i, k in enumerate(names)}
+        d["uu"] = d["U2mean"] - d["Umean"] ** 2
+        d["vv"] = d["V2mean"] - d["Vmean"] ** 2
+        d["ww"] = d["W2mean"] - d["Wmean"] ** 2
+        d["uv"] = d["UVmean"] - d["Umean"] * d["Vmean"]
+        d["samples"] = int(n.value)
+        return d
+
     # ---- instrumentation ----
     def timing(self, on=True):
         _lib.check(self.lib.cf_timing_enable(self.h, 1 if on else 0), "cf_timing_enable")

@@ -25,6 +25,7 @@ namespace cf {
 thread_local std::string g_err;
 
 void set_error(const std::string &m) { g_err = m; }
+std::string last_error_copy() { return g_err; }
 
 #define HCK(call)                                                                              \
     do {                                                                                       \
@@ -192,19 +193,33 @@ std::string Host::validate() const {
 // ---------------------------------------------------------------------------
 // Device context
 // ---------------------------------------------------------------------------
+// Device buffers come from hipMalloc, or from a caller-provided slab (a
+// region of a serving arena: bump allocation, no per-case hipMalloc/hipFree)
+// or from a dry slab that only counts the footprint.
 template <typename T>
-static int dalloc(T **p, size_t n, std::vector<void *> &owned, size_t &bytes) {
+static int dalloc(T **p, size_t n, std::vector<void *> &owned, size_t &bytes, Slab *slab) {
+    const size_t b = std::max<size_t>(n, 1) * sizeof(T);
+    if (slab) {
+        const size_t ba = (b + 255) & ~(size_t)255;
+        if (!slab->dry && slab->used + ba > slab->cap) {
+            set_error("device slab exhausted (footprint larger than reserved)");
+            return -1;
+        }
+        *p = slab->dry ? nullptr : (T *)(slab->base + slab->used);
+        slab->used += ba;
+        bytes += ba;
+        return 0;
+    }
     void *q = nullptr;
-    HCK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    HCK(hipMalloc(&q, b));
     owned.push_back(q);
-    bytes += std::max<size_t>(n, 1) * sizeof(T);
+    bytes += b;
     *p = (T *)q;
     return 0;
 }
 template <typename T>
-static int dupload(T **p, const std::vector<T> &h, std::vector<void *> &owned, size_t &bytes) {
-    if (dalloc(p, h.size(), owned, bytes)) return -1;
-    HCK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+static int upload(T *d, const std::vector<T> &h) {
+    HCK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -214,65 +229,70 @@ Ctx<S>::~Ctx() {
     for (auto &g : graphs) if (g) hipGraphExecDestroy(g);
     for (auto &e : evpool) hipEventDestroy(e);
     for (void *p : owned) hipFree(p);
-    if (stream) hipStreamDestroy(stream);
+    if (stream && !stream_ext) hipStreamDestroy(stream);
 }
 
 template <typename S>
 int Ctx<S>::init() {
     const Dims &d = h.d;
-    dev = h.prm.device;
-    HCK(hipSetDevice(dev));
-    HCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const bool dry = slab && slab->dry;
+    // ---- phase 1: every device buffer (a dry slab stops after this) ----
     const size_t nU = (size_t)d.nx * d.nyg * d.nzg, nV = (size_t)d.nxg * d.ny * d.nzg,
                  nW = (size_t)d.nxg * d.nyg * d.nz, nC = (size_t)d.nxg * d.nyg * d.nzg;
     n3[0] = nU; n3[1] = nV; n3[2] = nW;
+    const size_t nspec = (size_t)d.mh * d.nzp * d.nyp;
+    const size_t nphi = (size_t)d.nxp * d.nyp * d.nzp;
+    const int npart = std::max(std::max(3 * d.nzg, 2 * d.nzg), massflow_parts(d)) + 16;
+    const size_t nwx = (size_t)d.nx * 2 * d.nzg, nwz = (size_t)d.nxg * 2 * d.nz;
+    const int M = d.nxp / 2;
+    auto A = [&](auto **p, size_t n) { return dalloc(p, n, owned, bytes, slab); };
     for (int b = 0; b < 2; b++)
         for (int c = 0; c < 3; c++)
-            if (dalloc(&st[b][c], n3[c], owned, bytes)) return -1;
+            if (A(&st[b][c], n3[c])) return -1;
     for (int c = 0; c < 3; c++)
-        if (dalloc(&G[c], n3[c], owned, bytes)) return -1;
-    if (dalloc(&NT, nC, owned, bytes)) return -1;
-    HCK(hipMemset(NT, 0, nC * sizeof(S)));
+        if (A(&G[c], n3[c])) return -1;
+    if (A(&NT, nC) || A(&spec, nspec) || A(&cscr, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
+        A(&part, npart) || A(&dtscr, dt_scratch_doubles(d)) || A(&sc, 1) ||
+        (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, 7 * d.ny + 8) || A(&stats_acc, 7 * d.ny + 8))
+        return -1;
+    if (A(&m_rdxf, h.rdxf.size()) || A(&m_rdxc, h.rdxc.size()) || A(&m_rdyf, h.rdyf.size()) ||
+        A(&m_rdyc, h.rdyc.size()) || A(&m_rdzf, h.rdzf.size()) || A(&m_rdzc, h.rdzc.size()) ||
+        A(&m_w0, h.w0.size()) || A(&m_w1, h.w1.size()) || A(&m_dyg, h.dyg.size()) ||
+        A(&m_dxc, h.dxc.size()) || A(&m_dyc, h.dyc.size()) || A(&m_dzc, h.dzc.size()) ||
+        A(&m_x, h.x1.size()) || A(&m_y, h.y1.size()) || A(&m_z, h.z1.size()) || A(&m_xg, h.xg1.size()) ||
+        A(&m_yg, h.yg1.size()) || A(&m_zg, h.zg1.size()) || A(&m_kxx, h.kxx.size()) ||
+        A(&m_kzz, h.kzz.size()) || A(&m_dd, h.dd.size()) || A(&m_dl, h.dl.size()) || A(&m_du, h.du.size()) ||
+        A(&m_delta, h.delta.size()) || A(&m_jjz, h.jjz.size()) || A(&m_wz, h.wz.size()) ||
+        A(&m_jjx, h.jjx.size()) || A(&m_wx, h.wx.size()))
+        return -1;
+    if (A(&twM, (size_t)M + 1) || A(&twN, (size_t)d.nxp + 1) || A(&twZ, (size_t)d.nzp + 1)) return -1;
+    if (A(&alpha_x, nwx) || A(&alpha_z, nwz) || A(&azt, nwx) || A(&tau_init, nwx)) return -1;
+    if (dry) return 0;
+
+    // ---- phase 2: stream, contents ----
+    dev = h.prm.device;
+    HCK(hipSetDevice(dev));
+    if (!stream) HCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    else stream_ext = true;
+    HCK(hipMemsetAsync(NT, 0, nC * sizeof(S), stream));
     for (int b = 0; b < 2; b++)
-        for (int c = 0; c < 3; c++) HCK(hipMemset(st[b][c], 0, n3[c] * sizeof(S)));
-    const size_t nspec = (size_t)d.mh * d.nzp * d.nyp;
-    if (dalloc(&spec, nspec, owned, bytes)) return -1;
-    if (dalloc(&cscr, nspec, owned, bytes)) return -1;
-    const size_t nphi = (size_t)d.nxp * d.nyp * d.nzp;
-    if (dalloc(&phi1, nphi, owned, bytes)) return -1;
-    if (dalloc(&phi23, nphi, owned, bytes)) return -1;
-    HCK(hipMemset(phi1, 0, nphi * sizeof(S)));
-    const int npart = std::max(std::max(3 * d.nzg, 2 * d.nzg), massflow_parts(d)) + 16;
-    if (dalloc(&part, npart, owned, bytes)) return -1;
-    if (dalloc(&dtscr, dt_scratch_doubles(d), owned, bytes)) return -1;
-    if (dalloc(&sc, 1, owned, bytes)) return -1;
-    if (prec_mixed()) {
-        if (dalloc(&ubar, d.nyg + 2, owned, bytes)) return -1;
-        HCK(hipMemset(ubar, 0, (d.nyg + 2) * sizeof(double)));
-    }
-    if (dalloc(&stats_out, 7 * d.ny + 8, owned, bytes)) return -1;
-    // metrics
-    if (dupload(&m_rdxf, h.rdxf, owned, bytes) || dupload(&m_rdxc, h.rdxc, owned, bytes) ||
-        dupload(&m_rdyf, h.rdyf, owned, bytes) || dupload(&m_rdyc, h.rdyc, owned, bytes) ||
-        dupload(&m_rdzf, h.rdzf, owned, bytes) || dupload(&m_rdzc, h.rdzc, owned, bytes) ||
-        dupload(&m_w0, h.w0, owned, bytes) || dupload(&m_w1, h.w1, owned, bytes) ||
-        dupload(&m_dyg, h.dyg, owned, bytes) || dupload(&m_dxc, h.dxc, owned, bytes) ||
-        dupload(&m_dyc, h.dyc, owned, bytes) || dupload(&m_dzc, h.dzc, owned, bytes) ||
-        dupload(&m_x, h.x1, owned, bytes) || dupload(&m_y, h.y1, owned, bytes) ||
-        dupload(&m_z, h.z1, owned, bytes) || dupload(&m_xg, h.xg1, owned, bytes) ||
-        dupload(&m_yg, h.yg1, owned, bytes) || dupload(&m_zg, h.zg1, owned, bytes) ||
-        dupload(&m_kxx, h.kxx, owned, bytes) || dupload(&m_kzz, h.kzz, owned, bytes) ||
-        dupload(&m_dd, h.dd, owned, bytes) || dupload(&m_dl, h.dl, owned, bytes) ||
-        dupload(&m_du, h.du, owned, bytes) || dupload(&m_delta, h.delta, owned, bytes) ||
-        dupload(&m_jjz, h.jjz, owned, bytes) || dupload(&m_wz, h.wz, owned, bytes) ||
-        dupload(&m_jjx, h.jjx, owned, bytes) || dupload(&m_wx, h.wx, owned, bytes))
+        for (int c = 0; c < 3; c++) HCK(hipMemsetAsync(st[b][c], 0, n3[c] * sizeof(S), stream));
+    HCK(hipMemsetAsync(phi1, 0, nphi * sizeof(S), stream));
+    HCK(hipMemsetAsync(stats_acc, 0, (7 * d.ny + 8) * sizeof(double), stream));
+    if (ubar) HCK(hipMemsetAsync(ubar, 0, (d.nyg + 2) * sizeof(double), stream));
+    if (upload(m_rdxf, h.rdxf) || upload(m_rdxc, h.rdxc) || upload(m_rdyf, h.rdyf) || upload(m_rdyc, h.rdyc) ||
+        upload(m_rdzf, h.rdzf) || upload(m_rdzc, h.rdzc) || upload(m_w0, h.w0) || upload(m_w1, h.w1) ||
+        upload(m_dyg, h.dyg) || upload(m_dxc, h.dxc) || upload(m_dyc, h.dyc) || upload(m_dzc, h.dzc) ||
+        upload(m_x, h.x1) || upload(m_y, h.y1) || upload(m_z, h.z1) || upload(m_xg, h.xg1) ||
+        upload(m_yg, h.yg1) || upload(m_zg, h.zg1) || upload(m_kxx, h.kxx) || upload(m_kzz, h.kzz) ||
+        upload(m_dd, h.dd) || upload(m_dl, h.dl) || upload(m_du, h.du) || upload(m_delta, h.delta) ||
+        upload(m_jjz, h.jjz) || upload(m_wz, h.wz) || upload(m_jjx, h.jjx) || upload(m_wx, h.wx))
         return -1;
     met.rdxf = m_rdxf; met.rdxc = m_rdxc; met.rdyf = m_rdyf; met.rdyc = m_rdyc;
     met.rdzf = m_rdzf; met.rdzc = m_rdzc; met.w0 = m_w0; met.w1 = m_w1; met.dyg = m_dyg;
     wt.jjz = m_jjz; wt.wz = m_wz; wt.jjx = m_jjx; wt.wx = m_wx;
     // FFT plans and twiddles: x uses a length-M complex FFT plus the N-point split
     std::string err;
-    const int M = d.nxp / 2;
     pM = make_plan(M, err);
     pZ = make_plan(d.nzp, err);
     if (!err.empty()) { set_error(err); return -1; }
@@ -284,21 +304,12 @@ int Ctx<S>::init() {
         }
         return t;
     };
-    if (dupload(&twM, twid(M), owned, bytes) || dupload(&twN, twid(d.nxp), owned, bytes) ||
-        dupload(&twZ, twid(d.nzp), owned, bytes))
-        return -1;
+    if (upload(twM, twid(M)) || upload(twN, twid(d.nxp)) || upload(twZ, twid(d.nzp))) return -1;
     // wall model buffers
-    const size_t nwx = (size_t)d.nx * 2 * d.nzg, nwz = (size_t)d.nxg * 2 * d.nz;
-    if (dalloc(&alpha_x, nwx, owned, bytes) || dalloc(&alpha_z, nwz, owned, bytes) ||
-        dalloc(&azt, nwx, owned, bytes) || dalloc(&tau_init, nwx, owned, bytes))
-        return -1;
-    HCK(hipMemset(alpha_x, 0, nwx * sizeof(double)));
-    HCK(hipMemset(alpha_z, 0, nwz * sizeof(double)));
-    HCK(hipMemset(azt, 0, nwx * sizeof(double)));
-    {
-        std::vector<double> ti(nwx, h.prm.tau_mean_eqwm);
-        HCK(hipMemcpy(tau_init, ti.data(), nwx * sizeof(double), hipMemcpyHostToDevice));
-    }
+    HCK(hipMemsetAsync(alpha_x, 0, nwx * sizeof(double), stream));
+    HCK(hipMemsetAsync(alpha_z, 0, nwz * sizeof(double), stream));
+    HCK(hipMemsetAsync(azt, 0, nwx * sizeof(double), stream));
+    if (upload(tau_init, std::vector<double>(nwx, h.prm.tau_mean_eqwm))) return -1;
     // scalars
     Scalars s0{};
     s0.dt = h.prm.cfl < 0 ? -h.prm.cfl : 0.0;
@@ -307,7 +318,7 @@ int Ctx<S>::init() {
     s0.norm_u = h.norm_u;
     s0.norm_v = h.norm_v;
     HCK(hipMemcpy(sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice));
-    HCK(hipDeviceSynchronize());
+    HCK(hipStreamSynchronize(stream));
     return 0;
 }
 
@@ -497,6 +508,26 @@ int Ctx<S>::run_steps(int nsteps, bool use_graph) {
         }
     }
     HCK(hipGetLastError());
+    return 0;
+}
+
+// capture the step graphs of both ping-pong parities without running them
+template <typename S>
+int Ctx<S>::prepare_graphs() {
+    HCK(hipSetDevice(dev));
+    const int keep = cur;
+    for (int p = 0; p < 2; p++) {
+        if (graphs[p]) continue;
+        cur = p;
+        hipGraph_t g;
+        HCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        dt_launch();
+        step_launch();
+        HCK(hipStreamEndCapture(stream, &g));
+        HCK(hipGraphInstantiate(&graphs[p], g, nullptr, nullptr, 0));
+        HCK(hipGraphDestroy(g));
+    }
+    cur = keep;
     return 0;
 }
 
@@ -784,6 +815,42 @@ int Ctx<S>::statistics(double *out) {
     return 0;
 }
 
+// time-window statistics: the same plane sums as statistics(), added into
+// an FP64 device accumulator (no host round trip; may be captured)
+template <typename S>
+int Ctx<S>::stats_accumulate() {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    launch_stats<S>(d, met, st[cur][0], st[cur][1], st[cur][2], ubar, stats_out, stream);
+    launch_stats_accum(stats_out, stats_acc, 7 * d.ny, stream);
+    HCK(hipGetLastError());
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::stats_window(double *out, double *count) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    std::vector<double> a(7 * d.ny + 1);
+    HCK(hipStreamSynchronize(stream));
+    HCK(hipMemcpy(a.data(), stats_acc, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double n = a[7 * d.ny];
+    if (count) *count = n;
+    const double cu = (double)((d.nx - 2) * (d.nzg - 2)), cv = (double)((d.nxg - 2) * (d.nzg - 2)),
+                 cw = (double)((d.nxg - 2) * (d.nz - 2));
+    const double den[7] = {cu, cv, cw, cu, cv, cw, cv};
+    for (int c = 0; c < 7; c++)
+        for (int j = 0; j < d.ny; j++) out[c * d.ny + j] = n > 0 ? a[c * d.ny + j] / (den[c] * n) : 0.0;
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::stats_reset() {
+    HCK(hipSetDevice(dev));
+    HCK(hipMemsetAsync(stats_acc, 0, (7 * h.d.ny + 8) * sizeof(double), stream));
+    return 0;
+}
+
 template <typename S>
 int Ctx<S>::compute_rhs(const double *U, const double *V, const double *W, double *ru, double *rv,
                         double *rw) {
@@ -856,8 +923,8 @@ int Ctx<S>::attach_mlp(Mlp *m) {
     const Dims &d = h.d;
     const int n = d.nx * d.nzg * 2;
     if (!wall_u) {
-        if (dalloc(&wall_u, n, owned, bytes) || dalloc(&tau_pred, n, owned, bytes) ||
-            dalloc(&use_pred, n, owned, bytes))
+        if (dalloc(&wall_u, n, owned, bytes, (Slab *)nullptr) || dalloc(&tau_pred, n, owned, bytes, (Slab *)nullptr) ||
+            dalloc(&use_pred, n, owned, bytes, (Slab *)nullptr))
             return -1;
     }
     if (m->ensure_wall(n)) return -1;
@@ -895,6 +962,23 @@ std::string Ctx<S>::timing_report() {
 
 template struct Ctx<double>;
 template struct Ctx<float>;
+
+std::unique_ptr<CtxBase> make_ctx(const Host &h, Slab *slab, hipStream_t stream) {
+    std::unique_ptr<CtxBase> c;
+    if (h.prm.precision == 0) c.reset(new Ctx<double>(h));
+    else c.reset(new Ctx<float>(h));
+    c->slab = slab;
+    c->stream = stream;
+    if (c->init()) return nullptr;
+    return c;
+}
+
+size_t ctx_footprint(const Host &h) {
+    Slab dry;
+    dry.dry = true;
+    auto c = make_ctx(h, &dry, nullptr);
+    return c ? dry.used : 0;
+}
 
 }  // namespace cf
 
@@ -940,6 +1024,14 @@ int cf_create(const cf_params *p, const double *x, const double *y, const double
     c->impl = std::move(impl);
     *out = c;
     return 0;
+}
+
+long long cf_footprint(const cf_params *p, const double *x, const double *y, const double *z) {
+    if (!p || !x || !y || !z) { set_error("cf_footprint: null argument"); return -1; }
+    cf::Host h(*p, x, y, z);
+    std::string v = h.validate();
+    if (!v.empty()) { set_error("cf_footprint: " + v); return -1; }
+    return (long long)cf::ctx_footprint(h);
 }
 
 int cf_destroy(cf_ctx *c) {
@@ -1030,6 +1122,14 @@ int cf_set_wall_surrogate(cf_ctx *c, cf_mlp *m) {
     return c->impl->attach_mlp(m ? cf::mlp_of(m) : nullptr);
 }
 int cf_wall_fallbacks(cf_ctx *c) { CTX_CHECK(c); return c->impl->wall_fallbacks(); }
+
+int cf_stats_accumulate(cf_ctx *c) { CTX_CHECK(c); return c->impl->stats_accumulate(); }
+int cf_stats_window(cf_ctx *c, double *out, double *count) {
+    CTX_CHECK(c);
+    if (!out) { set_error("cf_stats_window: null output"); return -1; }
+    return c->impl->stats_window(out, count);
+}
+int cf_stats_reset(cf_ctx *c) { CTX_CHECK(c); return c->impl->stats_reset(); }
 
 int cf_timing_enable(cf_ctx *c, int on) { CTX_CHECK(c); c->impl->set_timing(on != 0); return 0; }
 int cf_timing_reset(cf_ctx *c) { CTX_CHECK(c); c->impl->timing_reset(); return 0; }

@@ -5,6 +5,7 @@
 
 #include <functional>
 #include <map>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -40,8 +41,20 @@ struct Host {
     std::string validate() const;
 };
 
+// A device region handed to a context: buffers are carved from it in
+// order (256-byte aligned).  dry: count the footprint without touching the
+// device.
+struct Slab {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    bool dry = false;
+};
+
 struct CtxBase {
     Host h;
+    Slab *slab = nullptr;          // optional: allocate from this region
+    hipStream_t stream = nullptr;  // optional: a caller's stream (set before init)
+    bool stream_ext = false;
     explicit CtxBase(const Host &hh) : h(hh) {}
     virtual ~CtxBase() {}
     virtual int init() = 0;
@@ -68,6 +81,9 @@ struct CtxBase {
     virtual int mean_mass_flow_u(double *q) = 0;
     virtual int check_divergence(double *ms, double *ma) = 0;
     virtual int statistics(double *out) = 0;
+    virtual int stats_accumulate() = 0;
+    virtual int stats_window(double *out, double *count) = 0;
+    virtual int stats_reset() = 0;
     virtual void set_timing(bool on) = 0;
     virtual void timing_reset() = 0;
     virtual std::string timing_report() = 0;
@@ -76,7 +92,14 @@ struct CtxBase {
     virtual hipStream_t get_stream() const = 0;
     virtual int attach_mlp(Mlp *m) = 0;
     virtual int wall_fallbacks() = 0;
+    virtual int prepare_graphs() = 0;
 };
+
+// a context for one case: device buffers from `slab` when given (else
+// hipMalloc), work on `stream` when given (else its own); null on failure
+std::unique_ptr<CtxBase> make_ctx(const Host &h, Slab *slab, hipStream_t stream);
+// device bytes a context of these parameters takes from a slab
+size_t ctx_footprint(const Host &h);
 
 template <typename S>
 struct Ctx : CtxBase {
@@ -84,7 +107,6 @@ struct Ctx : CtxBase {
     ~Ctx() override;
 
     int dev = -1;
-    hipStream_t stream = nullptr;
     std::vector<void *> owned;
     size_t bytes = 0;
     size_t n3[3] = {0, 0, 0};
@@ -165,6 +187,10 @@ struct Ctx : CtxBase {
     int mean_mass_flow_u(double *q) override;
     int check_divergence(double *ms, double *ma) override;
     int statistics(double *out) override;
+    int stats_accumulate() override;
+    int stats_window(double *out, double *count) override;
+    int stats_reset() override;
+    double *stats_acc = nullptr;
     void set_timing(bool on) override { collect(); timing = on; }
     void timing_reset() override { collect(); tstat.clear(); }
     std::string timing_report() override;
@@ -178,6 +204,7 @@ struct Ctx : CtxBase {
     int *use_pred = nullptr;
     int attach_mlp(Mlp *m) override;
     int wall_fallbacks() override;
+    int prepare_graphs() override;
 };
 
 }  // namespace cf

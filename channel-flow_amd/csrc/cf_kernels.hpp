@@ -50,6 +50,8 @@ template <typename S>
 void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                   const double *ubar, double *out, hipStream_t st);
 
+void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st);
+
 // ---- Poisson chain (cf_fft.hip) ----
 int fft_rows_per_wg(int M, int elems_per_row, int esize);
 int fft_cols_per_wg(int n, int esize);

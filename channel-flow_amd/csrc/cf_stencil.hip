@@ -775,6 +775,17 @@ k_stats_planes(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__
     }
 }
 
+// time-window accumulation of the plane statistics: acc[0..n) += out, acc[n] += 1
+__global__ void k_stats_accum(const double *__restrict__ out, double *__restrict__ acc, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) acc[q] += out[q];
+    if (q == n) acc[n] += 1.0;
+}
+
+void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_stats_accum, dim3((n + 1 + 255) / 256), dim3(256), 0, st, out, acc, n);
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------

@@ -146,6 +146,61 @@ int cf_set_wall_surrogate(cf_ctx *c, cf_mlp *m);
 /* wall points that fell back to the ODE solver in the last wall-model call */
 int cf_wall_fallbacks(cf_ctx *c);
 
+/* Time-window statistics.  The reference writes instantaneous plane means
+ * every nstats steps (statistics.f90:22-171); these add the same seven
+ * plane sums (U, V, W, their squares, UV at the y faces) into an FP64 device
+ * accumulator without a host round trip, and return their average over the
+ * accumulated samples (7*ny values, as cf_statistics) and the sample count.
+ * Reynolds stresses follow as <UU> - <U><U>, <UV> - <U><V>. */
+int cf_stats_accumulate(cf_ctx *c);
+int cf_stats_window(cf_ctx *c, double *out, double *count);
+int cf_stats_reset(cf_ctx *c);
+
+/* Device bytes that a context of these parameters needs (exact; the
+ * serving arena admits cases by this figure).  No device is touched. */
+long long cf_footprint(const cf_params *p, const double *x, const double *y, const double *z);
+
+/* ---- serving ----
+ * Many cases of different grids, Reynolds numbers and models on one GPU
+ * with continuous batching.  The reference has no counterpart: there one
+ * case is one run of main.f90 (main.f90:61-87).
+ * hbm_budget: bytes reserved once (one allocation) for all running cases;
+ * a case is admitted, in submission order once its arrival time has passed,
+ * when a free slot and a contiguous free region of cf_footprint bytes exist.
+ * max_active: concurrent cases (one HIP stream each); depth: RK3 steps kept
+ * in flight per case. */
+typedef struct cf_server cf_server;
+int cf_server_create(int device, long long hbm_budget, int max_active, int depth, cf_server **out);
+/* cancels what is left, stops the scheduler and frees the arena */
+int cf_server_destroy(cf_server *s);
+/* queue one case: raw initial field (projected at admission, as
+ * initialization.f90:156-160), nsteps RK3 steps, arrival time in seconds
+ * after cf_server_start.  Arrays are copied. */
+int cf_server_submit(cf_server *s, const cf_params *p, const double *x, const double *y, const double *z,
+                     const double *U, const double *V, const double *W, int nsteps, double arrival,
+                     long long *id);
+/* start the clock, the scheduler and the admission worker (cases may be
+ * submitted before and after) */
+int cf_server_start(cf_server *s);
+int cf_server_cancel(cf_server *s, long long id);
+/* wait until every submitted case is done, cancelled or failed; 1 on timeout
+ * (timeout_s <= 0 waits without limit) */
+int cf_server_wait(cf_server *s, double timeout_s);
+/* state: 0 pending, 1 preparing, 2 running, 3 done, 4 cancelled, 5 failed */
+int cf_server_status(cf_server *s, long long id, int *state, int *steps_done);
+/* final field of a finished case and its 7 scalars (as cf_get_scalars) */
+int cf_server_result(cf_server *s, long long id, double *U, double *V, double *W, double *scalars);
+/* info (up to 14): arrival, admitted, ready, finished (s after start), steps
+ * done, steps asked, footprint bytes, step ms p50, p99, max, state, unique
+ * cells, precision, iwall_model */
+int cf_server_case_info(cf_server *s, long long id, double *info, int n);
+/* per-step GPU durations (ms) of a case; returns how many were written */
+int cf_server_step_times(cf_server *s, long long id, float *ms, int n);
+/* out (up to 6): last finish time, peak arena bytes in use, peak running
+ * cases, cases that waited for memory at the head of the queue, arena bytes,
+ * cases submitted */
+int cf_server_summary(cf_server *s, double *out, int n);
+
 /* Per-kernel timing with HIP events recorded on the context's stream. */
 int cf_timing_enable(cf_ctx *c, int on);
 /* writes "name avg_ms count\n" lines; returns 0 */

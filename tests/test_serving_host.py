@@ -1,0 +1,48 @@
+"""Serving host logic without a GPU: the exact footprint that admission
+uses (a dry allocation pass of the context) and the seeded ensemble."""
+import numpy as np
+
+from channel_flow import cases
+from channel_flow.serving import ensemble_workload, footprint, GRIDS
+
+
+def _field_bytes(cfg, esize):
+    NX, NY, NZ = cfg.cells
+    nx, ny, nz = NX + 2, NY + 1, NZ + 2
+    return esize * (nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz)
+
+
+def test_footprint_covers_the_state_and_scales():
+    c = cases.case_c1()
+    fp = footprint(c)
+    # two ping-pong copies of (U, V, W) plus the RK3 accumulators: at least 3x one state
+    assert fp >= 3 * _field_bytes(c, 8)
+    assert fp % 256 == 0
+    c2 = cases.case_c1()
+    c2.cells = (128, 128, 128)
+    assert 7.0 < footprint(c2) / fp < 9.0  # 8x the cells
+    m = cases.case_c1()
+    m.precision = "mixed"
+    assert footprint(m) < 0.7 * fp
+
+
+def test_footprint_of_large_cases():
+    c = cases.case_c2()
+    fp = footprint(c)
+    assert 3 * _field_bytes(c, 8) < fp < 20 * _field_bytes(c, 8) / 3
+    assert fp < 6e9
+
+
+def test_ensemble_is_seeded_and_mixed():
+    a = ensemble_workload(n=64, seed=5)
+    b = ensemble_workload(n=64, seed=5)
+    assert [(x.cells, x.nu, x.iwall_model, n, t) for x, n, t in a] == \
+           [(x.cells, x.nu, x.iwall_model, n, t) for x, n, t in b]
+    arr = np.array([t for _, _, t in a])
+    assert np.all(np.diff(arr) > 0)
+    assert {x.cells for x, _, _ in a} == set(GRIDS)
+    wm = sum(x.iwall_model == 3 for x, _, _ in a)
+    assert 20 <= wm <= 44
+    assert all(200 <= n <= 2000 for _, n, _ in a)
+    s = ensemble_workload(n=8, seed=5, steps_scale=0.1)
+    assert all(20 <= n <= 200 for _, n, _ in s)
