@@ -47,6 +47,22 @@ def algorithmic_bytes(name, n_cells, esz, mh_ratio):
     return table.get(name)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk3_step_384x256x384_fp64_hbm_traffic.md")
+
+
+def measured_traffic(kernel, cells):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy of known
+    size); None when no summary covers this kernel and workload."""
+    if cells != (384, 256, 384) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == "fp64":
+            return int(f[3])
+    return None
+
+
 def run_precision(cfg, steps, warmup, cases):
     from channel_flow.solver import ChannelFlow
     g = ChannelFlow(cfg, device=0)
@@ -135,6 +151,20 @@ def serving_bench(scale, max_active, budget_gb, n_cases=64, seed=5):
             "case_p99_gain": round(seq["case_s_p99"] / mix["case_s_p99"], 3)}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size (torch's copy kernel),
+    for scaling the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiler
+    run: each launch reads and writes exactly `mib` MiB."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +178,12 @@ def main():
                     help="steps of the serving ensemble relative to 200-2000 per case")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB five times (counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs a single case per GPU and has no "
               "multi-GPU path; use --gpus 1", file=sys.stderr)
@@ -185,7 +220,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None},
+                     "algorithmic_bytes": ab, "traffic": measured_traffic(dom, cells)},
         "kernels_fp64": kernels,
         "ms_per_step_event_timed": round(ms64_ev, 4),
         "max_div_fp64": div64,

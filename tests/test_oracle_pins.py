@@ -317,3 +317,34 @@ def test_statistics_of_analytic_field():
     uf = g["w0"] * 1.5 * (1 - (yg[:-1] - 1) ** 2) + g["w1"] * 1.5 * (1 - (yg[1:] - 1) ** 2)
     np.testing.assert_allclose(s["Umean"], uf, rtol=1e-13)
     np.testing.assert_allclose(s["UVmean"], 0.25 * uf, rtol=1e-13)
+
+
+def test_wall_model_closed_form():
+    """compute_equilibrium_wall_model (wallmodel.f90:591-685) on fields whose
+    wall-parallel velocity is known in closed form: U = a + b x at the
+    matching height and W linear in z (the 3-point Lagrange interpolations
+    reproduce both exactly), nu_t = 0.  Then u_les = sqrt(U^2 + W(zg)^2) at
+    each U point, tau = the ODE stress of u_les, alpha_x = tau U/(u_les nu),
+    and alpha_z (4-point average of tau W/(u_les nu) on the U points) is
+    checked against the same average evaluated directly."""
+    c = cfg_small(cells=(12, 16, 10), stretch=0.0, iwall_model=3, les_model=0, eqwm_ncv=30,
+                  eqwm_tol=1e-10, eqwm_max_iter=300, nu=1e-4, dPdx=1e-3)
+    o = Oracle(c)
+    U, V, W = fields(o, lambda x, y, z: 0.8 + 0.05 * x + 0 * y * z, None,
+                     lambda x, y, z: 0.1 + 0.03 * z + 0 * x * y)
+    o.set_state(U, V, W)
+    o.set_tau_init(np.full((o.nx, 2, o.nzg), 1e-3, order="F"))
+    o.compute_wall_model()
+    ax, az, ti = o.wall_state()
+    g = o.grid()
+    zg = g["zg"]
+    ul = np.sqrt((0.8 + 0.05 * o.x)[:, None] ** 2 + (0.1 + 0.03 * zg)[None, :] ** 2)  # (nx, nzg)
+    tau = np.vectorize(lambda u: eqwm(u, 1e-4, g["yg"][o.p.jref_eqwm - 1], 1e-3, ncv=30, stretch=1.1,
+                                      tol=1e-10, max_iter=300)[0])(ul)
+    for wall in (0, 1):
+        np.testing.assert_allclose(ti[:, wall, :], tau, rtol=1e-12)
+        np.testing.assert_allclose(ax[:, wall, :], tau * (0.8 + 0.05 * o.x)[:, None] / ul / 1e-4, rtol=1e-12)
+    azt = tau * (0.1 + 0.03 * zg)[None, :] / ul / 1e-4  # at U points (i, k)
+    # interior W points: average of the 4 surrounding U-point values
+    exp = 0.25 * (azt[:-1, :-1] + azt[1:, :-1] + azt[:-1, 1:] + azt[1:, 1:])  # (nx-1, nzg-1) -> W (i, k)
+    np.testing.assert_allclose(az[1:o.nx - 2, 0, 1:o.nz - 2], exp[1:o.nx - 2, 1:o.nz - 2], rtol=1e-12)
