@@ -179,6 +179,7 @@ std::string Host::validate() const {
     if (p.nx % 2 || p.nz % 2) return "nx and nz must be even (initialization.f90:54-55)";
     if ((int64_t)d.nxg * d.nyg * d.nzg > (int64_t)2000000000) return "grid too large";
     if (d.nxp / 2 > 1024 || d.nzp > 2048) return "FFT length too large";
+    if (!thomas_supported(d)) return "too many y cells for the y solve";
     if (p.precision != 0 && p.precision != 1) return "precision must be 0 (fp64) or 1 (mixed)";
     if (p.les_model != 0 && p.les_model != 1) return "les_model must be 0 or 1";
     if (p.iwall_model != 0 && p.iwall_model != 3) return "iwall_model must be 0 or 3";
@@ -251,7 +252,8 @@ int Ctx<S>::init() {
             if (A(&st[b][c], n3[c])) return -1;
     for (int c = 0; c < 3; c++)
         if (A(&G[c], n3[c])) return -1;
-    if (A(&NT, nC) || A(&spec, nspec) || A(&cscr, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
+    if (thomas_uses_scratch((int)sizeof(Cx<S>)) && A(&cscr, nspec)) return -1;
+    if (A(&NT, nC) || A(&spec, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
         A(&part, npart) || A(&dtscr, dt_scratch_doubles(d)) || A(&sc, 1) ||
         (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, 7 * d.ny + 8) || A(&stats_acc, 7 * d.ny + 8))
         return -1;
@@ -305,6 +307,7 @@ int Ctx<S>::init() {
         return t;
     };
     if (upload(twM, twid(M)) || upload(twN, twid(d.nxp)) || upload(twZ, twid(d.nzp))) return -1;
+
     // wall model buffers
     HCK(hipMemsetAsync(alpha_x, 0, nwx * sizeof(double), stream));
     HCK(hipMemsetAsync(alpha_z, 0, nwz * sizeof(double), stream));

@@ -8,9 +8,10 @@
 // kx = 0..nxp/2 carries the same information, so only it is stored and
 // solved (initialization.f90:281-296 gives the symmetry).
 //
-// Spectrum layout in HBM: spec[(j' * nzp + kz) * mh + kx], j' = j-2, complex,
-// kx fastest.  x-FFT rows, z-FFT column batches and Thomas lanes all read it
-// with unit stride across threads.
+// Spectrum layout in HBM: spec[(j' * mh + kx) * nzp + kz], j' = j-2, complex,
+// kz fastest.  The x transforms read and write it in runs of consecutive kz
+// (one workgroup holds consecutive z rows of one y plane), and the fused
+// z-transform / y-solve kernels read whole contiguous kz rows.
 //
 // The FFT engine is an LDS Stockham autosort transform (radix 4, 2, 3) over a
 // batch of lines per workgroup; each pass stages its butterflies through
@@ -143,8 +144,9 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     __syncthreads();
     lds_fft<256>(lds, ls, rb, pM, twM, -1);
     const int mh = M + 1;
+    // consecutive threads take consecutive rows (kz) of one kx: runs of rb
     for (int e = threadIdx.x; e < rb * mh; e += 256) {
-        const int lr = e / mh, kx = e - lr * mh;
+        const int kx = e / rb, lr = e - kx * rb;
         const int r = row0 + lr;
         if (r >= nrows) continue;
         const Cx<T> *z = lds + lr * ls;
@@ -155,12 +157,15 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         const Cx<T> dd = csub(a, b);
         const Cx<T> O = {(T)0.5 * dd.y, -(T)0.5 * dd.x};  // (a-b)/(2i)
         const Cx<T> X = cadd(E, cmul(twN[kx], O));
-        spec[(int64_t)r * mh + kx] = X;
+        const int jp = r / d.nzp, kp = r - jp * d.nzp;
+        spec[((int64_t)jp * mh + kx) * d.nzp + kp] = X;
     }
 }
 
 // ---------------------------------------------------------------------------
-// C2C FFT along z over column batches of the spectrum (in place).
+// C2C FFT along z (in place).  With kz fastest, column c = j' * mh + kx is
+// the contiguous run spec[c * nzp .. c * nzp + nzp), and a workgroup's batch
+// of consecutive columns is one contiguous block of memory.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(512)
@@ -168,22 +173,69 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
        int sign) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int n = pZ.n, ls = n + 1, mh = d.mh;
-    const int nch = (mh + bcols - 1) / bcols;
-    const int jp = blockIdx.x / nch, ch = blockIdx.x - jp * nch;
-    const int kx0 = ch * bcols;
-    Cx<T> *base = spec + (int64_t)jp * n * mh;
-    for (int e = threadIdx.x; e < bcols * n; e += 512) {
-        const int kz = e / bcols, b = e - kz * bcols;
-        const int kx = kx0 + b;
-        lds[b * ls + kz] = kx < mh ? base[(int64_t)kz * mh + kx] : Cx<T>{(T)0, (T)0};
+    const int n = pZ.n, ls = n + 1;
+    const int64_t ncols = (int64_t)d.nyp * d.mh;
+    const int64_t c0 = (int64_t)blockIdx.x * bcols;
+    const int nb = (int)min((int64_t)bcols, ncols - c0);
+    Cx<T> *base = spec + c0 * n;
+    for (int e = threadIdx.x; e < nb * n; e += 512) {
+        const int b = e / n, kz = e - b * n;
+        lds[b * ls + kz] = base[e];
     }
     __syncthreads();
-    lds_fft<512>(lds, ls, bcols, pZ, twZ, sign);
-    for (int e = threadIdx.x; e < bcols * n; e += 512) {
-        const int kz = e / bcols, b = e - kz * bcols;
-        const int kx = kx0 + b;
-        if (kx < mh) base[(int64_t)kz * mh + kx] = lds[b * ls + kz];
+    lds_fft<512>(lds, ls, nb, pZ, twZ, sign);
+    for (int e = threadIdx.x; e < nb * n; e += 512) {
+        const int b = e / n, kz = e - b * n;
+        base[e] = lds[b * ls + kz];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonal solve in y, one lane per (kx,kz) mode: forward sweep keeping c'
+// in an FP64 scratch and d' in the spectrum, then back substitution.  Four
+// passes over the spectrum, but no serial phase: the faster form when the
+// spectrum is FP32 (its passes are half as large).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ cscr, const double *__restrict__ kxx,
+              const double *__restrict__ kzz, const double *__restrict__ dd, const double *__restrict__ dl,
+              const double *__restrict__ du) {
+    const int nzp = d.nzp, nyp = d.nyp;
+    const int nmodes = d.mh * nzp;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nmodes) return;
+    const int kx = q / nzp, kz = q - kx * nzp;
+    const double kx2 = kxx[kx], kz2 = kzz[kz];
+    const int64_t js = nmodes;
+    Cx<T> *col = spec + q;
+    double D0 = (dd[0] + kx2) + kz2;
+    if (kx == 0 && kz == 0) D0 = 1.5 * D0;
+    double r = 1.0 / D0;
+    double cp = du[0] * r;
+    const Cx<T> b0 = col[0];
+    double dpx = (double)b0.x * r, dpy = (double)b0.y * r;
+    col[0] = Cx<T>{(T)dpx, (T)dpy};
+    cscr[q] = cp;
+#pragma unroll 4
+    for (int jj = 1; jj < nyp; jj++) {
+        const double a = dl[jj - 1];
+        r = 1.0 / (((dd[jj] + kx2) + kz2) - a * cp);
+        const Cx<T> b = col[jj * js];
+        dpx = ((double)b.x - a * dpx) * r;
+        dpy = ((double)b.y - a * dpy) * r;
+        col[jj * js] = Cx<T>{(T)dpx, (T)dpy};
+        cp = (jj < nyp - 1) ? du[jj] * r : 0.0;
+        cscr[(int64_t)jj * nmodes + q] = cp;
+    }
+    double xr = dpx, xi = dpy;
+#pragma unroll 4
+    for (int jj = nyp - 2; jj >= 0; jj--) {
+        const double c = cscr[(int64_t)jj * nmodes + q];
+        const Cx<T> dv = col[jj * js];
+        xr = (double)dv.x - c * xr;
+        xi = (double)dv.y - c * xi;
+        col[jj * js] = Cx<T>{(T)xr, (T)xi};
     }
 }
 
@@ -191,55 +243,134 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 // Tridiagonal solve in y per (kx,kz) (projection.f90:90-113).
 //   D(j) = Dyy(j,j) + kxx(kx) + kzz(kz),  DL, DU from Dyy,  D(2) *= 3/2 for
 //   the (0,0) mode.  The matrix is diagonally dominant (kxx, kzz <= 0), so
-//   Gaussian elimination without pivoting (Thomas) is used; LAPACK zgtsv's
-//   row interchanges, when they happen, change results at round-off only.
-// One lane per mode; the forward sweep keeps c' in a scratch array so the
-// back substitution streams it back.  Arithmetic is FP64 in every mode.
+//   elimination without pivoting is used; LAPACK zgtsv's row interchanges,
+//   when they happen, change results at round-off only.
+// Partitioned solve, one read and one write of the spectrum: a workgroup
+// takes one kx and KZ consecutive kz; thread (kz, s) holds rows
+// [s*MS, s*MS+MS) of its system in registers and solves that segment for
+// the right-hand side (y) and for unit couplings to the neighbouring
+// segments' end values (v: lower, w: upper), so x = y + alpha L(s-1) +
+// beta F(s+1) with F, L the first and last unknown of each segment.  The
+// 2x2-block system of the segment end values is solved by one thread per
+// system through LDS, then every thread assembles and stores its rows.
+// Arithmetic is FP64 in every mode.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_thomas(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ cscr,
-         const double *__restrict__ kxx, const double *__restrict__ kzz,
-         const double *__restrict__ dd, const double *__restrict__ dl,
-         const double *__restrict__ du) {
-    const int mh = d.mh, nzp = d.nzp, nyp = d.nyp;
-    const int nmodes = mh * nzp;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nmodes) return;
-    const int kz = q / mh, kx = q - kz * mh;
-    const double kx2 = kxx[kx], kz2 = kzz[kz];
-    const int64_t jstride = (int64_t)nzp * mh;
-    Cx<T> *col = spec + q;
-    // forward elimination; D(j) = (Dyy(j,j) + kxx) + kzz as in the reference
-    double D0 = (dd[0] + kx2) + kz2;
-    if (kx == 0 && kz == 0) D0 = 1.5 * D0;
-    double r = 1.0 / D0;
-    double cp = du[0] * r;
-    Cx<T> b0 = col[0];
-    double dpx = (double)b0.x * r, dpy = (double)b0.y * r;
-    col[0] = Cx<T>{(T)dpx, (T)dpy};
-    cscr[q] = cp;
-#pragma unroll 4
-    for (int jj = 1; jj < nyp; jj++) {
-        const double a = dl[jj - 1];
-        const double mden = ((dd[jj] + kx2) + kz2) - a * cp;
-        r = 1.0 / mden;
-        const Cx<T> b = col[jj * jstride];
-        dpx = ((double)b.x - a * dpx) * r;
-        dpy = ((double)b.y - a * dpy) * r;
-        col[jj * jstride] = Cx<T>{(T)dpx, (T)dpy};
-        cp = (jj < nyp - 1) ? du[jj] * r : 0.0;
-        cscr[(int64_t)jj * nmodes + q] = cp;
+template <typename T, int MS>
+__global__ void __launch_bounds__(512)
+k_thomas(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ kxx, const double *__restrict__ kzz,
+         const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
+         int KZ, int P) {
+    extern __shared__ double sh[];  // 8 fields per (segment, kz), field-major: sh[f * NT + slot]
+    const int nzp = d.nzp, nyp = d.nyp;
+    const int nkzb = (nzp + KZ - 1) / KZ;
+    const int kx = blockIdx.x / nkzb, kz0 = (blockIdx.x - kx * nkzb) * KZ;
+    const int tid = threadIdx.x, kzl = tid % KZ, sg = tid / KZ;
+    const int kz = kz0 + kzl;
+    const bool active = sg < P && kz < nzp;
+    const int a = sg * MS;
+    const int len = active ? min(MS, nyp - a) : 0;
+    const int64_t rs = (int64_t)d.mh * nzp;
+    Cx<T> *col = spec + (int64_t)kx * nzp + kz;
+    const double kx2 = kxx[kx], kz2 = kz < nzp ? kzz[kz] : 0.0;
+    const bool zm = kx == 0 && kz == 0;
+    double yr[MS], yi[MS], v[MS], w[MS], cp[MS];
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        yr[i] = yi[i] = v[i] = w[i] = cp[i] = 0.0;
+        if (i < len) {
+            const Cx<T> b = col[(a + i) * rs];
+            yr[i] = (double)b.x;
+            yi[i] = (double)b.y;
+        }
     }
-    // back substitution
-    double xr = dpx, xi = dpy;
-#pragma unroll 4
-    for (int jj = nyp - 2; jj >= 0; jj--) {
-        const double c = cscr[(int64_t)jj * nmodes + q];
-        const Cx<T> dv = col[jj * jstride];
-        xr = (double)dv.x - c * xr;
-        xi = (double)dv.y - c * xi;
-        col[jj * jstride] = Cx<T>{(T)xr, (T)xi};
+    // segment forward sweep for the three right-hand sides (b, e_first, e_last)
+    {
+        double cprev = 0.0, yrp = 0.0, yip = 0.0, vp = 0.0, wp = 0.0;
+#pragma unroll
+        for (int i = 0; i < MS; i++) {
+            if (i < len) {
+                const int j = a + i;
+                double D = (dd[j] + kx2) + kz2;
+                if (j == 0 && zm) D = 1.5 * D;
+                const double lo = i > 0 ? dl[j - 1] : 0.0;
+                const double r = 1.0 / (D - lo * cprev);
+                cp[i] = (i < len - 1) ? du[j] * r : 0.0;
+                yr[i] = (yr[i] - lo * yrp) * r;
+                yi[i] = (yi[i] - lo * yip) * r;
+                v[i] = ((i == 0 ? 1.0 : 0.0) - lo * vp) * r;
+                w[i] = ((i == len - 1 ? 1.0 : 0.0) - lo * wp) * r;
+                cprev = cp[i]; yrp = yr[i]; yip = yi[i]; vp = v[i]; wp = w[i];
+            }
+        }
+#pragma unroll
+        for (int i = MS - 2; i >= 0; i--) {
+            if (i + 1 < len) {
+                yr[i] -= cp[i] * yr[i + 1];
+                yi[i] -= cp[i] * yi[i + 1];
+                v[i] -= cp[i] * v[i + 1];
+                w[i] -= cp[i] * w[i + 1];
+            }
+        }
+    }
+    // couplings to the neighbouring segments' end values
+    const double lo_c = (active && sg > 0) ? -dl[a - 1] : 0.0;             // times L(s-1)
+    const double up_c = (active && a + len < nyp) ? -du[a + len - 1] : 0.0;  // times F(s+1)
+    const int NT = blockDim.x;
+#define SH(f, slot) sh[(size_t)(f) * NT + (slot)]
+    if (active) {
+        double ymr = 0, ymi = 0, vm = 0, wm = 0;
+#pragma unroll
+        for (int i = 0; i < MS; i++)
+            if (i == len - 1) { ymr = yr[i]; ymi = yi[i]; vm = v[i]; wm = w[i]; }
+        SH(0, tid) = yr[0]; SH(1, tid) = yi[0]; SH(2, tid) = ymr; SH(3, tid) = ymi;
+        SH(4, tid) = lo_c * v[0]; SH(5, tid) = lo_c * vm;  // alpha at the first / last row
+        SH(6, tid) = up_c * w[0]; SH(7, tid) = up_c * wm;  // beta at the first / last row
+    }
+    __syncthreads();
+    // end values: F(t) = P(t) + Q(t) F(t+1), L(t) = G(t) + H(t) F(t+1)
+    if (sg == 0 && kz < nzp) {
+        double Gr = 0, Gi = 0, H = 0;
+        for (int t = 0; t < P; t++) {
+            const int q = t * KZ + kzl;
+            const double y1r = SH(0, q), y1i = SH(1, q), ymr = SH(2, q), ymi = SH(3, q);
+            const double a1 = SH(4, q), am = SH(5, q), b1 = SH(6, q), bm = SH(7, q);
+            const double rD = 1.0 / (1.0 - a1 * H);
+            const double Pr = (y1r + a1 * Gr) * rD, Pi = (y1i + a1 * Gi) * rD, Q = b1 * rD;
+            const double Gnr = ymr + am * Gr + am * H * Pr, Gni = ymi + am * Gi + am * H * Pi;
+            const double Hn = am * H * Q + bm;
+            SH(0, q) = Pr; SH(1, q) = Pi; SH(2, q) = Q; SH(3, q) = Gnr; SH(4, q) = Gni; SH(5, q) = Hn;
+            Gr = Gnr; Gi = Gni; H = Hn;
+        }
+        double Fr = 0, Fi = 0;
+        for (int t = P - 1; t >= 0; t--) {
+            const int q = t * KZ + kzl;
+            const double Pr = SH(0, q), Pi = SH(1, q), Q = SH(2, q), Gnr = SH(3, q), Gni = SH(4, q),
+                         Hn = SH(5, q);
+            const double Lr = Gnr + Hn * Fr, Li = Gni + Hn * Fi;
+            Fr = Pr + Q * Fr;
+            Fi = Pi + Q * Fi;
+            SH(0, q) = Fr; SH(1, q) = Fi; SH(2, q) = Lr; SH(3, q) = Li;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    double Lpr = 0, Lpi = 0, Fnr = 0, Fni = 0;
+    if (sg > 0) {
+        const int q = (sg - 1) * KZ + kzl;
+        Lpr = SH(2, q); Lpi = SH(3, q);
+    }
+    if (sg + 1 < P) {
+        const int q = (sg + 1) * KZ + kzl;
+        Fnr = SH(0, q); Fni = SH(1, q);
+    }
+#undef SH
+#pragma unroll
+    for (int i = 0; i < MS; i++) {
+        if (i < len) {
+            const double al = lo_c * v[i], be = up_c * w[i];
+            const double xr = yr[i] + al * Lpr + be * Fnr, xi = yi[i] + al * Lpi + be * Fni;
+            col[(a + i) * rs] = Cx<T>{(T)xr, (T)xi};
+        }
     }
 }
 
@@ -255,26 +386,40 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
               const Cx<T> *__restrict__ twN, int rb, double norm) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int M = pM.n, N = 2 * M, mh = M + 1, ls = M;
+    const int M = pM.n, N = 2 * M, mh = M + 1, ls = mh;
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
-    Cx<T> *xr = lds + rb * ls;  // raw spectrum rows (rb * mh)
+    // consecutive threads read consecutive rows (kz) of one kx: runs of rb
     for (int e = threadIdx.x; e < rb * mh; e += 256) {
-        const int lr = e / mh, kx = e - lr * mh;
+        const int kx = e / rb, lr = e - kx * rb;
         const int r = row0 + lr;
-        xr[e] = r < nrows ? spec[(int64_t)r * mh + kx] : Cx<T>{(T)0, (T)0};
+        Cx<T> v = {(T)0, (T)0};
+        if (r < nrows) {
+            const int jp = r / d.nzp, kp = r - jp * d.nzp;
+            v = spec[((int64_t)jp * mh + kx) * d.nzp + kp];
+        }
+        lds[lr * ls + kx] = v;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < rb * M; e += 256) {
-        const int lr = e / M, kx = e - lr * M;
-        const Cx<T> a = xr[lr * mh + kx];
-        const Cx<T> bc = xr[lr * mh + (M - kx)];
-        const Cx<T> b = {bc.x, -bc.y};
-        const Cx<T> s = cadd(a, b), df = csub(a, b);
-        Cx<T> wc = twN[kx];
-        wc.y = -wc.y;  // exp(+2 pi i k/N)
-        const Cx<T> t = cmul(df, wc);
-        lds[lr * ls + kx] = Cx<T>{s.x - t.y, s.y + t.x};  // s + i t
+    // packed length-M input of the complex inverse, in place: each thread
+    // owns the pair (k, M-k) (k = 0 pairs with M) and writes Z(k), Z(M-k)
+    // from the two spectrum values it read.
+    //   Z(k) = (X(k) + conj X(M-k)) + i (X(k) - conj X(M-k)) exp(+2 pi i k/N)
+    const int hp = M / 2 + 1;
+    for (int e = threadIdx.x; e < rb * hp; e += 256) {
+        const int lr = e / hp, k = e - lr * hp, k2 = M - k;
+        Cx<T> *row = lds + lr * ls;
+        const Cx<T> A = row[k], B = row[k2];
+        auto z = [&](Cx<T> a, Cx<T> bc, int kk) {
+            const Cx<T> b = {bc.x, -bc.y};
+            const Cx<T> s = cadd(a, b), df = csub(a, b);
+            Cx<T> wc = twN[kk];
+            wc.y = -wc.y;  // exp(+2 pi i k/N)
+            const Cx<T> t = cmul(df, wc);
+            return Cx<T>{s.x - t.y, s.y + t.x};  // s + i t
+        };
+        row[k] = z(A, B, k);
+        if (k != 0 && k2 != k) row[k2] = z(B, A, k2);
     }
     __syncthreads();
     lds_fft<256>(lds, ls, rb, pM, twM, +1);
@@ -310,10 +455,9 @@ int fft_rows_per_wg(int M, int elems_per_row, int esize) {
     int rb = 2048 / M;
     if (rb > 16) rb = 16;
     // tuning knob: cap on rows per workgroup (more workgroups per CU)
-    // (defaults from a sweep at 384x256x384: fewer rows per workgroup raise
-    // occupancy, especially for the inverse with its larger LDS footprint)
+    // (8 rows = runs of 8 consecutive kz in the spectrum, 128 bytes in FP64)
     const bool inv = elems_per_row > M;
-    const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", inv ? 6 : 8);
+    const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", inv ? 16 : 8);
     if (cap > 0 && rb > cap) rb = cap;
     const int lim = 65536 / (elems_per_row * esize);
     if (rb > lim) rb = lim;
@@ -333,6 +477,17 @@ int fft_cols_per_wg(int n, int esize) {
     return b;
 }
 
+// y-solve geometry: MS rows per thread segment, P = ceil(nyp/MS) segments,
+// KZ systems per workgroup (a power of two, KZ * P <= 512)
+// (8 rows: 16 would keep 80 doubles per thread and spill)
+static int thomas_ms() { return 8; }
+static void thomas_geometry(const Dims &d, int ms, int &KZ, int &P) {
+    P = (d.nyp + ms - 1) / ms;
+    KZ = 1;
+    while (KZ * 2 * P <= 512 && KZ * 2 <= 64) KZ *= 2;
+}
+bool thomas_supported(const Dims &d) { return (d.nyp + 7) / 8 <= 512; }
+
 template <typename S, typename T>
 void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      Cx<T> *spec, const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN,
@@ -348,26 +503,41 @@ template <typename T>
 void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ, int sign,
                  hipStream_t st) {
     const int b = fft_cols_per_wg(pZ.n, (int)sizeof(Cx<T>));
-    const int nch = (d.mh + b - 1) / b;
+    const int64_t ncols = (int64_t)d.nyp * d.mh;
     const size_t shm = (size_t)b * (pZ.n + 1) * sizeof(Cx<T>);
-    hipLaunchKernelGGL((k_fftz<T>), dim3(d.nyp * nch), dim3(512), shm, st, d, spec, pZ, twZ, b, sign);
+    hipLaunchKernelGGL((k_fftz<T>), dim3((unsigned)((ncols + b - 1) / b)), dim3(512), shm, st, d, spec, pZ, twZ,
+                       b, sign);
 }
+
+bool thomas_uses_scratch(int esize) { return env_int("CF_TH_LANE", esize < 16 ? 1 : 0) != 0; }
 
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st) {
-    const int nmodes = d.mh * d.nzp;
-    hipLaunchKernelGGL((k_thomas<T>), dim3((nmodes + 255) / 256), dim3(256), 0, st, d, spec, cscr, kxx,
-                       kzz, dd, dl, du);
+    if (cscr && thomas_uses_scratch((int)sizeof(Cx<T>))) {
+        const int nmodes = d.mh * d.nzp;
+        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((nmodes + 255) / 256), dim3(256), 0, st, d, spec, cscr, kxx,
+                           kzz, dd, dl, du);
+        return;
+    }
+    int KZ, P;
+    const int ms = thomas_ms();
+    thomas_geometry(d, ms, KZ, P);
+    const int nkzb = (d.nzp + KZ - 1) / KZ;
+    const int threads = KZ * P;
+    const size_t shm = (size_t)threads * 8 * sizeof(double);
+    (void)ms;
+    hipLaunchKernelGGL((k_thomas<T, 8>), dim3(d.mh * nkzb), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
+                       du, KZ, P);
 }
 
 template <typename S, typename T, typename P>
 void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
                         const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,
                         hipStream_t st) {
-    const int rb = fft_rows_per_wg(pM.n, 2 * pM.n + 1, (int)sizeof(Cx<T>));
+    const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = (size_t)rb * (2 * pM.n + 1) * sizeof(Cx<T>);
+    const size_t shm = (size_t)rb * (pM.n + 1) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_ifftx_proju<S, T, P>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m,
                        spec, phi, U, pM, twM, twN, rb, norm);
 }
@@ -383,9 +553,8 @@ template void launch_fftz<double>(const Dims &, Cx<double> *, const FFTPlan &, c
                                   hipStream_t);
 template void launch_fftz<float>(const Dims &, Cx<float> *, const FFTPlan &, const Cx<float> *, int,
                                  hipStream_t);
-template void launch_thomas<double>(const Dims &, Cx<double> *, double *, const double *,
-                                    const double *, const double *, const double *, const double *,
-                                    hipStream_t);
+template void launch_thomas<double>(const Dims &, Cx<double> *, double *, const double *, const double *,
+                                    const double *, const double *, const double *, hipStream_t);
 template void launch_thomas<float>(const Dims &, Cx<float> *, double *, const double *, const double *,
                                    const double *, const double *, const double *, hipStream_t);
 template void launch_ifftx_proju<double, double, double>(const Dims &, const Metrics &,

@@ -116,7 +116,7 @@ struct Ctx : CtxBase {
     S *NT = nullptr;
     bool nut_user = false;
     Cx<S> *spec = nullptr;
-    double *cscr = nullptr;
+    double *cscr = nullptr;  // y-solve scratch (lane form only)
     S *phi1 = nullptr, *phi23 = nullptr;
     double *part = nullptr, *stats_out = nullptr;
     double *dtscr = nullptr;

@@ -54,14 +54,18 @@ void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st);
 
 // ---- Poisson chain (cf_fft.hip) ----
 int fft_rows_per_wg(int M, int elems_per_row, int esize);
-int fft_cols_per_wg(int n, int esize);
 template <typename S, typename T>
 void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      Cx<T> *spec, const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN,
                      hipStream_t st);
+int fft_cols_per_wg(int n, int esize);
+bool thomas_supported(const Dims &d);
 template <typename T>
 void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ, int sign,
                  hipStream_t st);
+// the y solve form for a spectrum element of esize bytes needs an FP64
+// scratch of one double per spectrum element
+bool thomas_uses_scratch(int esize);
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st);

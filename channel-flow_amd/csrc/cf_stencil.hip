@@ -16,13 +16,31 @@ namespace cf {
 // with divisions by grid spacings replaced by multiplications with
 // precomputed reciprocals.
 // ---------------------------------------------------------------------------
+// The 1-D metrics a point needs, at stencil precision.  The i and j ones are
+// loaded once per thread (they do not change along the k march), the k ones
+// once per plane.  Every metric array is padded one entry past its range.
+template <typename R>
+struct PM {
+    R w0j, w1j, w0m, w1m;      // weight_y at j and j-1
+    R rxc, rxcm, rxf, rxfp;    // 1/dxc(i), 1/dxc(i-1), 1/dxf(i), 1/dxf(i+1)
+    R ryf, ryfp, ryc, rycm;    // 1/dyf(j), 1/dyf(j+1), 1/dyc(j), 1/dyc(j-1)
+    R rzf, rzfp, rzc, rzcm;    // 1/dzf(k), 1/dzf(k+1), 1/dzc(k), 1/dzc(k-1)
+    __device__ void set_ij(const Metrics &m, int i, int j) {
+        w0j = (R)m.w0[j]; w1j = (R)m.w1[j]; w0m = (R)m.w0[j - 1]; w1m = (R)m.w1[j - 1];
+        rxc = (R)m.rdxc[i]; rxcm = (R)m.rdxc[i - 1]; rxf = (R)m.rdxf[i]; rxfp = (R)m.rdxf[i + 1];
+        ryf = (R)m.rdyf[j]; ryfp = (R)m.rdyf[j + 1]; ryc = (R)m.rdyc[j]; rycm = (R)m.rdyc[j - 1];
+    }
+    __device__ void set_k(const Metrics &m, int k) {
+        rzf = (R)m.rdzf[k]; rzfp = (R)m.rdzf[k + 1]; rzc = (R)m.rdzc[k]; rzcm = (R)m.rdzc[k - 1];
+    }
+};
+
 template <typename R, bool NUT, bool ROT, class A>
-__device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, int j, int k, R nu, R om,
-                                          R dPdx, R dPdz, bool do_u, bool do_v, bool do_w, R &ru,
-                                          R &rv, R &rw) {
+__device__ __forceinline__ void rhs_point(const A &a, const PM<R> &pm, R nu, R om, R dPdx, R dPdz, bool do_u,
+                                          bool do_v, bool do_w, R &ru, R &rv, R &rw) {
     const R half = (R)0.5, quart = (R)0.25, two = (R)2;
-    const R w0j = (R)m.w0[j], w1j = (R)m.w1[j];
-    const R w0m = (R)m.w0[j - 1], w1m = (R)m.w1[j - 1];
+    const R w0j = pm.w0j, w1j = pm.w1j;
+    const R w0m = pm.w0m, w1m = pm.w1m;
 #define u(I, J, K) a.u(I, J, K)
 #define v(I, J, K) a.v(I, J, K)
 #define w(I, J, K) a.w(I, J, K)
@@ -30,16 +48,16 @@ __device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, i
     if (do_u) {
         R c1 = half * (u(0, 0, 0) + u(1, 0, 0));
         R c0 = half * (u(-1, 0, 0) + u(0, 0, 0));
-        R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdxc[i];
+        R t1 = -(c1 * c1 - c0 * c0) * pm.rxc;
         R uv1 = (w0j * u(0, 0, 0) + w1j * u(0, 1, 0)) * (half * (v(0, 0, 0) + v(1, 0, 0)));
         R uv0 = (w0m * u(0, -1, 0) + w1m * u(0, 0, 0)) * (half * (v(0, -1, 0) + v(1, -1, 0)));
-        R t2 = -(uv1 - uv0) * (R)m.rdyf[j];
+        R t2 = -(uv1 - uv0) * pm.ryf;
         R uw1 = (half * (u(0, 0, 0) + u(0, 0, 1))) * (half * (w(0, 0, 0) + w(1, 0, 0)));
         R uw0 = (half * (u(0, 0, -1) + u(0, 0, 0))) * (half * (w(0, 0, -1) + w(1, 0, -1)));
-        R t3 = -(uw1 - uw0) * (R)m.rdzf[k];
-        R rdz1 = (R)m.rdzc[k - 1], rdz2 = (R)m.rdzc[k], rdz3 = (R)m.rdzf[k];
-        R rdy3 = (R)m.rdyf[j];
-        R rdx1 = (R)m.rdxf[i], rdx2 = (R)m.rdxf[i + 1], rdx3 = (R)m.rdxc[i];
+        R t3 = -(uw1 - uw0) * pm.rzf;
+        R rdz1 = pm.rzcm, rdz2 = pm.rzc, rdz3 = pm.rzf;
+        R rdy3 = pm.ryf;
+        R rdx1 = pm.rxf, rdx2 = pm.rxfp, rdx3 = pm.rxc;
         R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
         if (NUT) {
             nux1 = nu + nt(0, 0, 0);
@@ -49,8 +67,8 @@ __device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, i
             nuz1 = nu + quart * (nt(0, 0, 0) + nt(0, 0, -1) + nt(1, 0, 0) + nt(1, 0, -1));
             nuz2 = nu + quart * (nt(0, 0, 0) + nt(0, 0, 1) + nt(1, 0, 0) + nt(1, 0, 1));
         }
-        R dudy1 = (u(0, 1, 0) - u(0, 0, 0)) * (R)m.rdyc[j];
-        R dudy0 = (u(0, 0, 0) - u(0, -1, 0)) * (R)m.rdyc[j - 1];
+        R dudy1 = (u(0, 1, 0) - u(0, 0, 0)) * pm.ryc;
+        R dudy0 = (u(0, 0, 0) - u(0, -1, 0)) * pm.rycm;
         R visc = two * rdx3 * (nux2 * (u(1, 0, 0) - u(0, 0, 0)) * rdx2 - nux1 * (u(0, 0, 0) - u(-1, 0, 0)) * rdx1) +
                  rdy3 * (nuy2 * (dudy1 + (v(1, 0, 0) - v(0, 0, 0)) * rdx2) -
                          nuy1 * (dudy0 + (v(1, -1, 0) - v(0, -1, 0)) * rdx1)) +
@@ -70,16 +88,16 @@ __device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, i
     if (do_v) {
         R c1 = half * (v(0, 0, 0) + v(0, 1, 0));
         R c0 = half * (v(0, -1, 0) + v(0, 0, 0));
-        R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdyc[j];
+        R t1 = -(c1 * c1 - c0 * c0) * pm.ryc;
         R uv1 = (w0j * u(0, 0, 0) + w1j * u(0, 1, 0)) * (half * (v(0, 0, 0) + v(1, 0, 0)));
         R uv0 = (w0j * u(-1, 0, 0) + w1j * u(-1, 1, 0)) * (half * (v(-1, 0, 0) + v(0, 0, 0)));
-        R t2 = -(uv1 - uv0) * (R)m.rdxf[i];
+        R t2 = -(uv1 - uv0) * pm.rxf;
         R vw1 = (half * (v(0, 0, 0) + v(0, 0, 1))) * (w0j * w(0, 0, 0) + w1j * w(0, 1, 0));
         R vw0 = (half * (v(0, 0, -1) + v(0, 0, 0))) * (w0j * w(0, 0, -1) + w1j * w(0, 1, -1));
-        R t3 = -(vw1 - vw0) * (R)m.rdzf[k];
-        R rdz1 = (R)m.rdzc[k - 1], rdz2 = (R)m.rdzc[k], rdz3 = (R)m.rdzf[k];
-        R rdy1 = (R)m.rdyf[j], rdy2 = (R)m.rdyf[j + 1], rdy3 = (R)m.rdyc[j];
-        R rdx1 = (R)m.rdxc[i - 1], rdx2 = (R)m.rdxc[i], rdx3 = (R)m.rdxf[i];
+        R t3 = -(vw1 - vw0) * pm.rzf;
+        R rdz1 = pm.rzcm, rdz2 = pm.rzc, rdz3 = pm.rzf;
+        R rdy1 = pm.ryf, rdy2 = pm.ryfp, rdy3 = pm.ryc;
+        R rdx1 = pm.rxcm, rdx2 = pm.rxc, rdx3 = pm.rxf;
         R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
         if (NUT) {
             nux1 = nu + half * (w0j * nt(-1, 0, 0) + w1j * nt(-1, 1, 0) + w0j * nt(0, 0, 0) + w1j * nt(0, 1, 0));
@@ -107,16 +125,16 @@ __device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, i
     if (do_w) {
         R c1 = half * (w(0, 0, 0) + w(0, 0, 1));
         R c0 = half * (w(0, 0, -1) + w(0, 0, 0));
-        R t1 = -(c1 * c1 - c0 * c0) * (R)m.rdzc[k];
+        R t1 = -(c1 * c1 - c0 * c0) * pm.rzc;
         R uw1 = (half * (u(0, 0, 0) + u(0, 0, 1))) * (half * (w(0, 0, 0) + w(1, 0, 0)));
         R uw0 = (half * (u(-1, 0, 0) + u(-1, 0, 1))) * (half * (w(-1, 0, 0) + w(0, 0, 0)));
-        R t2 = -(uw1 - uw0) * (R)m.rdxf[i];
+        R t2 = -(uw1 - uw0) * pm.rxf;
         R vw1 = (half * (v(0, 0, 0) + v(0, 0, 1))) * (w0j * w(0, 0, 0) + w1j * w(0, 1, 0));
         R vw0 = (half * (v(0, -1, 0) + v(0, -1, 1))) * (w0m * w(0, -1, 0) + w1m * w(0, 0, 0));
-        R t3 = -(vw1 - vw0) * (R)m.rdyf[j];
-        R rdz1 = (R)m.rdzf[k], rdz2 = (R)m.rdzf[k + 1], rdz3 = (R)m.rdzc[k];
-        R rdy3 = (R)m.rdyf[j];
-        R rdx1 = (R)m.rdxc[i - 1], rdx2 = (R)m.rdxc[i], rdx3 = (R)m.rdxf[i];
+        R t3 = -(vw1 - vw0) * pm.ryf;
+        R rdz1 = pm.rzf, rdz2 = pm.rzfp, rdz3 = pm.rzc;
+        R rdy3 = pm.ryf;
+        R rdx1 = pm.rxcm, rdx2 = pm.rxc, rdx3 = pm.rxf;
         R nux1 = nu, nux2 = nu, nuy1 = nu, nuy2 = nu, nuz1 = nu, nuz2 = nu;
         if (NUT) {
             nux1 = nu + quart * (nt(0, 0, 0) + nt(0, 0, 1) + nt(-1, 0, 0) + nt(-1, 0, 1));
@@ -126,8 +144,8 @@ __device__ __forceinline__ void rhs_point(const A &a, const Metrics &m, int i, i
             nuz1 = nu + nt(0, 0, 0);
             nuz2 = nu + nt(0, 0, 1);
         }
-        R dwdy1 = (w(0, 1, 0) - w(0, 0, 0)) * (R)m.rdyc[j];
-        R dwdy0 = (w(0, 0, 0) - w(0, -1, 0)) * (R)m.rdyc[j - 1];
+        R dwdy1 = (w(0, 1, 0) - w(0, 0, 0)) * pm.ryc;
+        R dwdy0 = (w(0, 0, 0) - w(0, -1, 0)) * pm.rycm;
         R visc = rdx3 * (nux2 * ((w(1, 0, 0) - w(0, 0, 0)) * rdx2 + (u(0, 0, 1) - u(0, 0, 0)) * rdz2) -
                          nux1 * ((w(0, 0, 0) - w(-1, 0, 0)) * rdx1 + (u(-1, 0, 1) - u(-1, 0, 0)) * rdz1)) +
                  rdy3 * (nuy2 * (dwdy1 + (v(0, 0, 1) - v(0, 0, 0)) * rdz2) -
@@ -220,21 +238,31 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const bool active = i <= nx - 1 && j <= nyg - 1;
 
     R pu[NPF], pv[NPF], pw[NPF], pn[NPF];
+    // in-plane offsets and bounds of this thread's halo-tile elements, fixed
+    // along the k march (plane strides added per plane)
+    int ou[NPF], ov[NPF], ow[NPF];
+    unsigned char ok[NPF];
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+        const int e = tid + q * NT_;
+        const int ex = e % LX, ey = e / LX;
+        const int gi = i0 - 1 + ex, gj = j0 - 1 + ey;
+        ou[q] = (gi - 1) + nx * (gj - 1);
+        ov[q] = (gi - 1) + nxg * (gj - 1);
+        ow[q] = (gi - 1) + nxg * (gj - 1);
+        ok[q] = (e < PL) ? (unsigned char)(((gj <= nyg && gi <= nx) ? 1 : 0) | ((gj <= nyg && gi <= nxg) ? 2 : 0) |
+                                           ((gj <= ny && gi <= nxg) ? 4 : 0))
+                         : (unsigned char)0;
+    }
+    const int64_t psu = (int64_t)nx * nyg, psv = (int64_t)nxg * ny, psw = (int64_t)nxg * nyg;
     auto fetch = [&](int kk) {  // global -> registers (tile + halo of plane kk)
 #pragma unroll
         for (int q = 0; q < NPF; q++) {
-            const int e = tid + q * NT_;
-            const int ex = e % LX, ey = e / LX;
-            const int gi = i0 - 1 + ex, gj = j0 - 1 + ey;
             R vu = 0, vv = 0, vw = 0, vn = 0;
-            if (e < PL) {
-                if (gj <= nyg) {
-                    if (gi <= nx && kk <= nzg) vu = (R)U[i3(nx, nyg, gi, gj, kk)];
-                    if (gi <= nxg && kk <= nz) vw = (R)W[i3(nxg, nyg, gi, gj, kk)];
-                    if (NUT && gi <= nxg && kk <= nzg) vn = (R)NT[i3(nxg, nyg, gi, gj, kk)];
-                }
-                if (gj <= ny && gi <= nxg && kk <= nzg) vv = (R)V[i3(nxg, ny, gi, gj, kk)];
-            }
+            if ((ok[q] & 1) && kk <= nzg) vu = (R)U[(int64_t)(kk - 1) * psu + ou[q]];
+            if ((ok[q] & 2) && kk <= nz) vw = (R)W[(int64_t)(kk - 1) * psw + ow[q]];
+            if (NUT && (ok[q] & 2) && kk <= nzg) vn = (R)NT[(int64_t)(kk - 1) * psw + ow[q]];
+            if ((ok[q] & 4) && kk <= nzg) vv = (R)V[(int64_t)(kk - 1) * psv + ov[q]];
             pu[q] = vu; pv[q] = vv; pw[q] = vw; pn[q] = vn;
         }
     };
@@ -269,16 +297,21 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     }
     const R nu = (R)nu_d, om = (R)om_d, dPdx = (R)sc->dPdx, dPdz = (R)sc->dPdz;
     const double dt = sc->dt;
+    PM<R> pm;
+    pm.set_ij(m, min(i, nx), min(j, ny));
     const R cnew = (R)(dt * a_new), cg = (R)(dt * a_g);
     for (int k = kbeg; k < kend; k++) {
         const int r0 = k - kbeg;
         if (active) {
             a.slot[0] = r0 & 3; a.slot[1] = (r0 + 1) & 3; a.slot[2] = (r0 + 2) & 3;
             R ru = 0, rv = 0, rw = 0;
-            rhs_point<R, NUT, ROT>(a, m, i, j, k, nu, om, dPdx, dPdz, true, j <= ny - 1, true, ru, rv, rw);
+            pm.set_k(m, k);
+            rhs_point<R, NUT, ROT>(a, pm, nu, om, dPdx, dPdz, true, j <= ny - 1, true, ru, rv, rw);
             const int c = a.slot[1] * PL + a.lj * LX + a.li;
             const R cu = lu[c], cv = lv[c], cw = lw[c];
-            const int64_t qu = i3(nx, nyg, i, j, k), qv = i3(nxg, ny, i, j, k), qw = i3(nxg, nyg, i, j, k);
+            const int64_t qu = (int64_t)(k - 1) * psu + (i - 1) + nx * (j - 1),
+                          qv = (int64_t)(k - 1) * psv + (i - 1) + nxg * (j - 1),
+                          qw = (int64_t)(k - 1) * psw + (i - 1) + nxg * (j - 1);
             if (SUB == 1) {
                 Uo[qu] = (S)(cu + cnew * ru);
                 Gu[qu] = (S)(cu + cg * ru);
@@ -323,8 +356,10 @@ k_rhs_only(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     GAcc<S, R> a{U, V, W, NT, ubar, i, j, k, d.nx, d.nxg, d.nyg, d.ny};
     const bool du = i <= d.nx - 1 && Ru, dv = j <= d.ny - 1 && Rv, dw = k <= d.nz - 1 && Rw;
     R ru = 0, rv = 0, rw = 0;
-    rhs_point<R, NUT, ROT>(a, m, i, j, k, (R)nu_d, (R)om_d, (R)sc->dPdx, (R)sc->dPdz, du, dv, dw, ru, rv,
-                           rw);
+    PM<R> pm;
+    pm.set_ij(m, i, j);
+    pm.set_k(m, k);
+    rhs_point<R, NUT, ROT>(a, pm, (R)nu_d, (R)om_d, (R)sc->dPdx, (R)sc->dPdz, du, dv, dw, ru, rv, rw);
     if (du) Ru[i3(d.nx - 2, d.nyg - 2, i - 1, j - 1, k - 1)] = (double)ru;
     if (dv) Rv[i3(d.nxg - 2, d.ny - 2, i - 1, j - 1, k - 1)] = (double)rv;
     if (dw) Rw[i3(d.nxg - 2, d.nyg - 2, i - 1, j - 1, k - 1)] = (double)rw;
