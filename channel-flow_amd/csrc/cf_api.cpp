@@ -36,10 +36,13 @@ std::string last_error_copy() { return g_err; }
         }                                                                                      \
     } while (0)
 
-static FFTPlan make_plan(int n, std::string &err) {
+// radix 8 pays for the z transform (many lines per workgroup); the x
+// transforms keep more threads busy per pass with radix 4
+static FFTPlan make_plan(int n, std::string &err, bool radix8) {
     FFTPlan p{};
     p.n = n;
     int m = n;
+    while (radix8 && m % 8 == 0 && p.nr < 16) { p.radix[p.nr++] = 8; m /= 8; }
     while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
     while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
@@ -295,8 +298,8 @@ int Ctx<S>::init() {
     wt.jjz = m_jjz; wt.wz = m_wz; wt.jjx = m_jjx; wt.wx = m_wx;
     // FFT plans and twiddles: x uses a length-M complex FFT plus the N-point split
     std::string err;
-    pM = make_plan(M, err);
-    pZ = make_plan(d.nzp, err);
+    pM = make_plan(M, err, false);
+    pZ = make_plan(d.nzp, err, true);
     if (!err.empty()) { set_error(err); return -1; }
     auto twid = [](int n) {
         std::vector<Cx<S>> t(n + 1);

@@ -13,7 +13,7 @@
 // (one workgroup holds consecutive z rows of one y plane), and the fused
 // z-transform / y-solve kernels read whole contiguous kz rows.
 //
-// The FFT engine is an LDS Stockham autosort transform (radix 4, 2, 3) over a
+// The FFT engine is an LDS Stockham autosort transform (radix 8, 4, 2, 3) over a
 // batch of lines per workgroup; each pass stages its butterflies through
 // registers so a single LDS buffer suffices.
 #include <cstdlib>
@@ -38,6 +38,28 @@ __device__ __forceinline__ void dft4(Cx<T> *a, int sign) {
     a[1] = cadd(d02, wd);
     a[3] = csub(d02, wd);
 }
+// radix 8 as two radix-4 transforms of the even and odd samples and one
+// radix-2 stage with the twiddles W8^k = exp(sign 2 pi i k / 8)
+template <typename T>
+__device__ __forceinline__ void dft8(Cx<T> *a, int sign) {
+    Cx<T> e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+    dft4(e, sign);
+    dft4(o, sign);
+    const T c = (T)0.70710678118654752440084436210485;
+    const T sg = sign < 0 ? (T)-1 : (T)1;
+    const Cx<T> w1 = {c, sg * c}, w3 = {-c, sg * c};
+    Cx<T> t[4];
+    t[0] = o[0];
+    t[1] = cmul(o[1], w1);
+    t[2] = Cx<T>{-sg * o[2].y, sg * o[2].x};  // times sign * i
+    t[3] = cmul(o[3], w3);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        a[k] = cadd(e[k], t[k]);
+        a[k + 4] = csub(e[k], t[k]);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void dft3(Cx<T> *a, int sign) {
     const T h = (T)0.5, s3 = (T)0.86602540378443864676372317075294;
@@ -50,6 +72,15 @@ __device__ __forceinline__ void dft3(Cx<T> *a, int sign) {
     a[2] = csub(m, r);
 }
 
+// a = q*b + r for 0 <= a < 2^20 and 1 <= b <= 2^12 with a precomputed 1/b:
+// the float quotient is off by at most one, which the remainder test fixes
+__device__ __forceinline__ void small_divmod(int a, int b, float inv_b, int &q, int &r) {
+    q = (int)((float)a * inv_b);
+    r = a - q * b;
+    if (r < 0) { q -= 1; r += b; }
+    else if (r >= b) { q += 1; r -= b; }
+}
+
 // One Stockham pass over `nlines` lines of length n held in LDS (line stride
 // ls).  MAXIT bounds the butterflies per thread: callers guarantee
 // nlines*n <= 8*WG.
@@ -59,13 +90,15 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
     constexpr int MAXIT = (8 + R - 1) / R;
     const int nb = n / R, items = nb * nlines;
     const int stw = n / (Ns * R);
+    const float inb = 1.0f / (float)nb, ins = 1.0f / (float)Ns;
     Cx<T> v[MAXIT][R];
 #pragma unroll
     for (int c = 0; c < MAXIT; c++) {
         const int it = threadIdx.x + c * WG;
         if (it < items) {
-            const int line = it / nb, jj = it - line * nb;
-            const int k = jj % Ns;
+            int line, jj, q, k;
+            small_divmod(it, nb, inb, line, jj);
+            small_divmod(jj, Ns, ins, q, k);
             const Cx<T> *src = buf + line * ls;
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -79,6 +112,7 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
             }
             if (R == 2) dft2(v[c]);
             else if (R == 4) dft4(v[c], sign);
+            else if (R == 8) dft8(v[c], sign);
             else dft3(v[c], sign);
         }
     }
@@ -87,9 +121,10 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
     for (int c = 0; c < MAXIT; c++) {
         const int it = threadIdx.x + c * WG;
         if (it < items) {
-            const int line = it / nb, jj = it - line * nb;
-            const int k = jj % Ns;
-            const int base = (jj / Ns) * Ns * R + k;
+            int line, jj, q, k;
+            small_divmod(it, nb, inb, line, jj);
+            small_divmod(jj, Ns, ins, q, k);
+            const int base = q * Ns * R + k;
             Cx<T> *dst = buf + line * ls;
 #pragma unroll
             for (int r = 0; r < R; r++) dst[base + r * Ns] = v[c][r];
@@ -104,7 +139,8 @@ __device__ void lds_fft(Cx<T> *buf, int ls, int nlines, const FFTPlan &p, const 
     int Ns = 1;
     for (int s = 0; s < p.nr; s++) {
         const int R = p.radix[s];
-        if (R == 4) fft_pass<4, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
+        if (R == 8) fft_pass<8, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
+        else if (R == 4) fft_pass<4, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else if (R == 2) fft_pass<2, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else fft_pass<3, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         Ns *= R;
@@ -129,6 +165,8 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const int row0 = blockIdx.x * rb;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
     T *ldr = reinterpret_cast<T *>(lds);
+    Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
+    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[e];
     for (int e = threadIdx.x; e < rb * N; e += 256) {
         const int lr = e / N, ip = e - lr * N;
         const int r = row0 + lr;
@@ -142,7 +180,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         ldr[lr * 2 * ls + ip] = dv;
     }
     __syncthreads();
-    lds_fft<256>(lds, ls, rb, pM, twM, -1);
+    lds_fft<256>(lds, ls, rb, pM, tw, -1);
     const int mh = M + 1;
     // consecutive threads take consecutive rows (kz) of one kx: runs of rb
     for (int e = threadIdx.x; e < rb * mh; e += 256) {
@@ -178,12 +216,14 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
     const int64_t c0 = (int64_t)blockIdx.x * bcols;
     const int nb = (int)min((int64_t)bcols, ncols - c0);
     Cx<T> *base = spec + c0 * n;
+    Cx<T> *tw = lds + bcols * ls;
+    for (int e = threadIdx.x; e < n; e += 512) tw[e] = twZ[e];
     for (int e = threadIdx.x; e < nb * n; e += 512) {
         const int b = e / n, kz = e - b * n;
         lds[b * ls + kz] = base[e];
     }
     __syncthreads();
-    lds_fft<512>(lds, ls, nb, pZ, twZ, sign);
+    lds_fft<512>(lds, ls, nb, pZ, tw, sign);
     for (int e = threadIdx.x; e < nb * n; e += 512) {
         const int b = e / n, kz = e - b * n;
         base[e] = lds[b * ls + kz];
@@ -389,6 +429,8 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const int M = pM.n, N = 2 * M, mh = M + 1, ls = mh;
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
+    Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
+    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[e];
     // consecutive threads read consecutive rows (kz) of one kx: runs of rb
     for (int e = threadIdx.x; e < rb * mh; e += 256) {
         const int kx = e / rb, lr = e - kx * rb;
@@ -422,7 +464,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         if (k != 0 && k2 != k) row[k2] = z(B, A, k2);
     }
     __syncthreads();
-    lds_fft<256>(lds, ls, rb, pM, twM, +1);
+    lds_fft<256>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;
     for (int e = threadIdx.x; e < rb * N; e += 256) {
@@ -459,7 +501,7 @@ int fft_rows_per_wg(int M, int elems_per_row, int esize) {
     const bool inv = elems_per_row > M;
     const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", inv ? 16 : 8);
     if (cap > 0 && rb > cap) rb = cap;
-    const int lim = 65536 / (elems_per_row * esize);
+    const int lim = (65536 / esize - M) / elems_per_row;  // + M twiddles
     if (rb > lim) rb = lim;
     if (rb < 1) rb = 1;
     return rb;
@@ -471,7 +513,7 @@ int fft_cols_per_wg(int n, int esize) {
     if (b > 16) b = 16;
     const int cap = env_int("CF_FFT_COLS", 0);
     if (cap > 0 && b > cap) b = cap;
-    const int lim = 65536 / ((n + 1) * esize);
+    const int lim = (65536 / esize - n) / (n + 1);  // + n twiddles
     if (b > lim) b = lim;
     if (b < 1) b = 1;
     return b;
@@ -494,7 +536,7 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
                      hipStream_t st) {
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = (size_t)rb * pM.n * sizeof(Cx<T>);
+    const size_t shm = ((size_t)rb + 1) * pM.n * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_div_fftx<S, T>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m, U, V,
                        W, spec, pM, twM, twN, rb);
 }
@@ -504,7 +546,7 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
                  hipStream_t st) {
     const int b = fft_cols_per_wg(pZ.n, (int)sizeof(Cx<T>));
     const int64_t ncols = (int64_t)d.nyp * d.mh;
-    const size_t shm = (size_t)b * (pZ.n + 1) * sizeof(Cx<T>);
+    const size_t shm = ((size_t)b * (pZ.n + 1) + pZ.n) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_fftz<T>), dim3((unsigned)((ncols + b - 1) / b)), dim3(512), shm, st, d, spec, pZ, twZ,
                        b, sign);
 }
@@ -537,7 +579,7 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
                         hipStream_t st) {
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = (size_t)rb * (pM.n + 1) * sizeof(Cx<T>);
+    const size_t shm = ((size_t)rb * (pM.n + 1) + pM.n) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_ifftx_proju<S, T, P>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m,
                        spec, phi, U, pM, twM, twN, rb, norm);
 }

@@ -5,7 +5,7 @@
 
 namespace cf {
 
-// Factorisation of one FFT length into radices 4, 2, 3 (applied in order).
+// Factorisation of one FFT length into radices 8, 4, 2, 3 (applied in order).
 struct FFTPlan {
     int n;
     int nr;

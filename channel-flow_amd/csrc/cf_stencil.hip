@@ -3,6 +3,8 @@
 // reductions, divergence check, constant-coefficient Smagorinsky and plane
 // statistics.  All are HBM-streaming kernels: one thread per unique cell,
 // x fastest, tiles remapped per XCD.
+#include <cstdlib>
+
 #include "cf_kernels.hpp"
 
 namespace cf {
@@ -448,29 +450,39 @@ __global__ void k_ghosts(Dims d, S *__restrict__ F, int field, int wall_mode,
 // the inverse-x-FFT kernel that produces the same row of phi.
 // phi holds unique cells: phi[((k-2)*nyp + (j-2))*nxp + (i-2)].
 // ---------------------------------------------------------------------------
+// Each thread marches over a chunk of kc planes at fixed (i, j), carrying
+// phi(k) from one plane to the next, so phi streams through once; the
+// phi(j+1) neighbour is the row the next thread row loads (cache hit).
 template <typename S, typename P>
 __global__ void __launch_bounds__(256)
 k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
-             S *__restrict__ W, int tiles_x, int tiles_y, int ntiles) {
+             S *__restrict__ W, int tiles_x, int tiles_y, int ntiles, int kc) {
     const int t = xcd_remap(blockIdx.x, ntiles);
-    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tc = t / (tiles_x * tiles_y);
     const int i = 2 + tx * 64 + (threadIdx.x & 63);
     const int j = 2 + ty * 4 + (threadIdx.x >> 6);
-    const int k = 2 + tz;
+    const int kb = 2 + tc * kc, ke = min(kb + kc, d.nzg - 1);  // planes kb..ke-1
     if (i > d.nx - 1 || j > d.nyg - 1) return;
     const int nxp = d.nxp, nyp = d.nyp;
-    const int64_t pc = ((int64_t)(k - 2) * nyp + (j - 2)) * nxp + (i - 2);
-    const double pc_v = (double)phi[pc];
-    if (j <= d.ny - 1) {
-        const double pn = (double)phi[pc + nxp];
-        const int64_t q = i3(d.nxg, d.ny, i, j, k);
-        V[q] = (S)((double)V[q] - (pn - pc_v) * m.rdyc[j]);
-    }
-    {
+    const int64_t pps = (int64_t)nyp * nxp;  // phi plane stride
+    const P *pcol = phi + (int64_t)(j - 2) * nxp + (i - 2);
+    const bool dov = j <= d.ny - 1;
+    const double ry = m.rdyc[j];
+    const int64_t vs = (int64_t)d.nxg * d.ny, ws = (int64_t)d.nxg * d.nyg;
+    S *vp = V + i3(d.nxg, d.ny, i, j, kb);
+    S *wp = W + i3(d.nxg, d.nyg, i, j, kb);
+    double pc = (double)pcol[(int64_t)(kb - 2) * pps];
+    for (int k = kb; k < ke; k++) {
         const int kn = (k + 1 == d.nzg - 1) ? 2 : k + 1;  // periodic image of phi(:,:,nz)
-        const double pn = (double)phi[((int64_t)(kn - 2) * nyp + (j - 2)) * nxp + (i - 2)];
-        const int64_t q = i3(d.nxg, d.nyg, i, j, k);
-        W[q] = (S)((double)W[q] - (pn - pc_v) * m.rdzc[k]);
+        const double pn = (double)pcol[(int64_t)(kn - 2) * pps];
+        if (dov) {
+            const double pj = (double)pcol[(int64_t)(k - 2) * pps + nxp];
+            *vp = (S)((double)*vp - (pj - pc) * ry);
+        }
+        *wp = (S)((double)*wp - (pn - pc) * m.rdzc[k]);
+        pc = pn;
+        vp += vs;
+        wp += ws;
     }
 }
 
@@ -883,8 +895,12 @@ void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *
 
 template <typename S, typename P>
 void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st) {
-    const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * d.nzp;
-    hipLaunchKernelGGL((k_project_vw<S, P>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt);
+    // planes per thread: a long march hides the latency of the narrow FP32
+    // accesses; FP64 rows move enough bytes per access with one plane
+    static const int kc_env = getenv("CF_PV_KC") ? atoi(getenv("CF_PV_KC")) : 0;
+    const int kc = kc_env > 0 ? kc_env : (sizeof(S) == 4 ? 16 : 1);
+    const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * ((d.nzp + kc - 1) / kc);
+    hipLaunchKernelGGL((k_project_vw<S, P>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc);
 }
 
 template <typename S>
