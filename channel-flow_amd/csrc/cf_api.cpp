@@ -311,6 +311,7 @@ int Ctx<S>::init() {
     };
     if (upload(twM, twid(M)) || upload(twN, twid(d.nxp)) || upload(twZ, twid(d.nzp))) return -1;
 
+
     // wall model buffers
     HCK(hipMemsetAsync(alpha_x, 0, nwx * sizeof(double), stream));
     HCK(hipMemsetAsync(alpha_z, 0, nwz * sizeof(double), stream));
@@ -377,10 +378,14 @@ void Ctx<S>::ghosts(S *const *F, int mask) {
     const Dims &d = h.d;
     const bool wm = h.prm.iwall_model == 3;
     const double dgb = h.yg[1] - h.yg[0], dgt = h.yg[d.nyg - 1] - h.yg[d.nyg - 2];
+    S *fs[3];
+    int fld[3], md[3], n = 0;
+    const double *al[3];
+    if (mask & 1) { fs[n] = F[0]; fld[n] = 0; md[n] = wm ? 1 : 0; al[n] = alpha_x; n++; }
+    if (mask & 2) { fs[n] = F[1]; fld[n] = 1; md[n] = 0; al[n] = nullptr; n++; }
+    if (mask & 4) { fs[n] = F[2]; fld[n] = 2; md[n] = wm ? 1 : 0; al[n] = alpha_z; n++; }
     tic("ghosts");
-    if (mask & 1) launch_ghosts<S>(d, F[0], 0, wm ? 1 : 0, alpha_x, dgb, dgt, stream);
-    if (mask & 2) launch_ghosts<S>(d, F[1], 1, 0, nullptr, dgb, dgt, stream);
-    if (mask & 4) launch_ghosts<S>(d, F[2], 2, wm ? 1 : 0, alpha_z, dgb, dgt, stream);
+    launch_ghosts3<S>(d, fs, fld, md, al, n, dgb, dgt, stream);
     toc();
 }
 

@@ -233,8 +233,9 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 // ---------------------------------------------------------------------------
 // Tridiagonal solve in y, one lane per (kx,kz) mode: forward sweep keeping c'
 // in an FP64 scratch and d' in the spectrum, then back substitution.  Four
-// passes over the spectrum, but no serial phase: the faster form when the
-// spectrum is FP32 (its passes are half as large).
+// passes over the spectrum and two over the scratch (written and read back
+// while still in the last-level cache), but no serial phase: the faster form
+// when the spectrum is FP32 (its passes are half as large).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256)

@@ -25,6 +25,9 @@ void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, co
 template <typename S>
 void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *alpha,
                    double dyg_bot, double dyg_top, hipStream_t st);
+template <typename S>
+void launch_ghosts3(const Dims &d, S *const *F, const int *field, const int *wall_mode,
+                    const double *const *alpha, int n, double dyg_bot, double dyg_top, hipStream_t st);
 template <typename S, typename P>
 void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st);
 template <typename S>
@@ -66,6 +69,7 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
 // the y solve form for a spectrum element of esize bytes needs an FP64
 // scratch of one double per spectrum element
 bool thomas_uses_scratch(int esize);
+
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st);

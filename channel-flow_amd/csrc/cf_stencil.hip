@@ -386,11 +386,13 @@ __device__ __forceinline__ int img_ctr(int i, int ng) {  // centres 1..ng, uniqu
     return i == 1 ? ng - 2 : (i == ng - 1 ? 2 : (i == ng ? 3 : i));
 }
 
+// ghost element g of one field (ordering: wall rows, z ghost planes, x ghost
+// columns); wall_mode: 0 Dirichlet (negate / zero), 1 nonzero Neumann with
+// alpha, 2 Neumann (copy)
 template <typename S>
-__global__ void k_ghosts(Dims d, S *__restrict__ F, int field, int wall_mode,
-                         const double *__restrict__ alpha, double dyg_bot, double dyg_top) {
-    // wall_mode: 0 Dirichlet (negate / zero), 1 nonzero Neumann with alpha,
-    //            2 Neumann (copy)
+__device__ __forceinline__ void ghost_one(const Dims &d, S *__restrict__ F, int field, int wall_mode,
+                                          const double *__restrict__ alpha, double dyg_bot, double dyg_top,
+                                          int64_t g) {
     int n1, n2, n3;
     bool xf, zf, yf;
     if (field == 0) { n1 = d.nx; n2 = d.nyg; n3 = d.nzg; xf = true; yf = false; zf = false; }
@@ -402,7 +404,6 @@ __global__ void k_ghosts(Dims d, S *__restrict__ F, int field, int wall_mode,
     const int64_t nY = (int64_t)n1 * 2 * n3;  // wall rows
     const int64_t nZ = (int64_t)n1 * (n2 - 2) * nzgh;
     const int64_t nX = (int64_t)nxgh * (n2 - 2) * (n3 - nzgh);
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (g < nY) {
         i = 1 + (int)(g % n1);
@@ -443,6 +444,38 @@ __global__ void k_ghosts(Dims d, S *__restrict__ F, int field, int wall_mode,
         val = F[i3(n1, n2, ii, j, kk)];
     }
     F[i3(n1, n2, i, j, k)] = val;
+}
+
+int64_t ghost_count(const Dims &d, int field) {
+    int n1, n2, n3, nxgh, nzgh;
+    if (field == 0) { n1 = d.nx; n2 = d.nyg; n3 = d.nzg; nxgh = 2; nzgh = 3; }
+    else if (field == 1) { n1 = d.nxg; n2 = d.ny; n3 = d.nzg; nxgh = 3; nzgh = 3; }
+    else if (field == 2) { n1 = d.nxg; n2 = d.nyg; n3 = d.nz; nxgh = 3; nzgh = 2; }
+    else { n1 = d.nxg; n2 = d.nyg; n3 = d.nzg; nxgh = 3; nzgh = 3; }
+    return (int64_t)n1 * 2 * n3 + (int64_t)n1 * (n2 - 2) * nzgh + (int64_t)nxgh * (n2 - 2) * (n3 - nzgh);
+}
+
+// ghosts of up to three fields in one launch: the thread range is split
+// into consecutive segments, one per field (each ghost is written once from
+// unique cells, so the fields are independent)
+struct GhostSet {
+    void *F[3];
+    int field[3], mode[3];
+    const double *alpha[3];
+    int64_t end[3];  // cumulative counts
+    int n;
+};
+template <typename S>
+__global__ void k_ghosts(Dims d, GhostSet gs, double dyg_bot, double dyg_top) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t beg = 0;
+    for (int f = 0; f < gs.n; f++) {
+        if (g < gs.end[f]) {
+            ghost_one<S>(d, (S *)gs.F[f], gs.field[f], gs.mode[f], gs.alpha[f], dyg_bot, dyg_top, g - beg);
+            return;
+        }
+        beg = gs.end[f];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -881,16 +914,26 @@ void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, co
 template <typename S>
 void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *alpha,
                    double dyg_bot, double dyg_top, hipStream_t st) {
-    int n1, n2, n3, nxgh, nzgh;
-    if (field == 0) { n1 = d.nx; n2 = d.nyg; n3 = d.nzg; nxgh = 2; nzgh = 3; }
-    else if (field == 1) { n1 = d.nxg; n2 = d.ny; n3 = d.nzg; nxgh = 3; nzgh = 3; }
-    else if (field == 2) { n1 = d.nxg; n2 = d.nyg; n3 = d.nz; nxgh = 3; nzgh = 2; }
-    else { n1 = d.nxg; n2 = d.nyg; n3 = d.nzg; nxgh = 3; nzgh = 3; }
-    const int64_t n = (int64_t)n1 * 2 * n3 + (int64_t)n1 * (n2 - 2) * nzgh +
-                      (int64_t)nxgh * (n2 - 2) * (n3 - nzgh);
-    const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL((k_ghosts<S>), dim3(blocks), dim3(256), 0, st, d, F, field, wall_mode, alpha,
-                       dyg_bot, dyg_top);
+    launch_ghosts3<S>(d, &F, &field, &wall_mode, &alpha, 1, dyg_bot, dyg_top, st);
+}
+
+template <typename S>
+void launch_ghosts3(const Dims &d, S *const *F, const int *field, const int *wall_mode,
+                    const double *const *alpha, int n, double dyg_bot, double dyg_top, hipStream_t st) {
+    GhostSet gs{};
+    int64_t tot = 0;
+    for (int f = 0; f < n; f++) {
+        gs.F[f] = F[f];
+        gs.field[f] = field[f];
+        gs.mode[f] = wall_mode[f];
+        gs.alpha[f] = alpha[f];
+        tot += ghost_count(d, field[f]);
+        gs.end[f] = tot;
+    }
+    gs.n = n;
+    if (tot == 0) return;
+    hipLaunchKernelGGL((k_ghosts<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d, gs, dyg_bot,
+                       dyg_top);
 }
 
 template <typename S, typename P>
@@ -979,6 +1022,8 @@ INST(float, float)
 #define INST1(S)                                                                                       \
     template void launch_ghosts<S>(const Dims &, S *, int, int, const double *, double, double,       \
                                    hipStream_t);                                                       \
+    template void launch_ghosts3<S>(const Dims &, S *const *, const int *, const int *,               \
+                                    const double *const *, int, double, double, hipStream_t);          \
     template void launch_massflow<S>(const Dims &, const Metrics &, const S *, double *, Scalars *,  \
                                      int, hipStream_t);                                                \
     template void launch_add_corr_u<S>(const Dims &, S *, const Scalars *, hipStream_t);              \
@@ -1021,15 +1066,19 @@ __global__ void k_advance_time(Scalars *sc) { sc->t = sc->t + 1.0 * sc->dt; }
 // correction adds corr to U on the interior (time_integration.f90:155), i.e.
 // to ubar(2..nyg-1); the ghost entries follow the wall condition the ghost
 // kernel applies to U' (Dirichlet: antisymmetric, Neumann: symmetric).
-__global__ void k_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann) {
+__global__ void __launch_bounds__(256) k_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann) {
     const double c = sc->corr;
-    for (int j = 2; j <= nyg - 1; j++) ubar[j] += c;
-    ubar[1] = neumann ? ubar[2] : -ubar[2];
-    ubar[nyg] = neumann ? ubar[nyg - 1] : -ubar[nyg - 1];
-    sc->qbar_u = sc->qbar_u + c * sc->norm_u;
+    __syncthreads();  // every thread has read corr before thread 0 moves on
+    for (int j = 2 + threadIdx.x; j <= nyg - 1; j += blockDim.x) ubar[j] += c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ubar[1] = neumann ? ubar[2] : -ubar[2];
+        ubar[nyg] = neumann ? ubar[nyg - 1] : -ubar[nyg - 1];
+        sc->qbar_u = sc->qbar_u + c * sc->norm_u;
+    }
 }
 void launch_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann, hipStream_t st) {
-    hipLaunchKernelGGL(k_ubar_shift, dim3(1), dim3(1), 0, st, sc, ubar, nyg, neumann);
+    hipLaunchKernelGGL(k_ubar_shift, dim3(1), dim3(256), 0, st, sc, ubar, nyg, neumann);
 }
 
 void launch_reset_counters(Scalars *sc, hipStream_t st) {
