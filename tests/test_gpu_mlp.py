@@ -207,3 +207,41 @@ def test_gain_on_c4_wall(trained):
     assert t_mlp < t_ode
     assert np.median(rel) < 1e-2 and np.percentile(rel, 99) < 5e-2
     g.set_wall_surrogate(None)
+
+
+def test_wmles_runs_with_surrogate_track_eqwm(trained):
+    """The 256x96x256 wall-modelled LES run with the learned wall model
+    against the same run with the ODE, from the same initial field: mean
+    velocity profile over a window of steps within 1%, and the whole RK3 step
+    time of both."""
+    from channel_flow.solver import ChannelFlow
+    m, _ = trained
+    cfg = cases.case_c4()
+    steps, start, every = 300, 100, 5
+    out = {}
+    for name, model in (("ode", None), ("mlp", m)):
+        g = ChannelFlow(cfg)
+        U, V, W = _loglaw_state(g, cfg)
+        g.initialise(U, V, W)
+        if model is not None:
+            g.set_wall_surrogate(model)
+        g.run(start, graph=True)
+        g.stats_reset()
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range((steps - start) // every):
+            g.run(every, graph=True)
+            g.stats_accumulate()
+        g.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / (steps - start)
+        w = g.stats_window()
+        assert np.all(np.isfinite(w["Umean"]))
+        out[name] = (w, ms, g.wall_fallbacks() if model is not None else 0)
+        g.set_wall_surrogate(None)
+        g.close()
+    (wo, mso, _), (wm, msm, nf) = out["ode"], out["mlp"]
+    err = float(np.max(np.abs(wm["Umean"] - wo["Umean"])) / np.max(np.abs(wo["Umean"])))
+    err_uu = float(np.max(np.abs(wm["uu"] - wo["uu"])) / np.max(np.abs(wo["uu"])))
+    print("\n[mlp] c4 run, steps %d-%d: mean-profile err %.3e, uu err %.3e; step %.3f ms (ODE) "
+          "vs %.3f ms (MLP), last-step fallbacks %d" % (start + 1, steps, err, err_uu, mso, msm, nf))
+    assert err < 1e-2
