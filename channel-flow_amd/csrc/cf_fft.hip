@@ -72,6 +72,17 @@ __device__ __forceinline__ void dft3(Cx<T> *a, int sign) {
     a[2] = csub(m, r);
 }
 
+// LDS lines are padded with one element after every 2^5 so that the
+// strided writes of the early Stockham passes (stride R) and the transposed
+// accesses spread over the LDS banks; fpad maps a line position to its slot.
+// (FP64 complex only: with 8-byte FP32 complex elements the unpadded
+// layout measured faster)
+template <typename T>
+__host__ __device__ __forceinline__ int fpad(int e) { return sizeof(T) == 8 ? e + (e >> 5) : e; }
+// slots of a padded line of n elements
+template <typename T>
+__host__ __device__ __forceinline__ int fft_line(int n) { return fpad<T>(n - 1) + 1; }
+
 // a = q*b + r for 0 <= a < 2^20 and 1 <= b <= 2^12 with a precomputed 1/b:
 // the float quotient is off by at most one, which the remainder test fixes
 __device__ __forceinline__ void small_divmod(int a, int b, float inv_b, int &q, int &r) {
@@ -102,7 +113,7 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
             const Cx<T> *src = buf + line * ls;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                Cx<T> a = src[jj + r * nb];
+                Cx<T> a = src[fpad<T>(jj + r * nb)];
                 if (r) {
                     Cx<T> w = tw[r * k * stw];
                     if (sign > 0) w.y = -w.y;
@@ -127,7 +138,7 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
             const int base = q * Ns * R + k;
             Cx<T> *dst = buf + line * ls;
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[base + r * Ns] = v[c][r];
+            for (int r = 0; r < R; r++) dst[fpad<T>(base + r * Ns)] = v[c][r];
         }
     }
     __syncthreads();
@@ -160,7 +171,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const Cx<T> *__restrict__ twM, const Cx<T> *__restrict__ twN, int rb) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int M = pM.n, N = 2 * M, ls = M;
+    const int M = pM.n, N = 2 * M, ls = fft_line<T>(M);
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
@@ -177,7 +188,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
                  ((T)V[i3(nxg, ny, i, j, k)] - (T)V[i3(nxg, ny, i, j - 1, k)]) * (T)m.rdyf[j] +
                  ((T)W[i3(nxg, nyg, i, j, k)] - (T)W[i3(nxg, nyg, i, j, k - 1)]) * (T)m.rdzf[k];
         }
-        ldr[lr * 2 * ls + ip] = dv;
+        ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
     __syncthreads();
     lds_fft<256>(lds, ls, rb, pM, tw, -1);
@@ -188,8 +199,8 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         const int r = row0 + lr;
         if (r >= nrows) continue;
         const Cx<T> *z = lds + lr * ls;
-        const Cx<T> a = z[kx % M];
-        const Cx<T> bc = z[(M - kx) % M];
+        const Cx<T> a = z[fpad<T>(kx % M)];
+        const Cx<T> bc = z[fpad<T>((M - kx) % M)];
         const Cx<T> b = {bc.x, -bc.y};  // conj(Z[M-k])
         const Cx<T> E = {(T)0.5 * (a.x + b.x), (T)0.5 * (a.y + b.y)};
         const Cx<T> dd = csub(a, b);
@@ -211,7 +222,7 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
        int sign) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int n = pZ.n, ls = n + 1;
+    const int n = pZ.n, ls = fft_line<T>(n) + 1;
     const int64_t ncols = (int64_t)d.nyp * d.mh;
     const int64_t c0 = (int64_t)blockIdx.x * bcols;
     const int nb = (int)min((int64_t)bcols, ncols - c0);
@@ -220,13 +231,13 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
     for (int e = threadIdx.x; e < n; e += 512) tw[e] = twZ[e];
     for (int e = threadIdx.x; e < nb * n; e += 512) {
         const int b = e / n, kz = e - b * n;
-        lds[b * ls + kz] = base[e];
+        lds[b * ls + fpad<T>(kz)] = base[e];
     }
     __syncthreads();
     lds_fft<512>(lds, ls, nb, pZ, tw, sign);
     for (int e = threadIdx.x; e < nb * n; e += 512) {
         const int b = e / n, kz = e - b * n;
-        base[e] = lds[b * ls + kz];
+        base[e] = lds[b * ls + fpad<T>(kz)];
     }
 }
 
@@ -427,7 +438,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
               const Cx<T> *__restrict__ twN, int rb, double norm) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int M = pM.n, N = 2 * M, mh = M + 1, ls = mh;
+    const int M = pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
@@ -441,7 +452,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
             const int jp = r / d.nzp, kp = r - jp * d.nzp;
             v = spec[((int64_t)jp * mh + kx) * d.nzp + kp];
         }
-        lds[lr * ls + kx] = v;
+        lds[lr * ls + fpad<T>(kx)] = v;
     }
     __syncthreads();
     // packed length-M input of the complex inverse, in place: each thread
@@ -452,7 +463,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     for (int e = threadIdx.x; e < rb * hp; e += 256) {
         const int lr = e / hp, k = e - lr * hp, k2 = M - k;
         Cx<T> *row = lds + lr * ls;
-        const Cx<T> A = row[k], B = row[k2];
+        const Cx<T> A = row[fpad<T>(k)], B = row[fpad<T>(k2)];
         auto z = [&](Cx<T> a, Cx<T> bc, int kk) {
             const Cx<T> b = {bc.x, -bc.y};
             const Cx<T> s = cadd(a, b), df = csub(a, b);
@@ -461,8 +472,8 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
             const Cx<T> t = cmul(df, wc);
             return Cx<T>{s.x - t.y, s.y + t.x};  // s + i t
         };
-        row[k] = z(A, B, k);
-        if (k != 0 && k2 != k) row[k2] = z(B, A, k2);
+        row[fpad<T>(k)] = z(A, B, k);
+        if (k != 0 && k2 != k) row[fpad<T>(k2)] = z(B, A, k2);
     }
     __syncthreads();
     lds_fft<256>(lds, ls, rb, pM, tw, +1);
@@ -473,10 +484,10 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         const int r = row0 + lr;
         if (r >= nrows) continue;
         const int jp = r / d.nzp, kp = r - jp * d.nzp;
-        const double ph = (double)zr[lr * 2 * ls + ip] / norm;
+        const double ph = (double)zr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] / norm;
         phi[((int64_t)kp * d.nyp + jp) * d.nxp + ip] = (P)ph;
         const int ipn = ip + 1 == N ? 0 : ip + 1;
-        const double phn = (double)zr[lr * 2 * ls + ipn] / norm;
+        const double phn = (double)zr[2 * (lr * ls + fpad<T>(ipn >> 1)) + (ipn & 1)] / norm;
         const int i = 2 + ip, j = 2 + jp, k = 2 + kp;
         const int64_t qu = i3(nx, nyg, i, j, k);
         U[qu] = (S)((double)U[qu] - (phn - ph) * m.rdxc[i]);
@@ -494,6 +505,9 @@ static int env_int(const char *name, int dflt) {
     return v ? atoi(v) : dflt;
 }
 
+// host twin of fft_line for a complex element of esize bytes
+static int line_slots(int n, int esize) { return esize == 16 ? fft_line<double>(n) : fft_line<float>(n); }
+
 int fft_rows_per_wg(int M, int elems_per_row, int esize) {
     int rb = 2048 / M;
     if (rb > 16) rb = 16;
@@ -502,7 +516,7 @@ int fft_rows_per_wg(int M, int elems_per_row, int esize) {
     const bool inv = elems_per_row > M;
     const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", inv ? 16 : 8);
     if (cap > 0 && rb > cap) rb = cap;
-    const int lim = (65536 / esize - M) / elems_per_row;  // + M twiddles
+    const int lim = (65536 / esize - M) / line_slots(elems_per_row, esize);  // + M twiddles
     if (rb > lim) rb = lim;
     if (rb < 1) rb = 1;
     return rb;
@@ -514,7 +528,7 @@ int fft_cols_per_wg(int n, int esize) {
     if (b > 16) b = 16;
     const int cap = env_int("CF_FFT_COLS", 0);
     if (cap > 0 && b > cap) b = cap;
-    const int lim = (65536 / esize - n) / (n + 1);  // + n twiddles
+    const int lim = (65536 / esize - n) / (line_slots(n, esize) + 1);  // + n twiddles
     if (b > lim) b = lim;
     if (b < 1) b = 1;
     return b;
@@ -537,7 +551,7 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
                      hipStream_t st) {
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = ((size_t)rb + 1) * pM.n * sizeof(Cx<T>);
+    const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_div_fftx<S, T>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m, U, V,
                        W, spec, pM, twM, twN, rb);
 }
@@ -547,7 +561,7 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
                  hipStream_t st) {
     const int b = fft_cols_per_wg(pZ.n, (int)sizeof(Cx<T>));
     const int64_t ncols = (int64_t)d.nyp * d.mh;
-    const size_t shm = ((size_t)b * (pZ.n + 1) + pZ.n) * sizeof(Cx<T>);
+    const size_t shm = ((size_t)b * (fft_line<T>(pZ.n) + 1) + pZ.n) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_fftz<T>), dim3((unsigned)((ncols + b - 1) / b)), dim3(512), shm, st, d, spec, pZ, twZ,
                        b, sign);
 }
@@ -580,7 +594,7 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
                         hipStream_t st) {
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = ((size_t)rb * (pM.n + 1) + pM.n) * sizeof(Cx<T>);
+    const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>);
     hipLaunchKernelGGL((k_ifftx_proju<S, T, P>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m,
                        spec, phi, U, pM, twM, twN, rb, norm);
 }
