@@ -270,7 +270,7 @@ int Ctx<S>::init() {
         A(&m_delta, h.delta.size()) || A(&m_jjz, h.jjz.size()) || A(&m_wz, h.wz.size()) ||
         A(&m_jjx, h.jjx.size()) || A(&m_wx, h.wx.size()))
         return -1;
-    if (A(&twM, (size_t)M + 1) || A(&twN, (size_t)d.nxp + 1) || A(&twZ, (size_t)d.nzp + 1)) return -1;
+    if (A(&twM, 2 * (size_t)M + 2) || A(&twN, (size_t)d.nxp + 1) || A(&twZ, 2 * (size_t)d.nzp + 2)) return -1;
     if (A(&alpha_x, nwx) || A(&alpha_z, nwz) || A(&azt, nwx) || A(&tau_init, nwx)) return -1;
     if (dry) return 0;
 
@@ -309,7 +309,28 @@ int Ctx<S>::init() {
         }
         return t;
     };
-    if (upload(twM, twid(M)) || upload(twN, twid(d.nxp)) || upload(twZ, twid(d.nzp))) return -1;
+    // twiddles of the LDS transforms: the plain table exp(-2 pi i q/n),
+    // q = 0..n, then per-pass tables: for the pass with current length Ns and
+    // radix R, exp(-2 pi i r k / (Ns R)) at Ns-1 + k (R-1) + r-1 (k < Ns,
+    // 1 <= r < R); n-1 entries in all
+    auto pass_tw = [&](const FFTPlan &p) {
+        std::vector<Cx<S>> t = twid(p.n);  // plain table, then the pass table
+        t.resize(2 * (size_t)p.n + 2, Cx<S>{(S)1, (S)0});
+        const int o = p.n + 1;
+        int Ns = 1;
+        for (int s = 0; s < p.nr; s++) {
+            const int R = p.radix[s];
+            for (int k = 0; k < Ns; k++)
+                for (int r = 1; r < R; r++) {
+                    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)(r * k) /
+                                    (long double)(Ns * R);
+                    t[o + Ns - 1 + k * (R - 1) + (r - 1)] = Cx<S>{(S)std::cos(a), (S)(-std::sin(a))};
+                }
+            Ns *= R;
+        }
+        return t;
+    };
+    if (upload(twM, pass_tw(pM)) || upload(twN, twid(d.nxp)) || upload(twZ, pass_tw(pZ))) return -1;
 
 
     // wall model buffers

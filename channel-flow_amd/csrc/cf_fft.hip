@@ -83,6 +83,12 @@ __host__ __device__ __forceinline__ int fpad(int e) { return sizeof(T) == 8 ? e 
 template <typename T>
 __host__ __device__ __forceinline__ int fft_line(int n) { return fpad<T>(n - 1) + 1; }
 
+// Twiddle buffer of a length-n transform: the plain table exp(-2 pi i q/n),
+// q = 0..n, followed by the per-pass table (n-1 entries); FP64 transforms
+// use the second, FP32 ones the first.
+template <typename T>
+__device__ __forceinline__ int tw_base(int n) { return sizeof(T) == 8 ? n + 1 : 0; }
+
 // a = q*b + r for 0 <= a < 2^20 and 1 <= b <= 2^12 with a precomputed 1/b:
 // the float quotient is off by at most one, which the remainder test fixes
 __device__ __forceinline__ void small_divmod(int a, int b, float inv_b, int &q, int &r) {
@@ -115,7 +121,10 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
             for (int r = 0; r < R; r++) {
                 Cx<T> a = src[fpad<T>(jj + r * nb)];
                 if (r) {
-                    Cx<T> w = tw[r * k * stw];
+                    // FP64: pass table, exp(-2 pi i r k / (Ns R)) at Ns-1 + k(R-1) + r-1
+                    // (consecutive in k, fewer LDS bank conflicts); FP32: the plain
+                    // table exp(-2 pi i q / n) at q = r k n/(Ns R) measured faster
+                    Cx<T> w = sizeof(T) == 8 ? tw[Ns - 1 + k * (R - 1) + (r - 1)] : tw[r * k * stw];
                     if (sign > 0) w.y = -w.y;
                     a = cmul(a, w);
                 }
@@ -177,7 +186,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
     T *ldr = reinterpret_cast<T *>(lds);
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
-    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[e];
+    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[tw_base<T>(M) + e];
     for (int e = threadIdx.x; e < rb * N; e += 256) {
         const int lr = e / N, ip = e - lr * N;
         const int r = row0 + lr;
@@ -228,7 +237,7 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
     const int nb = (int)min((int64_t)bcols, ncols - c0);
     Cx<T> *base = spec + c0 * n;
     Cx<T> *tw = lds + bcols * ls;
-    for (int e = threadIdx.x; e < n; e += 512) tw[e] = twZ[e];
+    for (int e = threadIdx.x; e < n; e += 512) tw[e] = twZ[tw_base<T>(n) + e];
     for (int e = threadIdx.x; e < nb * n; e += 512) {
         const int b = e / n, kz = e - b * n;
         lds[b * ls + fpad<T>(kz)] = base[e];
@@ -442,7 +451,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
-    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[e];
+    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[tw_base<T>(M) + e];
     // consecutive threads read consecutive rows (kz) of one kx: runs of rb
     for (int e = threadIdx.x; e < rb * mh; e += 256) {
         const int kx = e / rb, lr = e - kx * rb;
