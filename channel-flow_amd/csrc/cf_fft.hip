@@ -173,8 +173,8 @@ __device__ void lds_fft(Cx<T> *buf, int ls, int nlines, const FFTPlan &p, const 
 // the x part of the 2-D DFT (projection.f90:84-86), as a length-M complex FFT
 // of z[m] = div[2m] + i div[2m+1] (M = nxp/2) and the standard split.
 // ---------------------------------------------------------------------------
-template <typename S, typename T>
-__global__ void __launch_bounds__(256)
+template <typename S, typename T, int WG>
+__global__ void __launch_bounds__(WG)
 k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const S *__restrict__ W, Cx<T> *__restrict__ spec, FFTPlan pM,
            const Cx<T> *__restrict__ twM, const Cx<T> *__restrict__ twN, int rb) {
@@ -186,8 +186,8 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
     T *ldr = reinterpret_cast<T *>(lds);
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
-    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[tw_base<T>(M) + e];
-    for (int e = threadIdx.x; e < rb * N; e += 256) {
+    for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
+    for (int e = threadIdx.x; e < rb * N; e += WG) {
         const int lr = e / N, ip = e - lr * N;
         const int r = row0 + lr;
         T dv = (T)0;
@@ -200,10 +200,10 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
     __syncthreads();
-    lds_fft<256>(lds, ls, rb, pM, tw, -1);
+    lds_fft<WG>(lds, ls, rb, pM, tw, -1);
     const int mh = M + 1;
     // consecutive threads take consecutive rows (kz) of one kx: runs of rb
-    for (int e = threadIdx.x; e < rb * mh; e += 256) {
+    for (int e = threadIdx.x; e < rb * mh; e += WG) {
         const int kx = e / rb, lr = e - kx * rb;
         const int r = row0 + lr;
         if (r >= nrows) continue;
@@ -440,8 +440,8 @@ k_thomas(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ kxx, const
 // nxp*nzp, real part), writes phi and projects U on the same row
 // (projection.f90:208-211), phi(nx) being the periodic copy of phi(2).
 // ---------------------------------------------------------------------------
-template <typename S, typename T, typename P>
-__global__ void __launch_bounds__(256)
+template <typename S, typename T, typename P, int WG>
+__global__ void __launch_bounds__(WG)
 k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__ phi,
               S *__restrict__ U, FFTPlan pM, const Cx<T> *__restrict__ twM,
               const Cx<T> *__restrict__ twN, int rb, double norm) {
@@ -451,9 +451,9 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
-    for (int e = threadIdx.x; e < M; e += 256) tw[e] = twM[tw_base<T>(M) + e];
+    for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
     // consecutive threads read consecutive rows (kz) of one kx: runs of rb
-    for (int e = threadIdx.x; e < rb * mh; e += 256) {
+    for (int e = threadIdx.x; e < rb * mh; e += WG) {
         const int kx = e / rb, lr = e - kx * rb;
         const int r = row0 + lr;
         Cx<T> v = {(T)0, (T)0};
@@ -469,7 +469,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     // from the two spectrum values it read.
     //   Z(k) = (X(k) + conj X(M-k)) + i (X(k) - conj X(M-k)) exp(+2 pi i k/N)
     const int hp = M / 2 + 1;
-    for (int e = threadIdx.x; e < rb * hp; e += 256) {
+    for (int e = threadIdx.x; e < rb * hp; e += WG) {
         const int lr = e / hp, k = e - lr * hp, k2 = M - k;
         Cx<T> *row = lds + lr * ls;
         const Cx<T> A = row[fpad<T>(k)], B = row[fpad<T>(k2)];
@@ -485,10 +485,10 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         if (k != 0 && k2 != k) row[fpad<T>(k2)] = z(B, A, k2);
     }
     __syncthreads();
-    lds_fft<256>(lds, ls, rb, pM, tw, +1);
+    lds_fft<WG>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;
-    for (int e = threadIdx.x; e < rb * N; e += 256) {
+    for (int e = threadIdx.x; e < rb * N; e += WG) {
         const int lr = e / N, ip = e - lr * N;
         const int r = row0 + lr;
         if (r >= nrows) continue;
@@ -514,11 +514,18 @@ static int env_int(const char *name, int dflt) {
     return v ? atoi(v) : dflt;
 }
 
+// threads per workgroup of the x transforms (env CF_FFT_XWG: 256 or 512)
+static int fft_x_wg(int esize) {
+    const int v = env_int("CF_FFT_XWG", 256);
+    (void)esize;
+    return v == 512 ? 512 : 256;
+}
+
 // host twin of fft_line for a complex element of esize bytes
 static int line_slots(int n, int esize) { return esize == 16 ? fft_line<double>(n) : fft_line<float>(n); }
 
 int fft_rows_per_wg(int M, int elems_per_row, int esize) {
-    int rb = 2048 / M;
+    int rb = 8 * fft_x_wg(esize) / M;
     if (rb > 16) rb = 16;
     // tuning knob: cap on rows per workgroup (more workgroups per CU)
     // (8 rows = runs of 8 consecutive kz in the spectrum, 128 bytes in FP64)
@@ -561,8 +568,12 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>);
-    hipLaunchKernelGGL((k_div_fftx<S, T>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m, U, V,
-                       W, spec, pM, twM, twN, rb);
+    if (fft_x_wg((int)sizeof(Cx<T>)) == 512)
+        hipLaunchKernelGGL((k_div_fftx<S, T, 512>), dim3((nrows + rb - 1) / rb), dim3(512), shm, st, d, m, U, V,
+                           W, spec, pM, twM, twN, rb);
+    else
+        hipLaunchKernelGGL((k_div_fftx<S, T, 256>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m, U, V,
+                           W, spec, pM, twM, twN, rb);
 }
 
 template <typename T>
@@ -604,8 +615,12 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>);
-    hipLaunchKernelGGL((k_ifftx_proju<S, T, P>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m,
-                       spec, phi, U, pM, twM, twN, rb, norm);
+    if (fft_x_wg((int)sizeof(Cx<T>)) == 512)
+        hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512>), dim3((nrows + rb - 1) / rb), dim3(512), shm, st, d, m,
+                           spec, phi, U, pM, twM, twN, rb, norm);
+    else
+        hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m,
+                           spec, phi, U, pM, twM, twN, rb, norm);
 }
 
 template void launch_div_fftx<double, double>(const Dims &, const Metrics &, const double *,
