@@ -165,6 +165,27 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 5}))
 
 
+def measured_copy_gbps(mib=1024, reps=10):
+    """Read+write rate of a plain 1 GiB device copy on this GPU (torch's copy,
+    HIP events on torch's stream): the practical ceiling a streaming kernel
+    that reads and writes is compared with, beside the spec peak."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * n * 8 / (ms / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,7 +201,11 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB five times (counter calibration)")
+    ap.add_argument("--copy-ceiling", action="store_true", help="only measure a 1 GiB device copy")
     a = ap.parse_args()
+    if a.copy_ceiling:
+        print(json.dumps({"copy_GBps": measured_copy_gbps()}))
+        return
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
@@ -221,10 +246,23 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "algorithmic_bytes": ab, "traffic": measured_traffic(dom, cells)},
+        "copy_ceiling": None,
         "kernels_fp64": kernels,
         "ms_per_step_event_timed": round(ms64_ev, 4),
         "max_div_fp64": div64,
     }
+    # in a child process: torch brings its own HIP runtime, which must not
+    # share a process with the one the library is linked against
+    try:
+        import subprocess
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-ceiling"], capture_output=True,
+                           text=True, timeout=300)
+        cg = json.loads(r.stdout.strip().splitlines()[-1])["copy_GBps"]
+        out["copy_ceiling"] = {"GBps": round(cg, 1), "frac_of_peak": round(cg / HBM_PEAK_GBS, 4),
+                               "dominant_kernel_frac_of_copy": round(achieved / cg, 4) if achieved else None,
+                               "what": "1 GiB device copy, read+write bytes / HIP-event time"}
+    except Exception as e:
+        log("copy ceiling not measured: %s" % e)
     if not a.no_mixed:
         U64, V64, W64 = g64.get_state()
         s64 = g64.compute_statistics()

@@ -553,13 +553,13 @@ int fft_cols_per_wg(int n, int esize) {
 // y-solve geometry: MS rows per thread segment, P = ceil(nyp/MS) segments,
 // KZ systems per workgroup (a power of two, KZ * P <= 512)
 // (8 rows: 16 would keep 80 doubles per thread and spill)
-static int thomas_ms() { return 8; }
+static int thomas_ms() { return env_int("CF_TH_MS", 8) == 4 ? 4 : 8; }
 static void thomas_geometry(const Dims &d, int ms, int &KZ, int &P) {
     P = (d.nyp + ms - 1) / ms;
     KZ = 1;
     while (KZ * 2 * P <= 512 && KZ * 2 <= 64) KZ *= 2;
 }
-bool thomas_supported(const Dims &d) { return (d.nyp + 7) / 8 <= 512; }
+bool thomas_supported(const Dims &d) { return (d.nyp + 3) / 4 <= 512; }
 
 template <typename S, typename T>
 void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
@@ -603,9 +603,12 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
     const int nkzb = (d.nzp + KZ - 1) / KZ;
     const int threads = KZ * P;
     const size_t shm = (size_t)threads * 8 * sizeof(double);
-    (void)ms;
-    hipLaunchKernelGGL((k_thomas<T, 8>), dim3(d.mh * nkzb), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
-                       du, KZ, P);
+    if (ms == 4)
+        hipLaunchKernelGGL((k_thomas<T, 4>), dim3(d.mh * nkzb), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
+                           du, KZ, P);
+    else
+        hipLaunchKernelGGL((k_thomas<T, 8>), dim3(d.mh * nkzb), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
+                           du, KZ, P);
 }
 
 template <typename S, typename T, typename P>
