@@ -122,6 +122,34 @@ def cpu_baseline(cells, steps_cap_s=30.0):
             "sample": "%d RK3 step(s) of the %dx%dx%d workload, %.2f s/step" % (n, *cells, sps)}
 
 
+def other_configs(steps=50, warmup=5):
+    """Step time of the other workloads (graph-replayed, inputs resident):
+    64^3 DNS (FP64, fixed dt), 128x64x128 wall-modelled LES (Smagorinsky +
+    EQWM, mixed and FP64) and 256x96x256 wall-modelled LES at Re_tau 5200
+    (mixed, EQWM)."""
+    from channel_flow import cases
+    from channel_flow.solver import ChannelFlow
+    res = {}
+    todo = [("dns64_fp64", cases.case_c1(), "fp64"), ("wmles128_mixed", cases.case_c3(), "mixed"),
+            ("wmles128_fp64", cases.case_c3(), "fp64"), ("wmles256_mixed", cases.case_c4(), "mixed")]
+    for name, cfg, prec in todo:
+        cfg.precision = prec
+        cfg.wall_surrogate = 0
+        g = ChannelFlow(cfg, device=0)
+        g.initialise(*cases.initial_field(cfg))
+        g.run(warmup, graph=True)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(steps, graph=True)
+        g.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        n = cfg.cells[0] * cfg.cells[1] * cfg.cells[2]
+        res[name] = {"cells": list(cfg.cells), "ms_per_step": round(ms, 4), "mcell_per_s": round(n / ms / 1e3, 1)}
+        log("%s: %.3f ms/step" % (name, ms))
+        g.close()
+    return res
+
+
 def serving_bench(scale, max_active, budget_gb, n_cases=64, seed=5):
     """Ensemble of channel cases (seeded Poisson arrivals; grids 64^3,
     96x64x96, 128x96x128, 192x128x192; Re_tau ~ U[180, 2000]; DNS/WMLES
@@ -195,6 +223,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-others", action="store_true", help="skip the 64^3 and LES step timings")
     ap.add_argument("--serve-scale", type=float, default=0.1,
                     help="steps of the serving ensemble relative to 200-2000 per case")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -285,6 +314,8 @@ def main():
         g64.close()
     if not a.no_serve:
         out["serving"] = serving_bench(a.serve_scale, a.serve_slots, a.serve_budget_gb)
+    if not a.no_others:
+        out["other_configs"] = other_configs()
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cells)

@@ -255,7 +255,7 @@ int Ctx<S>::init() {
             if (A(&st[b][c], n3[c])) return -1;
     for (int c = 0; c < 3; c++)
         if (A(&G[c], n3[c])) return -1;
-    if (thomas_uses_scratch((int)sizeof(Cx<S>)) && A(&cscr, nspec)) return -1;
+    if (thomas_uses_scratch((int)sizeof(Cx<S>), d.mh * d.nzp) && A(&cscr, nspec)) return -1;
     if (A(&NT, nC) || A(&spec, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
         A(&part, npart) || A(&dtscr, dt_scratch_doubles(d)) || A(&sc, 1) ||
         (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, 7 * d.ny + 8) || A(&stats_acc, 7 * d.ny + 8))
@@ -438,8 +438,8 @@ void Ctx<S>::eddy_viscosity(S *const *F) {
     const Dims &d = h.d;
     if (h.prm.les_model != 1) return;  // nu_t stays 0 (subgrid.f90:41-45)
     tic("smagorinsky");
-    launch_smagorinsky<S>(d, m_x, m_y, m_z, m_xg, m_yg, m_zg, m_delta, F[0], F[1], F[2], NT, sc, ubar,
-                          h.prm.cs, h.prm.nu, h.prm.dirichlet_nu_t, stream);
+    launch_smagorinsky<S>(d, met, m_y, m_yg, m_delta, F[0], F[1], F[2], NT, sc, ubar, h.prm.cs, h.prm.nu,
+                          h.prm.dirichlet_nu_t, stream);
     toc();
     tic("ghosts_nut");
     launch_ghosts<S>(d, NT, 3, h.prm.dirichlet_nu_t == 1 ? 0 : 2, nullptr, 0, 0, stream);

@@ -1,7 +1,8 @@
 // Equilibrium wall model (eqwm.f90:20-223, wallmodel.f90:591-685) on the GPU.
 // One lane per wall point: each lane runs the Picard iteration of the 1-D
-// equilibrium ODE on its own stretched grid (FP64, transcendental-bound),
-// so lanes with different iteration counts simply retire at different times.
+// equilibrium ODE on the block's shared stretched grid (FP64, bound by the
+// exponentials and divisions), so lanes with different iteration counts
+// simply retire at different times.
 #include "cf_kernels.hpp"
 #include "cf_eqwm.hpp"
 
@@ -10,8 +11,10 @@ namespace cf {
 // Normalised wall-model grid of eqwm.f90:20-50: g_fa = y_fa/ymax, g_cv =
 // y_cv/ymax (the reference scales as y/ymax*h, so y = g*h bit for bit).  The
 // maximum is taken over the ncv assigned centroids (the reference's Maxval
-// also reads one unassigned slot, y_cv(ncv+1)).  Filled cooperatively into LDS.
-__device__ void eqwm_grid(int ncv, double stretch, double *g_fa, double *g_cv) {
+// also reads one unassigned slot, y_cv(ncv+1)).  Also the reciprocal
+// spacings r_cv[q] = 1/(g_cv[q+1]-g_cv[q]) and r_w = 1/(g_cv[0]-g_fa[0]).
+// Filled by one thread into LDS.
+__device__ void eqwm_grid(int ncv, double stretch, double *g_fa, double *g_cv, double *r_cv) {
     if (threadIdx.x == 0) {
         const int nfa = ncv + 1;
         g_fa[0] = 0.0;
@@ -25,95 +28,88 @@ __device__ void eqwm_grid(int ncv, double stretch, double *g_fa, double *g_cv) {
         for (int q = 1; q < ncv; q++) ymax = g_cv[q] > ymax ? g_cv[q] : ymax;
         for (int q = 0; q < nfa; q++) g_fa[q] = g_fa[q] / ymax;
         for (int q = 0; q < ncv; q++) g_cv[q] = g_cv[q] / ymax;
+        for (int q = 0; q + 1 < ncv; q++) r_cv[q] = 1.0 / (g_cv[q + 1] - g_cv[q]);
+        r_cv[ncv - 1] = 1.0 / (g_cv[0] - g_fa[0]);
     }
     __syncthreads();
 }
 
 // solve_u_eqwm (eqwm.f90:90-223): Picard iteration on the mixing-length
-// equilibrium ODE.  The tridiagonal system of each iteration is assembled
-// row by row inside the Thomas forward sweep; c' and d' live in LDS with a
-// stride of one wavefront (cp[q*64 + lane]).
+// equilibrium ODE, with no pressure-gradient term (the right-hand side is 0
+// except the top row, u = u_les).  The forward sweep of the tridiagonal
+// solve therefore keeps d' = 0 on every row below the top, and the back
+// substitution reduces to u(1) = u_les * prod(-c'(q)): each lane carries the
+// running product through the sweep and needs no per-row storage.  Spacing
+// divisions use the reciprocals of the normalised grid scaled by 1/h.
+// (The oracle keeps the reference's stored-array form; the two agree to the
+// Picard tolerance.)
 __device__ double eqwm_solve(double umag_les, double mu_lam, double rho_eq, double h_wm, int ncv,
                              double tol, int max_iter, double tau_in, const double *g_fa,
-                             const double *g_cv, double *cp, double *dp, int *iter_out) {
+                             const double *r_cv, int *iter_out) {
     const double kappa = 0.41, Aplus = 17.0;
-    const int L = blockDim.x, lane = threadIdx.x;
     const double tau_wall_lam = mu_lam * umag_les / h_wm;
     const double nu_eq = mu_lam / rho_eq;
-#define YFA(ifa) (g_fa[(ifa)-1] * h_wm)
-#define YCV(icv) (g_cv[(icv)-1] * h_wm)
+    const double rh = 1.0 / h_wm;
+    const double rw = r_cv[ncv - 1] * rh;  // 1/(y_cv(1) - y_fa(1))
     int iter = 0;
     bool done = false;
     double tau_wall = tau_in;
     while (!done) {
         const double tau_prev = tau_wall;
         const double utau_eq = sqrt(tau_wall / rho_eq);
-        // total viscosity at face ifa (eqwm.f90:136-147)
+        const double ex = -utau_eq / nu_eq / Aplus;
+        const double ck = rho_eq * kappa * utau_eq;
+        // total viscosity at face ifa >= 2 (eqwm.f90:136-147)
         auto mu_face = [&](int ifa) -> double {
-            if (ifa == 1) return mu_lam * 1.0;
-            const double yf = YFA(ifa);
-            double D = 1.0 - exp(-1.0 * yf * utau_eq / nu_eq / Aplus);
+            const double yf = g_fa[ifa - 1] * h_wm;
+            double D = 1.0 - exp(yf * ex);
             D = D * D;
-            const double mut = rho_eq * kappa * yf * utau_eq * D;
-            return mu_lam * 1.0 + mut;
+            return mu_lam + ck * yf * D;
         };
         // row 1, the wall CV (eqwm.f90:167-171)
-        const double mu1 = mu_face(1);
         double mu_lo = mu_face(2);
-        const double sup = mu_lo / (YCV(2) - YCV(1));
-        const double bb = -1.0 * (sup + mu1 / (YCV(1) - YFA(1)));
-        double cpp = sup / bb, dpp = 0.0 / bb;
-        cp[lane] = cpp;
-        dp[lane] = dpp;
+        const double sup = mu_lo * (r_cv[0] * rh);
+        double cpp = sup / (-(sup + mu_lam * rw));
+        double prod = -cpp;
         // interior rows (eqwm.f90:156-164)
         for (int icv = 2; icv <= ncv - 1; icv++) {
             const double mu_hi = mu_face(icv + 1);
-            const double s_up = mu_hi / (YCV(icv + 1) - YCV(icv));
-            const double s_dn = mu_lo / (YCV(icv) - YCV(icv - 1));
-            const double b_ = -1.0 * (s_up + s_dn);
-            const double m = b_ - cpp * s_dn;
+            const double s_up = mu_hi * (r_cv[icv - 1] * rh);
+            const double s_dn = mu_lo * (r_cv[icv - 2] * rh);
+            const double m = -(s_up + s_dn) - cpp * s_dn;
             cpp = s_up / m;
-            dpp = (0.0 - dpp * s_dn) / m;
-            cp[(icv - 1) * L + lane] = cpp;
-            dp[(icv - 1) * L + lane] = dpp;
+            prod *= -cpp;
             mu_lo = mu_hi;
         }
-        // top row: A(ncv,ncv) = 1, A(ncv,ncv-1) = 0, b = u_les
-        const double mt = 1.0 - cpp * 0.0;
-        double u = (umag_les - dpp * 0.0) / mt;
-        for (int q = ncv - 2; q >= 0; q--) u = dp[q * L + lane] - cp[q * L + lane] * u;
-        tau_wall = mu1 * (u - 0.0) / (YCV(1) - YFA(1));
+        // top row u(ncv) = u_les, back substitution u(q) = -c'(q) u(q+1)
+        const double u1 = umag_les * prod;
+        tau_wall = mu_lam * u1 * rw;
         if (fabs((tau_wall - tau_prev) / tau_wall_lam) < tol) done = true;
         if (!done) iter = iter + 1;
         if (iter > max_iter) done = true;
     }
-#undef YFA
-#undef YCV
     *iter_out = iter;
     return tau_wall;
 }
 
 size_t eqwm_lds_bytes(int ncv, int wg) {
-    return sizeof(double) * ((size_t)(ncv + 1) + (size_t)ncv + 2 * (size_t)ncv * wg);
+    (void)wg;
+    return sizeof(double) * ((size_t)(ncv + 1) + 2 * (size_t)ncv);
 }
-// lanes per block: a full wavefront when the per-lane c', d' arrays fit in
-// 64 KiB of LDS, else the largest power of two that does
 int eqwm_block(int ncv) {
-    int wg = EQWM_WG;
-    while (wg > 1 && eqwm_lds_bytes(ncv, wg) > 65536) wg >>= 1;
-    return wg;
+    (void)ncv;
+    return EQWM_WG;
 }
 
 struct EqwmLds {
-    double *g_fa, *g_cv, *cp, *dp;
+    double *g_fa, *g_cv, *r_cv;
 };
 __device__ EqwmLds eqwm_lds(int ncv) {
     extern __shared__ double eq_smem[];
     EqwmLds s;
     s.g_fa = eq_smem;
     s.g_cv = s.g_fa + (ncv + 1);
-    s.cp = s.g_cv + ncv;
-    s.dp = s.cp + (size_t)ncv * blockDim.x;
+    s.r_cv = s.g_cv + ncv;
     return s;
 }
 
@@ -122,12 +118,11 @@ k_eqwm_batch(int n, const double *__restrict__ u, const double *__restrict__ nu,
              const double *__restrict__ h, const double *__restrict__ tau_in,
              double *__restrict__ tau, int *__restrict__ iter, EqwmParams p) {
     EqwmLds s = eqwm_lds(p.ncv);
-    eqwm_grid(p.ncv, p.stretch, s.g_fa, s.g_cv);
+    eqwm_grid(p.ncv, p.stretch, s.g_fa, s.g_cv, s.r_cv);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     int it;
-    tau[q] = eqwm_solve(u[q], nu[q], p.rho, h[q], p.ncv, p.tol, p.max_iter, tau_in[q], s.g_fa, s.g_cv,
-                        s.cp, s.dp, &it);
+    tau[q] = eqwm_solve(u[q], nu[q], p.rho, h[q], p.ncv, p.tol, p.max_iter, tau_in[q], s.g_fa, s.r_cv, &it);
     iter[q] = it;
 }
 
@@ -156,7 +151,7 @@ __global__ void __launch_bounds__(EQWM_WG) k_wall_model(Dims d, const S *__restr
                              const int *__restrict__ use_pred, const double *__restrict__ ubar) {
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
     EqwmLds s = eqwm_lds(p.ncv);
-    eqwm_grid(p.ncv, p.stretch, s.g_fa, s.g_cv);
+    eqwm_grid(p.ncv, p.stretch, s.g_fa, s.g_cv, s.r_cv);
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nx * nzg * 2) return;
     const int i = 1 + q % nx;
@@ -186,8 +181,7 @@ __global__ void __launch_bounds__(EQWM_WG) k_wall_model(Dims d, const S *__restr
         tau = tau_pred[qa];
         solved = false;
     } else {
-        tau = eqwm_solve(ul, nu, 1.0, h, p.ncv, p.tol, p.max_iter, tau_init[qa], s.g_fa, s.g_cv, s.cp,
-                         s.dp, &it);
+        tau = eqwm_solve(ul, nu, 1.0, h, p.ncv, p.tol, p.max_iter, tau_init[qa], s.g_fa, s.r_cv, &it);
     }
     if (solved) {
         if (isnan(tau)) { atomicAdd(&sc->n_nan_eqwm, 1); tau = tau_mean; }

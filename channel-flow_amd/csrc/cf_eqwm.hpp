@@ -6,7 +6,7 @@
 namespace cf {
 
 constexpr int EQWM_MAXNCV = 128;
-constexpr int EQWM_WG = 64;  // one wavefront per block; per-lane arrays in LDS
+constexpr int EQWM_WG = 128;  // lanes per block (one wall point each; only the grid is in LDS)
 
 // 3-point Lagrange stencils of interpolate_z_2nd / interpolate_x_2nd
 // (interpolation.f90:114-252) for the wall planes: 1-based centre index and

@@ -586,13 +586,16 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
                        b, sign);
 }
 
-bool thomas_uses_scratch(int esize) { return env_int("CF_TH_LANE", esize < 16 ? 1 : 0) != 0; }
+// lane form for FP32 spectra with enough modes to fill the GPU one lane each
+bool thomas_uses_scratch(int esize, int nmodes) {
+    return env_int("CF_TH_LANE", (esize < 16 && nmodes >= 65536) ? 1 : 0) != 0;
+}
 
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st) {
-    if (cscr && thomas_uses_scratch((int)sizeof(Cx<T>))) {
-        const int nmodes = d.mh * d.nzp;
+    const int nmodes = d.mh * d.nzp;
+    if (cscr && thomas_uses_scratch((int)sizeof(Cx<T>), nmodes)) {
         hipLaunchKernelGGL((k_thomas_lane<T>), dim3((nmodes + 255) / 256), dim3(256), 0, st, d, spec, cscr, kxx,
                            kzz, dd, dl, du);
         return;

@@ -45,10 +45,9 @@ template <typename S>
 void launch_div(const Dims &d, const double *x, const double *y, const double *z, const S *U,
                 const S *V, const S *W, double *part, hipStream_t st);
 template <typename S>
-void launch_smagorinsky(const Dims &d, const double *x, const double *y, const double *z,
-                        const double *xg, const double *yg, const double *zg, const double *delta,
-                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc,
-                        const double *ubar, double cs, double nu, int damping, hipStream_t st);
+void launch_smagorinsky(const Dims &d, const Metrics &m, const double *y, const double *yg, const double *delta,
+                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc, const double *ubar,
+                        double cs, double nu, int damping, hipStream_t st);
 template <typename S>
 void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                   const double *ubar, double *out, hipStream_t st);
@@ -68,7 +67,7 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
                  hipStream_t st);
 // the y solve form for a spectrum element of esize bytes needs an FP64
 // scratch of one double per spectrum element
-bool thomas_uses_scratch(int esize);
+bool thomas_uses_scratch(int esize, int nmodes);
 
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,

@@ -756,12 +756,12 @@ k_div_partial(Dims d, const double *__restrict__ x, const double *__restrict__ y
 // (subgrid.f90:891-949) fused: one thread per unique cell computes the three
 // diagonal strains, the 4 edge values of each off-diagonal strain and their
 // centre average, |S| = sqrt(2 SijSij), and nu_t = (Cs Delta f)^2 |S|.
+// Arithmetic in R (FP64, or FP32 in the mixed path), divisions by grid
+// spacings as multiplications with the precomputed reciprocals.
 // ---------------------------------------------------------------------------
-template <typename S>
+template <typename S, typename R>
 __global__ void __launch_bounds__(256)
-k_smagorinsky(Dims d, const double *__restrict__ x, const double *__restrict__ y,
-              const double *__restrict__ z, const double *__restrict__ xg,
-              const double *__restrict__ yg, const double *__restrict__ zg,
+k_smagorinsky(Dims d, Metrics m, const double *__restrict__ y, const double *__restrict__ yg,
               const double *__restrict__ delta, const S *__restrict__ U, const S *__restrict__ V,
               const S *__restrict__ W, S *__restrict__ NT, const Scalars *__restrict__ sc,
               const double *__restrict__ ubar, double cs, double nu, int damping, int tiles_x,
@@ -773,42 +773,52 @@ k_smagorinsky(Dims d, const double *__restrict__ x, const double *__restrict__ y
     const int k = 2 + tz;
     if (i > d.nx - 1 || j > d.nyg - 1) return;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
-#define u(I, J, K) ((double)U[i3(nx, nyg, I, J, K)] + (ubar ? ubar[J] : 0.0))
-#define v(I, J, K) ((double)V[i3(nxg, ny, I, J, K)])
-#define w(I, J, K) ((double)W[i3(nxg, nyg, I, J, K)])
-    const double s11 = (u(i, j, k) - u(i - 1, j, k)) / (x[i] - x[i - 1]);
-    const double s22 = (v(i, j, k) - v(i, j - 1, k)) / (y[j] - y[j - 1]);
-    const double s33 = (w(i, j, k) - w(i, j, k - 1)) / (z[k] - z[k - 1]);
-    auto e4 = [&](int I, int J, int K) {
-        return 0.5 * ((u(I - 1, J, K) - u(I - 1, J - 1, K)) / (yg[J] - yg[J - 1]) +
-                      (v(I, J - 1, K) - v(I - 1, J - 1, K)) / (xg[I] - xg[I - 1]));
+    // profile of the deviation storage at j-1, j, j+1
+    const R ub0 = ubar ? (R)ubar[j - 1] : (R)0, ub1 = ubar ? (R)ubar[j] : (R)0,
+            ub2 = ubar ? (R)ubar[j + 1] : (R)0;
+#define u(I, J, K) ((R)U[i3(nx, nyg, I, J, K)] + ((J) == j ? ub1 : ((J) < j ? ub0 : ub2)))
+#define v(I, J, K) ((R)V[i3(nxg, ny, I, J, K)])
+#define w(I, J, K) ((R)W[i3(nxg, nyg, I, J, K)])
+    // 1/(x(i)-x(i-1)) = rdxf(i); 1/(xg(I)-xg(I-1)) = rdxc(I-1); likewise y, z
+    const R rxf = (R)m.rdxf[i], ryf = (R)m.rdyf[j], rzf = (R)m.rdzf[k];
+    const R rxc0 = (R)m.rdxc[i - 1], rxc1 = (R)m.rdxc[i];
+    const R ryc0 = (R)m.rdyc[j - 1], ryc1 = (R)m.rdyc[j];
+    const R rzc0 = (R)m.rdzc[k - 1], rzc1 = (R)m.rdzc[k];
+    const R h = (R)0.5, q = (R)0.25;
+    const R s11 = (u(i, j, k) - u(i - 1, j, k)) * rxf;
+    const R s22 = (v(i, j, k) - v(i, j - 1, k)) * ryf;
+    const R s33 = (w(i, j, k) - w(i, j, k - 1)) * rzf;
+    // edge values at (I, J, K), I in {i, i+1}, J in {j, j+1}, K in {k, k+1}
+    auto e4 = [&](int I, int J, int K, R ryc, R rxc) {
+        return h * ((u(I - 1, J, K) - u(I - 1, J - 1, K)) * ryc + (v(I, J - 1, K) - v(I - 1, J - 1, K)) * rxc);
     };
-    auto e5 = [&](int I, int J, int K) {
-        return 0.5 * ((u(I - 1, J, K) - u(I - 1, J, K - 1)) / (zg[K] - zg[K - 1]) +
-                      (w(I, J, K - 1) - w(I - 1, J, K - 1)) / (xg[I] - xg[I - 1]));
+    auto e5 = [&](int I, int J, int K, R rzc, R rxc) {
+        return h * ((u(I - 1, J, K) - u(I - 1, J, K - 1)) * rzc + (w(I, J, K - 1) - w(I - 1, J, K - 1)) * rxc);
     };
-    auto e6 = [&](int I, int J, int K) {
-        return 0.5 * ((v(I, J - 1, K) - v(I, J - 1, K - 1)) / (zg[K] - zg[K - 1]) +
-                      (w(I, J, K - 1) - w(I, J - 1, K - 1)) / (yg[J] - yg[J - 1]));
+    auto e6 = [&](int I, int J, int K, R rzc, R ryc) {
+        return h * ((v(I, J - 1, K) - v(I, J - 1, K - 1)) * rzc + (w(I, J, K - 1) - w(I, J - 1, K - 1)) * ryc);
     };
-    const double s12 = 0.25 * (e4(i, j, k) + e4(i + 1, j, k) + e4(i, j + 1, k) + e4(i + 1, j + 1, k));
-    const double s13 = 0.25 * (e5(i, j, k) + e5(i + 1, j, k) + e5(i, j, k + 1) + e5(i + 1, j, k + 1));
-    const double s23 = 0.25 * (e6(i, j, k) + e6(i, j + 1, k) + e6(i, j, k + 1) + e6(i, j + 1, k + 1));
-    double s = 0.0;
-    s = s + 2.0 * s11 * s11;
-    s = s + 2.0 * s22 * s22;
-    s = s + 2.0 * s33 * s33;
-    s = s + 4.0 * s12 * s12;
-    s = s + 4.0 * s13 * s13;
-    s = s + 4.0 * s23 * s23;
+    const R s12 = q * (e4(i, j, k, ryc0, rxc0) + e4(i + 1, j, k, ryc0, rxc1) + e4(i, j + 1, k, ryc1, rxc0) +
+                       e4(i + 1, j + 1, k, ryc1, rxc1));
+    const R s13 = q * (e5(i, j, k, rzc0, rxc0) + e5(i + 1, j, k, rzc0, rxc1) + e5(i, j, k + 1, rzc1, rxc0) +
+                       e5(i + 1, j, k + 1, rzc1, rxc1));
+    const R s23 = q * (e6(i, j, k, rzc0, ryc0) + e6(i, j + 1, k, rzc0, ryc1) + e6(i, j, k + 1, rzc1, ryc0) +
+                       e6(i, j + 1, k + 1, rzc1, ryc1));
+    R s = 0;
+    s = s + (R)2 * s11 * s11;
+    s = s + (R)2 * s22 * s22;
+    s = s + (R)2 * s33 * s33;
+    s = s + (R)4 * s12 * s12;
+    s = s + (R)4 * s13 * s13;
+    s = s + (R)4 * s23 * s23;
     double f = 1.0;
     if (damping) {
         const double utau_t = sqrt(sc->dPdx);
         if (j < nyg / 2) f = 1.0 - exp(-(yg[j] - y[1]) / 25.0 / (nu / utau_t));
         else f = 1.0 - exp(-(y[ny] - yg[j]) / 25.0 / (nu / utau_t));
     }
-    const double c = cs * delta[j] * f;
-    const double val = fmax((c * c) * sqrt(s), 0.0);
+    const R c = (R)(cs * delta[j] * f);
+    const R val = fmax((c * c) * sqrt(s), (R)0);
     NT[i3(nxg, nyg, i, j, k)] = (S)val;
 #undef u
 #undef v
@@ -993,13 +1003,12 @@ void launch_div(const Dims &d, const double *x, const double *y, const double *z
 }
 
 template <typename S>
-void launch_smagorinsky(const Dims &d, const double *x, const double *y, const double *z,
-                        const double *xg, const double *yg, const double *zg, const double *delta,
-                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc,
-                        const double *ubar, double cs, double nu, int damping, hipStream_t st) {
+void launch_smagorinsky(const Dims &d, const Metrics &m, const double *y, const double *yg, const double *delta,
+                        const S *U, const S *V, const S *W, S *NT, const Scalars *sc, const double *ubar,
+                        double cs, double nu, int damping, hipStream_t st) {
     const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * d.nzp;
-    hipLaunchKernelGGL((k_smagorinsky<S>), dim3(nt), dim3(256), 0, st, d, x, y, z, xg, yg, zg, delta,
-                       U, V, W, NT, sc, ubar, cs, nu, damping, tx, ty, nt);
+    hipLaunchKernelGGL((k_smagorinsky<S, S>), dim3(nt), dim3(256), 0, st, d, m, y, yg, delta, U, V, W, NT, sc,
+                       ubar, cs, nu, damping, tx, ty, nt);
 }
 
 template <typename S>
@@ -1032,8 +1041,7 @@ INST(float, float)
                                Scalars *, double, double, hipStream_t);                                \
     template void launch_div<S>(const Dims &, const double *, const double *, const double *,         \
                                 const S *, const S *, const S *, double *, hipStream_t);              \
-    template void launch_smagorinsky<S>(const Dims &, const double *, const double *, const double *, \
-                                        const double *, const double *, const double *,               \
+    template void launch_smagorinsky<S>(const Dims &, const Metrics &, const double *, const double *,   \
                                         const double *, const S *, const S *, const S *, S *,         \
                                         const Scalars *, const double *, double, double, int,         \
                                         hipStream_t);                                                  \
