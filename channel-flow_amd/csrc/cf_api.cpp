@@ -454,10 +454,10 @@ void Ctx<S>::wall_model(S *const *F) {
     const double hw = h.yg[h.prm.jref_eqwm - 1];
     const double *pred = nullptr;
     const int *use = nullptr;
-    if (surrogate) surrogate(this, F, &pred, &use);
     tic("reset_counters");
     launch_reset_counters(sc, stream);
     toc();
+    if (surrogate) surrogate(this, F, &pred, &use);
     tic("wall_model");
     launch_wall_model<S>(d, F[0], F[2], h.prm.les_model == 1 ? NT : nullptr, wt, tau_init, alpha_x, azt,
                          alpha_z, sc, ep, h.prm.nu, hw, h.prm.jref_eqwm, h.prm.tau_mean_eqwm, pred, use,
@@ -959,7 +959,6 @@ int Ctx<S>::attach_mlp(Mlp *m) {
             dalloc(&use_pred, n, owned, bytes, (Slab *)nullptr))
             return -1;
     }
-    if (m->ensure_wall(n)) return -1;
     const double hw = h.yg[h.prm.jref_eqwm - 1];
     surrogate = [this, m, n, hw](Ctx<S> *, S *const *F, const double **pred, const int **use) {
         tic("mlp_wall");

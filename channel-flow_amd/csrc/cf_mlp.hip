@@ -8,11 +8,12 @@
 // 64->1 output layer are FP32 FMAs (K = 4 and N = 1 do not fill a matrix
 // tile).  Master weights and Adam moments are FP32.
 //
-// One wavefront owns 16 samples at a time; activations live in LDS in the
-// wavefront's own region, so no workgroup barriers are needed.  Gradients
-// of the 64x64 layers accumulate in MFMA accumulators across all chunks a
-// wavefront processes, then one partial per wavefront is reduced in a fixed
-// order (deterministic).
+// Training: one single-wavefront workgroup owns 16 samples at a time, with
+// its activations in LDS (the backward pass needs them in both layouts).
+// Gradients of the 64x64 layers accumulate in MFMA accumulators across all
+// chunks a wavefront processes, then one partial per wavefront is reduced
+// in a fixed order (deterministic).  Inference (k_mlp_infer) keeps the
+// activations in registers from features to stress.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -263,43 +264,160 @@ __host__ __device__ inline void mlp_features(double u, double h, double nu, doub
     for (int q = 0; q < 4; q++) f[q] = (float)((r[q] - nrm[q]) / nrm[4 + q]);
 }
 
-// per-point arrays, or constants (h, nu) and a device scalar (dp/dx) when
-// the arrays are null
-__global__ void k_mlp_feats(int n, const double *__restrict__ u, const double *__restrict__ h,
-                            const double *__restrict__ nu, const double *__restrict__ dpdx, double hc,
-                            double nuc, const double *__restrict__ dpc, const float *__restrict__ nrm,
-                            const float *__restrict__ hull, float *__restrict__ X, int *__restrict__ inside) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    float f[4];
-    const double uq = u[q];
-    mlp_features(uq, h ? h[q] : hc, nu ? nu[q] : nuc, dpdx ? dpdx[q] : (dpc ? *dpc : 0.0), nrm, f);
-    // the network sees the true features (it extrapolates outside the hull;
-    // callers take the ODE there); non-finite ones (u <= 0) become 0
-    bool in = uq > 0.0 && isfinite(uq);
-    for (int c = 0; c < 4; c++) {
-        X[(size_t)q * 4 + c] = isfinite(f[c]) ? f[c] : 0.f;
-        in = in && f[c] >= hull[c] && f[c] <= hull[4 + c];
+// ---------------------------------------------------------------------------
+// inference
+// ---------------------------------------------------------------------------
+// The 64x64 layers run as transposed products H^T = W^T A^T on
+// v_mfma_f32_16x16x16bf16_1k: the weights are the A operand and the 16
+// samples of a wavefront are the columns.  The accumulator of output tile t
+// holds, in lane (r16, q4), units 16t+4q4+i of sample r16 -- the B-operand
+// layout of reduction block t of the next layer -- so activations stay in
+// registers from the features to the output.  The BF16 A-operand fragments
+// of both 64x64 layers are loaded into registers once per wavefront and
+// reused over its chunks of 16 samples; the FP32 input- and output-layer
+// weights and the biases are read from LDS as float4 per 4 units.
+//
+// tanh here is 1 - 2/(2^(2z log2 e) + 1) on v_exp_f32 and v_rcp_f32:
+// absolute error below 5e-7 against tanhf, far under the BF16 rounding
+// (relative 2^-9) the activations take before the next layer.
+
+// raw inputs: per-sample arrays, or constants (h, nu) and a device scalar
+// (dp/dx) where the arrays are null
+struct MlpInputs {
+    const double *u, *h, *nu, *dpdx;
+    double hc, nuc;
+    const double *dpc;
+};
+
+__device__ inline float tanh_fast(float z) {
+    const float e = __builtin_amdgcn_exp2f(z * 2.8853900817779268f);  // exp(2z)
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+constexpr int MLP_INFER_WG = 256;
+
+// BF16 bits of x, rounded to nearest even (v_cvt_pk_bf16_f32; the same
+// rounding as f2bf for finite x)
+__device__ inline short bf16_bits(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+
+// raw feature q4 of sample q: ln u, ln h, ln nu, asinh(dp/dx / 1e-3)
+__device__ inline double mlp_raw_feature(const MlpInputs &in, int q4, int q) {
+    if (q4 == 3) return asinh((in.dpdx ? in.dpdx[q] : (in.dpc ? *in.dpc : 0.0)) / 1e-3);
+    const double a = q4 == 0 ? in.u[q] : (q4 == 1 ? (in.h ? in.h[q] : in.hc) : (in.nu ? in.nu[q] : in.nuc));
+    return log(a);
+}
+
+// tau[q] = exp(y * std + mean) of the network output; where the sample lies
+// outside the training box, inside[q] = 0 and (mask_tau) tau[q] = 0.  nfall,
+// when given, gains the number of outside samples.
+__global__ void __launch_bounds__(MLP_INFER_WG)
+k_mlp_infer(int n, MlpInputs in, const float *__restrict__ P, const unsigned short *__restrict__ W2b,
+            const unsigned short *__restrict__ W3b, const float *__restrict__ nrm, const float *__restrict__ hull,
+            int mask_tau, double *__restrict__ tau, int *__restrict__ inside, int *__restrict__ nfall) {
+    using L = PLayout;
+    // FP32 parameters of the input and output layers and the biases, shared
+    // by the workgroup: [W1 (4 x 64) | b1 | b2 | b3 | w4], unit-fastest
+    __shared__ float4 sp[8 * MLP_H / 4];
+    for (int e = threadIdx.x; e < 8 * MLP_H; e += MLP_INFER_WG) {
+        const int blk = e / MLP_H, u = e % MLP_H;
+        const int src = blk < 4 ? L::W1 + blk * MLP_H + u
+                                : (blk == 4 ? L::B1 : blk == 5 ? L::B2 : blk == 6 ? L::B3 : L::W4) + u;
+        reinterpret_cast<float *>(sp)[e] = P[src];
     }
-    if (inside) inside[q] = in ? 1 : 0;
+    __syncthreads();
+    const int l = threadIdx.x & 63;
+    const int r16 = l & 15, q4 = l >> 4;
+    const int wave = blockIdx.x * (MLP_INFER_WG / 64) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (MLP_INFER_WG / 64);
+    // lane (r16, q4) owns units 16t + 4q4 + i: the float4 at sp[blk*16 + 4t + q4]
+    auto par = [&](int blk, int t) { return sp[blk * (MLP_H / 4) + 4 * t + q4]; };
+    bf16x4 a2[4][4], a3[4][4];  // [output tile][reduction block]
+    for (int t = 0; t < 4; t++)
+        for (int kk = 0; kk < 4; kk++)
+            for (int i = 0; i < 4; i++) {
+                const int k = 16 * kk + 4 * q4 + i;
+                a2[t][kk][i] = (short)W2b[k * MLP_H + 16 * t + r16];
+                a3[t][kk][i] = (short)W3b[k * MLP_H + 16 * t + r16];
+            }
+    const float b4 = P[L::B4];
+    // lane q4 standardises feature q4 of sample r16; features that do not
+    // vary per sample are evaluated once
+    const double mu = nrm[q4], sd = nrm[4 + q4];
+    const float lo = hull[q4], hi = hull[4 + q4];
+    const double ym = nrm[8], ys = nrm[9];
+    const bool per_sample = q4 == 0 || (q4 == 1 && in.h) || (q4 == 2 && in.nu) || (q4 == 3 && in.dpdx);
+    const double rconst = per_sample ? 0.0 : mlp_raw_feature(in, q4, 0);
+    int c = wave;
+    double rnext = (per_sample && c * 16 + r16 < n) ? mlp_raw_feature(in, q4, c * 16 + r16) : rconst;
+    for (; c * 16 < n; c += nwaves) {
+        const int q = c * 16 + r16;
+        const bool valid = q < n;
+        const double r = rnext;
+        {   // the next chunk's input, in flight while this one computes
+            const int qn = q + 16 * nwaves;
+            if (per_sample && qn < n) rnext = mlp_raw_feature(in, q4, qn);
+        }
+        float f = (float)((r - mu) / sd);
+        const bool fin = valid && f >= lo && f <= hi;  // false for NaN and infinities
+        if (!valid || !isfinite(f)) f = 0.0f;
+        const unsigned long long m = __ballot(fin);
+        const bool ok = ((m >> r16) & (m >> (r16 + 16)) & (m >> (r16 + 32)) & (m >> (r16 + 48)) & 1ull) != 0;
+        float x[4];
+        for (int g = 0; g < 4; g++) x[g] = __shfl(f, r16 + 16 * g, 64);
+        // layer 1 (FP32)
+        bf16x4 h1[4];
+        for (int t = 0; t < 4; t++) {
+            const float4 bb = par(4, t), w0 = par(0, t), w1 = par(1, t), w2 = par(2, t), w3 = par(3, t);
+            const float zb[4] = {bb.x, bb.y, bb.z, bb.w};
+            const float z0[4] = {w0.x, w0.y, w0.z, w0.w}, z1[4] = {w1.x, w1.y, w1.z, w1.w};
+            const float z2[4] = {w2.x, w2.y, w2.z, w2.w}, z3[4] = {w3.x, w3.y, w3.z, w3.w};
+            for (int i = 0; i < 4; i++) {
+                float z = zb[i];
+                z += x[0] * z0[i];
+                z += x[1] * z1[i];
+                z += x[2] * z2[i];
+                z += x[3] * z3[i];
+                h1[t][i] = bf16_bits(tanh_fast(z));
+            }
+        }
+        // layer 2
+        bf16x4 h2[4];
+        for (int t = 0; t < 4; t++) {
+            floatx4 acc = {0, 0, 0, 0};
+            for (int kk = 0; kk < 4; kk++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][kk], h1[kk], acc, 0, 0, 0);
+            const float4 bb = par(5, t);
+            const float zb[4] = {bb.x, bb.y, bb.z, bb.w};
+            for (int i = 0; i < 4; i++) h2[t][i] = bf16_bits(tanh_fast(acc[i] + zb[i]));
+        }
+        // layer 3 and the output layer (FP32)
+        float y = 0.0f;
+        for (int t = 0; t < 4; t++) {
+            floatx4 acc = {0, 0, 0, 0};
+            for (int kk = 0; kk < 4; kk++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[t][kk], h2[kk], acc, 0, 0, 0);
+            const float4 bb = par(6, t), ww = par(7, t);
+            const float zb[4] = {bb.x, bb.y, bb.z, bb.w}, zw[4] = {ww.x, ww.y, ww.z, ww.w};
+            for (int i = 0; i < 4; i++) y += tanh_fast(acc[i] + zb[i]) * zw[i];
+        }
+        y += __shfl_xor(y, 16, 64);
+        y += __shfl_xor(y, 32, 64);
+        y += b4;
+        if (valid && q4 == 0) {
+            tau[q] = (!mask_tau || ok) ? exp((double)y * ys + ym) : 0.0;
+            if (inside) inside[q] = ok ? 1 : 0;
+        }
+        if (nfall) {
+            const unsigned long long out = __ballot(valid && q4 == 0 && !ok);
+            if (l == 0 && out) atomicAdd(nfall, (int)__popcll(out));
+        }
+    }
 }
 
-// tau = exp(y * ystd + ymean) where inside the hull
-__global__ void k_mlp_tau(int n, const float *__restrict__ y, const float *__restrict__ nrm,
-                          const int *__restrict__ inside, double *__restrict__ tau) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    tau[q] = (inside == nullptr || inside[q]) ? exp((double)y[q] * nrm[9] + nrm[8]) : 0.0;
-}
-
-__global__ void k_count(int n, const int *__restrict__ inside, Scalars *sc) {
-    __shared__ int c;
-    if (threadIdx.x == 0) c = 0;
-    __syncthreads();
-    for (int q = threadIdx.x; q < n; q += blockDim.x)
-        if (!inside[q]) atomicAdd(&c, 1);
-    __syncthreads();
-    if (threadIdx.x == 0) sc->n_fallback_mlp = c;
+// waves of the inference grid: 2048 (two per SIMD on 256 CUs) at most, so
+// that each wave reuses its registered weights over several chunks
+inline int mlp_infer_blocks(int n) {
+    const int chunks = (n + 15) / 16;
+    const int waves = std::min(chunks, 2048);
+    return (waves + MLP_INFER_WG / 64 - 1) / (MLP_INFER_WG / 64);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,35 +598,15 @@ int Mlp::step(size_t off, int bs, float lr, bool update, float *grad_out, float 
     return 0;
 }
 
-// forward of n standardised samples already in X -> yo
-int Mlp::forward_dev(const float *Xd, int n, float *ydev, hipStream_t st) {
-    const int chunks = 4;
-    const int nw = (n + 16 * chunks - 1) / (16 * chunks);
-    hipLaunchKernelGGL(k_mlp<false>, dim3(nw), dim3(64), 0, st, Xd, (const float *)nullptr, n, chunks, 1.0f, P,
-                       W2b, W3b, ydev, (float *)nullptr, (float *)nullptr);
-    return 0;
-}
-
-// wall-model hook: features of the wall points, forward, tau and mask
+// wall-model hook: network stress at every wall point inside the training
+// box, inside flags for the ODE fallback, and the count of fallbacks (the
+// caller zeroes it at the start of the wall-model evaluation)
 void Mlp::wall_predict(int n, const double *u_les, double h, double nu, const double *dpdx_dev,
                        double *tau_pred, int *use_pred, Scalars *sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_mlp_feats, dim3((n + 255) / 256), dim3(256), 0, st, n, u_les, (const double *)nullptr,
-                       (const double *)nullptr, (const double *)nullptr, h, nu, dpdx_dev, nrm_d, hull_d, wX,
-                       use_pred);
-    forward_dev(wX, n, wY, st);
-    hipLaunchKernelGGL(k_mlp_tau, dim3((n + 255) / 256), dim3(256), 0, st, n, wY, nrm_d, use_pred, tau_pred);
-    hipLaunchKernelGGL(k_count, dim3(1), dim3(256), 0, st, n, use_pred, sc);
-}
-
-int Mlp::ensure_wall(int n) {
-    if (n > cap_wall) {
-        MCK(hipMalloc(&wX, (size_t)n * 16));
-        MCK(hipMalloc(&wY, (size_t)n * 4));
-        owned.push_back(wX);
-        owned.push_back(wY);
-        cap_wall = n;
-    }
-    return 0;
+    if (n <= 0) return;
+    const MlpInputs in{u_les, nullptr, nullptr, nullptr, h, nu, dpdx_dev};
+    hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_infer_blocks(n)), dim3(MLP_INFER_WG), 0, st, n, in, P, W2b, W3b, nrm_d,
+                       hull_d, 1, tau_pred, use_pred, &sc->n_fallback_mlp);
 }
 
 }  // namespace cf
@@ -609,28 +707,24 @@ int cf_mlp_predict(cf_mlp *p, long long n, const double *feat, double *tau, int 
     cf::Mlp &m = p->m;
     double *d = nullptr;
     int *di = nullptr;
-    float *xd = nullptr, *yd = nullptr;
     const size_t nn = (size_t)n;
     hipError_t e = hipMalloc(&d, nn * 5 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&di, nn * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&xd, nn * 16);
-    if (e == hipSuccess) e = hipMalloc(&yd, nn * 4);
     std::vector<double> cols(nn * 4);
     for (size_t q = 0; q < nn; q++)
         for (int c = 0; c < 4; c++) cols[c * nn + q] = feat[q * 4 + c];
     if (e == hipSuccess) e = hipMemcpy(d, cols.data(), nn * 4 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(cf::k_mlp_feats, dim3((n + 255) / 256), dim3(256), 0, m.stream, (int)n, d, d + nn,
-                           d + 2 * nn, d + 3 * nn, 0.0, 0.0, (const double *)nullptr, m.nrm_d, m.hull_d, xd, di);
-        m.forward_dev(xd, (int)n, yd, m.stream);
-        hipLaunchKernelGGL(cf::k_mlp_tau, dim3((n + 255) / 256), dim3(256), 0, m.stream, (int)n, yd, m.nrm_d,
-                           (const int *)nullptr, d + 4 * nn);
+        const cf::MlpInputs in{d, d + nn, d + 2 * nn, d + 3 * nn, 0.0, 0.0, nullptr};
+        hipLaunchKernelGGL(cf::k_mlp_infer, dim3(cf::mlp_infer_blocks((int)n)), dim3(cf::MLP_INFER_WG), 0, m.stream,
+                           (int)n, in, m.P, m.W2b, m.W3b, m.nrm_d, m.hull_d, 0, d + 4 * nn, di, (int *)nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(m.stream);
     if (e == hipSuccess) e = hipMemcpy(tau, d + 4 * nn, nn * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && inside) e = hipMemcpy(inside, di, nn * sizeof(int), hipMemcpyDeviceToHost);
-    hipFree(d); hipFree(di); hipFree(xd); hipFree(yd);
+    hipFree(d);
+    hipFree(di);
     if (e != hipSuccess) { set_error(std::string("cf_mlp_predict: ") + hipGetErrorString(e)); return -1; }
     return 0;
 }

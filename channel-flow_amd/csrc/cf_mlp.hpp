@@ -22,20 +22,17 @@ struct Mlp {
     float nrm_h[16] = {0, 0, 0, 0, 1, 1, 1, 1, 0, 1};     // feature mean[4], std[4], ln tau mean, std
     float hull_h[8] = {-1e30f, -1e30f, -1e30f, -1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
     float *X = nullptr, *Yd = nullptr, *yo = nullptr, *gpart = nullptr, *lpart = nullptr, *gbuf = nullptr;
-    float *wX = nullptr, *wY = nullptr;
     size_t cap_data = 0, cap_waves = 0, ndata = 0;
-    int cap_wall = 0, adam_t = 0;
+    int adam_t = 0;
 
     ~Mlp();
     int init(unsigned seed);
     int set_params(const float *h);
     int get_params(float *h);
     int ensure(size_t ndata, int nwaves);
-    int ensure_wall(int n);
     int fit_normalisation(const double *feat_raw, const double *tau, size_t n);
     int upload(const double *feat_raw, const double *tau, size_t n);
     int step(size_t off, int bs, float lr, bool update, float *grad_out, float *loss_out);
-    int forward_dev(const float *Xd, int n, float *ydev, hipStream_t st);
     void wall_predict(int n, const double *u_les, double h, double nu, const double *dpdx_dev,
                       double *tau_pred, int *use_pred, Scalars *sc, hipStream_t st);
 };

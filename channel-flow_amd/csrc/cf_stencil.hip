@@ -1065,6 +1065,7 @@ namespace cf {
 __global__ void k_reset_counters(Scalars *sc) {
     sc->n_nan_eqwm = 0;  // wallmodel.f90:613-614
     sc->n_iter_eqwm = 0;
+    sc->n_fallback_mlp = 0;
 }
 __global__ void k_dt_fixed(Scalars *sc, double cfl) { sc->dt = -cfl; }
 // t = to + rk_t(3)*dt with rk_t(3) = 1 (time_integration.f90:146)
