@@ -306,14 +306,20 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ cscr, const
 //   the (0,0) mode.  The matrix is diagonally dominant (kxx, kzz <= 0), so
 //   elimination without pivoting is used; LAPACK zgtsv's row interchanges,
 //   when they happen, change results at round-off only.
-// Partitioned solve, one read and one write of the spectrum: a workgroup
-// takes one kx and KZ consecutive kz; thread (kz, s) holds rows
-// [s*MS, s*MS+MS) of its system in registers and solves that segment for
-// the right-hand side (y) and for unit couplings to the neighbouring
-// segments' end values (v: lower, w: upper), so x = y + alpha L(s-1) +
-// beta F(s+1) with F, L the first and last unknown of each segment.  The
-// 2x2-block system of the segment end values is solved by one thread per
-// system through LDS, then every thread assembles and stores its rows.
+// Partitioned solve, one read and one write of the spectrum: a tile is one
+// kx and KZ consecutive kz; thread (kz, s) holds rows [s*MS, s*MS+MS) of
+// its system in registers and solves that segment for the right-hand side
+// (y) and for unit couplings to the neighbouring segments' end values
+// (v: lower, w: upper), so x = y + alpha L(s-1) + beta F(s+1) with F, L the
+// first and last unknown of each segment.  The 2x2-block system of the
+// segment end values is solved by one thread per system through LDS, then
+// every thread assembles and stores its rows.
+// The workgroups are persistent: each walks the tiles with stride gridDim.x
+// and issues the loads of its next tile before solving the current one, so
+// that the memory latency of a tile overlaps the sweeps and the reduced
+// solve of the previous (one workgroup fills a CU's registers).  The y
+// operator and the wavenumbers sit in LDS, so no other global load queues
+// behind the prefetch.
 // Arithmetic is FP64 in every mode.
 // ---------------------------------------------------------------------------
 template <typename T, int MS>
@@ -321,117 +327,140 @@ __global__ void __launch_bounds__(512)
 k_thomas(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ kxx, const double *__restrict__ kzz,
          const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
          int KZ, int P) {
-    extern __shared__ double sh[];  // 8 fields per (segment, kz), field-major: sh[f * NT + slot]
-    const int nzp = d.nzp, nyp = d.nyp;
-    const int nkzb = (nzp + KZ - 1) / KZ;
-    const int kx = blockIdx.x / nkzb, kz0 = (blockIdx.x - kx * nkzb) * KZ;
+    // [8 fields per (segment, kz), field-major: sh[f * NT + slot]] | dd | dl | du | kxx | kzz
+    extern __shared__ double sh[];
+    const int nzp = d.nzp, nyp = d.nyp, mh = d.mh;
+    const int nkzb = (nzp + KZ - 1) / KZ, ntiles = mh * nkzb;
     const int tid = threadIdx.x, kzl = tid % KZ, sg = tid / KZ;
-    const int kz = kz0 + kzl;
-    const bool active = sg < P && kz < nzp;
+    const int NT = blockDim.x;
+    double *s_dd = sh + 8 * NT, *s_dl = s_dd + nyp, *s_du = s_dl + nyp, *s_kx = s_du + nyp, *s_kz = s_kx + mh;
+    for (int e = tid; e < nyp; e += NT) { s_dd[e] = dd[e]; s_dl[e] = dl[e]; s_du[e] = du[e]; }
+    for (int e = tid; e < mh; e += NT) s_kx[e] = kxx[e];
+    for (int e = tid; e < nzp; e += NT) s_kz[e] = kzz[e];
     const int a = sg * MS;
-    const int len = active ? min(MS, nyp - a) : 0;
-    const int64_t rs = (int64_t)d.mh * nzp;
-    Cx<T> *col = spec + (int64_t)kx * nzp + kz;
-    const double kx2 = kxx[kx], kz2 = kz < nzp ? kzz[kz] : 0.0;
-    const bool zm = kx == 0 && kz == 0;
-    double yr[MS], yi[MS], v[MS], w[MS], cp[MS];
-#pragma unroll
-    for (int i = 0; i < MS; i++) {
-        yr[i] = yi[i] = v[i] = w[i] = cp[i] = 0.0;
-        if (i < len) {
-            const Cx<T> b = col[(a + i) * rs];
-            yr[i] = (double)b.x;
-            yi[i] = (double)b.y;
-        }
-    }
-    // segment forward sweep for the three right-hand sides (b, e_first, e_last)
-    {
-        double cprev = 0.0, yrp = 0.0, yip = 0.0, vp = 0.0, wp = 0.0;
+    const int len = sg < P ? min(MS, nyp - a) : 0;
+    const int64_t rs = (int64_t)mh * nzp;
+    // couplings to the neighbouring segments' end values
+    const double lo_s = (sg < P && sg > 0) ? -dl[a - 1] : 0.0;        // times L(s-1)
+    const double up_s = (sg < P && a + len < nyp) ? -du[a + len - 1] : 0.0;  // times F(s+1)
+    auto load = [&](int tile, double *r, double *im) {
+        const int kx = tile / nkzb, kz = (tile - kx * nkzb) * KZ + kzl;
+        const bool act = tile < ntiles && sg < P && kz < nzp;
+        const Cx<T> *c = spec + (int64_t)kx * nzp + kz;
 #pragma unroll
         for (int i = 0; i < MS; i++) {
-            if (i < len) {
-                const int j = a + i;
-                double D = (dd[j] + kx2) + kz2;
-                if (j == 0 && zm) D = 1.5 * D;
-                const double lo = i > 0 ? dl[j - 1] : 0.0;
-                const double r = 1.0 / (D - lo * cprev);
-                cp[i] = (i < len - 1) ? du[j] * r : 0.0;
-                yr[i] = (yr[i] - lo * yrp) * r;
-                yi[i] = (yi[i] - lo * yip) * r;
-                v[i] = ((i == 0 ? 1.0 : 0.0) - lo * vp) * r;
-                w[i] = ((i == len - 1 ? 1.0 : 0.0) - lo * wp) * r;
-                cprev = cp[i]; yrp = yr[i]; yip = yi[i]; vp = v[i]; wp = w[i];
+            r[i] = im[i] = 0.0;
+            if (act && i < len) {
+                const Cx<T> b = c[(a + i) * rs];
+                r[i] = (double)b.x;
+                im[i] = (double)b.y;
             }
         }
+    };
+    double yr[MS], yi[MS];
+    int tile = blockIdx.x;
+    load(tile, yr, yi);
+    __syncthreads();
+    for (; tile < ntiles; tile += gridDim.x) {
+        double nr[MS], ni[MS];
+        load(tile + gridDim.x, nr, ni);  // the next tile, in flight while this one is solved
+        const int kx = tile / nkzb, kz = (tile - kx * nkzb) * KZ + kzl;
+        const bool active = sg < P && kz < nzp;
+        const double kx2 = s_kx[kx], kz2 = kz < nzp ? s_kz[kz] : 0.0;
+        const bool zm = kx == 0 && kz == 0;
+        double v[MS], w[MS], cp[MS];
+        // segment forward sweep for the three right-hand sides (b, e_first, e_last)
+        {
+            double cprev = 0.0, yrp = 0.0, yip = 0.0, vp = 0.0, wp = 0.0;
 #pragma unroll
-        for (int i = MS - 2; i >= 0; i--) {
-            if (i + 1 < len) {
-                yr[i] -= cp[i] * yr[i + 1];
-                yi[i] -= cp[i] * yi[i + 1];
-                v[i] -= cp[i] * v[i + 1];
-                w[i] -= cp[i] * w[i + 1];
+            for (int i = 0; i < MS; i++) {
+                v[i] = w[i] = cp[i] = 0.0;
+                if (i < len) {
+                    const int j = a + i;
+                    double D = (s_dd[j] + kx2) + kz2;
+                    if (j == 0 && zm) D = 1.5 * D;
+                    const double lo = i > 0 ? s_dl[j - 1] : 0.0;
+                    const double r = 1.0 / (D - lo * cprev);
+                    cp[i] = (i < len - 1) ? s_du[j] * r : 0.0;
+                    yr[i] = (yr[i] - lo * yrp) * r;
+                    yi[i] = (yi[i] - lo * yip) * r;
+                    v[i] = ((i == 0 ? 1.0 : 0.0) - lo * vp) * r;
+                    w[i] = ((i == len - 1 ? 1.0 : 0.0) - lo * wp) * r;
+                    cprev = cp[i]; yrp = yr[i]; yip = yi[i]; vp = v[i]; wp = w[i];
+                }
+            }
+#pragma unroll
+            for (int i = MS - 2; i >= 0; i--) {
+                if (i + 1 < len) {
+                    yr[i] -= cp[i] * yr[i + 1];
+                    yi[i] -= cp[i] * yi[i + 1];
+                    v[i] -= cp[i] * v[i + 1];
+                    w[i] -= cp[i] * w[i + 1];
+                }
             }
         }
-    }
-    // couplings to the neighbouring segments' end values
-    const double lo_c = (active && sg > 0) ? -dl[a - 1] : 0.0;             // times L(s-1)
-    const double up_c = (active && a + len < nyp) ? -du[a + len - 1] : 0.0;  // times F(s+1)
-    const int NT = blockDim.x;
+        const double lo_c = active ? lo_s : 0.0, up_c = active ? up_s : 0.0;
 #define SH(f, slot) sh[(size_t)(f) * NT + (slot)]
-    if (active) {
-        double ymr = 0, ymi = 0, vm = 0, wm = 0;
+        if (active) {
+            double ymr = 0, ymi = 0, vm = 0, wm = 0;
 #pragma unroll
-        for (int i = 0; i < MS; i++)
-            if (i == len - 1) { ymr = yr[i]; ymi = yi[i]; vm = v[i]; wm = w[i]; }
-        SH(0, tid) = yr[0]; SH(1, tid) = yi[0]; SH(2, tid) = ymr; SH(3, tid) = ymi;
-        SH(4, tid) = lo_c * v[0]; SH(5, tid) = lo_c * vm;  // alpha at the first / last row
-        SH(6, tid) = up_c * w[0]; SH(7, tid) = up_c * wm;  // beta at the first / last row
-    }
-    __syncthreads();
-    // end values: F(t) = P(t) + Q(t) F(t+1), L(t) = G(t) + H(t) F(t+1)
-    if (sg == 0 && kz < nzp) {
-        double Gr = 0, Gi = 0, H = 0;
-        for (int t = 0; t < P; t++) {
-            const int q = t * KZ + kzl;
-            const double y1r = SH(0, q), y1i = SH(1, q), ymr = SH(2, q), ymi = SH(3, q);
-            const double a1 = SH(4, q), am = SH(5, q), b1 = SH(6, q), bm = SH(7, q);
-            const double rD = 1.0 / (1.0 - a1 * H);
-            const double Pr = (y1r + a1 * Gr) * rD, Pi = (y1i + a1 * Gi) * rD, Q = b1 * rD;
-            const double Gnr = ymr + am * Gr + am * H * Pr, Gni = ymi + am * Gi + am * H * Pi;
-            const double Hn = am * H * Q + bm;
-            SH(0, q) = Pr; SH(1, q) = Pi; SH(2, q) = Q; SH(3, q) = Gnr; SH(4, q) = Gni; SH(5, q) = Hn;
-            Gr = Gnr; Gi = Gni; H = Hn;
+            for (int i = 0; i < MS; i++)
+                if (i == len - 1) { ymr = yr[i]; ymi = yi[i]; vm = v[i]; wm = w[i]; }
+            SH(0, tid) = yr[0]; SH(1, tid) = yi[0]; SH(2, tid) = ymr; SH(3, tid) = ymi;
+            SH(4, tid) = lo_c * v[0]; SH(5, tid) = lo_c * vm;  // alpha at the first / last row
+            SH(6, tid) = up_c * w[0]; SH(7, tid) = up_c * wm;  // beta at the first / last row
         }
-        double Fr = 0, Fi = 0;
-        for (int t = P - 1; t >= 0; t--) {
-            const int q = t * KZ + kzl;
-            const double Pr = SH(0, q), Pi = SH(1, q), Q = SH(2, q), Gnr = SH(3, q), Gni = SH(4, q),
-                         Hn = SH(5, q);
-            const double Lr = Gnr + Hn * Fr, Li = Gni + Hn * Fi;
-            Fr = Pr + Q * Fr;
-            Fi = Pi + Q * Fi;
-            SH(0, q) = Fr; SH(1, q) = Fi; SH(2, q) = Lr; SH(3, q) = Li;
+        __syncthreads();
+        // end values: F(t) = P(t) + Q(t) F(t+1), L(t) = G(t) + H(t) F(t+1)
+        if (sg == 0 && kz < nzp) {
+            double Gr = 0, Gi = 0, H = 0;
+            for (int t = 0; t < P; t++) {
+                const int q = t * KZ + kzl;
+                const double y1r = SH(0, q), y1i = SH(1, q), ymr = SH(2, q), ymi = SH(3, q);
+                const double a1 = SH(4, q), am = SH(5, q), b1 = SH(6, q), bm = SH(7, q);
+                const double rD = 1.0 / (1.0 - a1 * H);
+                const double Pr = (y1r + a1 * Gr) * rD, Pi = (y1i + a1 * Gi) * rD, Q = b1 * rD;
+                const double Gnr = ymr + am * Gr + am * H * Pr, Gni = ymi + am * Gi + am * H * Pi;
+                const double Hn = am * H * Q + bm;
+                SH(0, q) = Pr; SH(1, q) = Pi; SH(2, q) = Q; SH(3, q) = Gnr; SH(4, q) = Gni; SH(5, q) = Hn;
+                Gr = Gnr; Gi = Gni; H = Hn;
+            }
+            double Fr = 0, Fi = 0;
+            for (int t = P - 1; t >= 0; t--) {
+                const int q = t * KZ + kzl;
+                const double Pr = SH(0, q), Pi = SH(1, q), Q = SH(2, q), Gnr = SH(3, q), Gni = SH(4, q),
+                             Hn = SH(5, q);
+                const double Lr = Gnr + Hn * Fr, Li = Gni + Hn * Fi;
+                Fr = Pr + Q * Fr;
+                Fi = Pi + Q * Fi;
+                SH(0, q) = Fr; SH(1, q) = Fi; SH(2, q) = Lr; SH(3, q) = Li;
+            }
         }
-    }
-    __syncthreads();
-    if (!active) return;
-    double Lpr = 0, Lpi = 0, Fnr = 0, Fni = 0;
-    if (sg > 0) {
-        const int q = (sg - 1) * KZ + kzl;
-        Lpr = SH(2, q); Lpi = SH(3, q);
-    }
-    if (sg + 1 < P) {
-        const int q = (sg + 1) * KZ + kzl;
-        Fnr = SH(0, q); Fni = SH(1, q);
-    }
+        __syncthreads();
+        if (active) {
+            double Lpr = 0, Lpi = 0, Fnr = 0, Fni = 0;
+            if (sg > 0) {
+                const int q = (sg - 1) * KZ + kzl;
+                Lpr = SH(2, q); Lpi = SH(3, q);
+            }
+            if (sg + 1 < P) {
+                const int q = (sg + 1) * KZ + kzl;
+                Fnr = SH(0, q); Fni = SH(1, q);
+            }
+            Cx<T> *col = spec + (int64_t)kx * nzp + kz;
+#pragma unroll
+            for (int i = 0; i < MS; i++) {
+                if (i < len) {
+                    const double al = lo_c * v[i], be = up_c * w[i];
+                    const double xr = yr[i] + al * Lpr + be * Fnr, xi = yi[i] + al * Lpi + be * Fni;
+                    col[(a + i) * rs] = Cx<T>{(T)xr, (T)xi};
+                }
+            }
+        }
 #undef SH
+        __syncthreads();  // the reduced-system slots are reused by the next tile
 #pragma unroll
-    for (int i = 0; i < MS; i++) {
-        if (i < len) {
-            const double al = lo_c * v[i], be = up_c * w[i];
-            const double xr = yr[i] + al * Lpr + be * Fnr, xi = yi[i] + al * Lpi + be * Fni;
-            col[(a + i) * rs] = Cx<T>{(T)xr, (T)xi};
-        }
+        for (int i = 0; i < MS; i++) { yr[i] = nr[i]; yi[i] = ni[i]; }
     }
 }
 
@@ -605,13 +634,21 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
     thomas_geometry(d, ms, KZ, P);
     const int nkzb = (d.nzp + KZ - 1) / KZ;
     const int threads = KZ * P;
-    const size_t shm = (size_t)threads * 8 * sizeof(double);
+    const size_t shm = ((size_t)threads * 8 + 3 * (size_t)d.nyp + d.mh + d.nzp) * sizeof(double);
+    // persistent workgroups: two per CU (one resident at a time; the second
+    // wave of workgroups evens out the tile counts)
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
+    }
+    const int grid = std::min(d.mh * nkzb, ncu * env_int("CF_TH_WGS_PER_CU", 2));
     if (ms == 4)
-        hipLaunchKernelGGL((k_thomas<T, 4>), dim3(d.mh * nkzb), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
-                           du, KZ, P);
+        hipLaunchKernelGGL((k_thomas<T, 4>), dim3(grid), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl, du, KZ,
+                           P);
     else
-        hipLaunchKernelGGL((k_thomas<T, 8>), dim3(d.mh * nkzb), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
-                           du, KZ, P);
+        hipLaunchKernelGGL((k_thomas<T, 8>), dim3(grid), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl, du, KZ,
+                           P);
 }
 
 template <typename S, typename T, typename P>
