@@ -98,6 +98,23 @@ __device__ __forceinline__ void small_divmod(int a, int b, float inv_b, int &q, 
     else if (r >= b) { q += 1; r -= b; }
 }
 
+// e = q n + r along e = e0, e0 + step, e0 + 2 step, ...: two divisions at
+// construction, then one add, one compare and at most one correction per step
+struct StepDiv {
+    int q, r, dq, dr, n;
+    __device__ __forceinline__ StepDiv(int e0, int step, int n_) : n(n_) {
+        q = e0 / n;
+        r = e0 - q * n;
+        dq = step / n;
+        dr = step - dq * n;
+    }
+    __device__ __forceinline__ void next() {
+        q += dq;
+        r += dr;
+        if (r >= n) { r -= n; q += 1; }
+    }
+};
+
 // One Stockham pass over `nlines` lines of length n held in LDS (line stride
 // ls).  MAXIT bounds the butterflies per thread: callers guarantee
 // nlines*n <= 8*WG.
@@ -187,15 +204,19 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     T *ldr = reinterpret_cast<T *>(lds);
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
     for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
-    for (int e = threadIdx.x; e < rb * N; e += WG) {
-        const int lr = e / N, ip = e - lr * N;
+    const float inz = 1.0f / (float)d.nzp;
+    StepDiv el(threadIdx.x, WG, N);  // element (row lr = q, ip = r)
+    for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
+        const int lr = el.q, ip = el.r;
         const int r = row0 + lr;
         T dv = (T)0;
         if (r < nrows) {
-            const int j = 2 + r / d.nzp, k = 2 + r % d.nzp, i = 2 + ip;
-            dv = ((T)U[i3(nx, nyg, i, j, k)] - (T)U[i3(nx, nyg, i - 1, j, k)]) * (T)m.rdxf[i] +
-                 ((T)V[i3(nxg, ny, i, j, k)] - (T)V[i3(nxg, ny, i, j - 1, k)]) * (T)m.rdyf[j] +
-                 ((T)W[i3(nxg, nyg, i, j, k)] - (T)W[i3(nxg, nyg, i, j, k - 1)]) * (T)m.rdzf[k];
+            int jr, kr;
+            small_divmod(r, d.nzp, inz, jr, kr);
+            const int j = 2 + jr, k = 2 + kr, i = 2 + ip;
+            const int64_t qu = i3(nx, nyg, i, j, k), qv = i3(nxg, ny, i, j, k), qw = i3(nxg, nyg, i, j, k);
+            dv = ((T)U[qu] - (T)U[qu - 1]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)m.rdyf[j] +
+                 ((T)W[qw] - (T)W[qw - (int64_t)nxg * nyg]) * (T)m.rdzf[k];
         }
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
@@ -203,19 +224,21 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     lds_fft<WG>(lds, ls, rb, pM, tw, -1);
     const int mh = M + 1;
     // consecutive threads take consecutive rows (kz) of one kx: runs of rb
-    for (int e = threadIdx.x; e < rb * mh; e += WG) {
-        const int kx = e / rb, lr = e - kx * rb;
+    StepDiv ek(threadIdx.x, WG, rb);  // (kx = q, row lr = r)
+    for (int e = threadIdx.x; e < rb * mh; e += WG, ek.next()) {
+        const int kx = ek.q, lr = ek.r;
         const int r = row0 + lr;
         if (r >= nrows) continue;
         const Cx<T> *z = lds + lr * ls;
-        const Cx<T> a = z[fpad<T>(kx % M)];
-        const Cx<T> bc = z[fpad<T>((M - kx) % M)];
+        const Cx<T> a = z[fpad<T>(kx == M ? 0 : kx)];
+        const Cx<T> bc = z[fpad<T>(kx == 0 ? 0 : M - kx)];
         const Cx<T> b = {bc.x, -bc.y};  // conj(Z[M-k])
         const Cx<T> E = {(T)0.5 * (a.x + b.x), (T)0.5 * (a.y + b.y)};
         const Cx<T> dd = csub(a, b);
         const Cx<T> O = {(T)0.5 * dd.y, -(T)0.5 * dd.x};  // (a-b)/(2i)
         const Cx<T> X = cadd(E, cmul(twN[kx], O));
-        const int jp = r / d.nzp, kp = r - jp * d.nzp;
+        int jp, kp;
+        small_divmod(r, d.nzp, inz, jp, kp);
         spec[((int64_t)jp * mh + kx) * d.nzp + kp] = X;
     }
 }
@@ -481,16 +504,21 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const int row0 = blockIdx.x * rb;
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
     for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
+    const float inz = 1.0f / (float)d.nzp;
     // consecutive threads read consecutive rows (kz) of one kx: runs of rb
-    for (int e = threadIdx.x; e < rb * mh; e += WG) {
-        const int kx = e / rb, lr = e - kx * rb;
-        const int r = row0 + lr;
-        Cx<T> v = {(T)0, (T)0};
-        if (r < nrows) {
-            const int jp = r / d.nzp, kp = r - jp * d.nzp;
-            v = spec[((int64_t)jp * mh + kx) * d.nzp + kp];
+    {
+        StepDiv ek(threadIdx.x, WG, rb);  // (kx = q, row lr = r)
+        for (int e = threadIdx.x; e < rb * mh; e += WG, ek.next()) {
+            const int kx = ek.q, lr = ek.r;
+            const int r = row0 + lr;
+            Cx<T> v = {(T)0, (T)0};
+            if (r < nrows) {
+                int jp, kp;
+                small_divmod(r, d.nzp, inz, jp, kp);
+                v = spec[((int64_t)jp * mh + kx) * d.nzp + kp];
+            }
+            lds[lr * ls + fpad<T>(kx)] = v;
         }
-        lds[lr * ls + fpad<T>(kx)] = v;
     }
     __syncthreads();
     // packed length-M input of the complex inverse, in place: each thread
@@ -498,8 +526,9 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     // from the two spectrum values it read.
     //   Z(k) = (X(k) + conj X(M-k)) + i (X(k) - conj X(M-k)) exp(+2 pi i k/N)
     const int hp = M / 2 + 1;
-    for (int e = threadIdx.x; e < rb * hp; e += WG) {
-        const int lr = e / hp, k = e - lr * hp, k2 = M - k;
+    StepDiv ep(threadIdx.x, WG, hp);  // (row lr = q, k = r)
+    for (int e = threadIdx.x; e < rb * hp; e += WG, ep.next()) {
+        const int lr = ep.q, k = ep.r, k2 = M - k;
         Cx<T> *row = lds + lr * ls;
         const Cx<T> A = row[fpad<T>(k)], B = row[fpad<T>(k2)];
         auto z = [&](Cx<T> a, Cx<T> bc, int kk) {
@@ -517,11 +546,13 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     lds_fft<WG>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;
-    for (int e = threadIdx.x; e < rb * N; e += WG) {
-        const int lr = e / N, ip = e - lr * N;
+    StepDiv el(threadIdx.x, WG, N);  // (row lr = q, ip = r)
+    for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
+        const int lr = el.q, ip = el.r;
         const int r = row0 + lr;
         if (r >= nrows) continue;
-        const int jp = r / d.nzp, kp = r - jp * d.nzp;
+        int jp, kp;
+        small_divmod(r, d.nzp, inz, jp, kp);
         const double ph = (double)zr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] / norm;
         phi[((int64_t)kp * d.nyp + jp) * d.nxp + ip] = (P)ph;
         const int ipn = ip + 1 == N ? 0 : ip + 1;
