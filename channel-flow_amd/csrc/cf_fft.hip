@@ -546,6 +546,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     lds_fft<WG>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;
+    const double rnorm = 1.0 / norm;
     StepDiv el(threadIdx.x, WG, N);  // (row lr = q, ip = r)
     for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
         const int lr = el.q, ip = el.r;
@@ -553,10 +554,10 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         if (r >= nrows) continue;
         int jp, kp;
         small_divmod(r, d.nzp, inz, jp, kp);
-        const double ph = (double)zr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] / norm;
+        const double ph = (double)zr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] * rnorm;
         phi[((int64_t)kp * d.nyp + jp) * d.nxp + ip] = (P)ph;
         const int ipn = ip + 1 == N ? 0 : ip + 1;
-        const double phn = (double)zr[2 * (lr * ls + fpad<T>(ipn >> 1)) + (ipn & 1)] / norm;
+        const double phn = (double)zr[2 * (lr * ls + fpad<T>(ipn >> 1)) + (ipn & 1)] * rnorm;
         const int i = 2 + ip, j = 2 + jp, k = 2 + kp;
         const int64_t qu = i3(nx, nyg, i, j, k);
         U[qu] = (S)((double)U[qu] - (phn - ph) * m.rdxc[i]);
