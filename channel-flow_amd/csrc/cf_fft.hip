@@ -274,52 +274,108 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 }
 
 // ---------------------------------------------------------------------------
-// Tridiagonal solve in y, one lane per (kx,kz) mode: forward sweep keeping c'
-// in an FP64 scratch and d' in the spectrum, then back substitution.  Four
-// passes over the spectrum and two over the scratch (written and read back
-// while still in the last-level cache), but no serial phase: the faster form
-// when the spectrum is FP32 (its passes are half as large).
+// Tridiagonal solve in y, one lane per (kx,kz) mode (the forward sweep and
+// back substitution of projection.f90:90-113 in FP64 arithmetic).  Rows go
+// in blocks of LB: the next block's spectrum values are loaded while the
+// current block is eliminated, d' is stored in the spectrum, and c' is kept
+// only at the last row of each block (a checkpoint).  The back substitution
+// recomputes a block's c' from the previous block's checkpoint -- the same
+// operations on the same values, so the same c' -- and walks the block
+// backwards.  Two passes over the spectrum and 1/LB of a pass of FP64
+// scratch; no serial phase: the faster form when the spectrum is FP32.
 // ---------------------------------------------------------------------------
+constexpr int TH_LB = 16;
+size_t thomas_scratch_elems(const Dims &d) { return (size_t)((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp; }
+
+// pivot reciprocal 1 / (D(j) - dl(j-1) c'(j-1)); c'(j) = du(j) times it
+__device__ __forceinline__ double thomas_pivot(double D, double lo, double cprev) { return 1.0 / (D - lo * cprev); }
+
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ cscr, const double *__restrict__ kxx,
+k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const double *__restrict__ kxx,
               const double *__restrict__ kzz, const double *__restrict__ dd, const double *__restrict__ dl,
               const double *__restrict__ du) {
+    extern __shared__ double sco[];  // dd | dl | du
     const int nzp = d.nzp, nyp = d.nyp;
+    double *s_dd = sco, *s_dl = sco + nyp, *s_du = sco + 2 * nyp;
+    for (int e = threadIdx.x; e < nyp; e += blockDim.x) { s_dd[e] = dd[e]; s_dl[e] = dl[e]; s_du[e] = du[e]; }
+    __syncthreads();
     const int nmodes = d.mh * nzp;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nmodes) return;
     const int kx = q / nzp, kz = q - kx * nzp;
     const double kx2 = kxx[kx], kz2 = kzz[kz];
+    const bool zm = kx == 0 && kz == 0;
     const int64_t js = nmodes;
     Cx<T> *col = spec + q;
-    double D0 = (dd[0] + kx2) + kz2;
-    if (kx == 0 && kz == 0) D0 = 1.5 * D0;
-    double r = 1.0 / D0;
-    double cp = du[0] * r;
-    const Cx<T> b0 = col[0];
-    double dpx = (double)b0.x * r, dpy = (double)b0.y * r;
-    col[0] = Cx<T>{(T)dpx, (T)dpy};
-    cscr[q] = cp;
-#pragma unroll 4
-    for (int jj = 1; jj < nyp; jj++) {
-        const double a = dl[jj - 1];
-        r = 1.0 / (((dd[jj] + kx2) + kz2) - a * cp);
-        const Cx<T> b = col[jj * js];
-        dpx = ((double)b.x - a * dpx) * r;
-        dpy = ((double)b.y - a * dpy) * r;
-        col[jj * js] = Cx<T>{(T)dpx, (T)dpy};
-        cp = (jj < nyp - 1) ? du[jj] * r : 0.0;
-        cscr[(int64_t)jj * nmodes + q] = cp;
+    const int nb = (nyp + TH_LB - 1) / TH_LB;
+    auto Dj = [&](int j) {
+        const double D = (s_dd[j] + kx2) + kz2;
+        return (j == 0 && zm) ? 1.5 * D : D;
+    };
+    Cx<T> cur[TH_LB], nxt[TH_LB];
+    auto loadblk = [&](int b, Cx<T> *dst) {
+#pragma unroll
+        for (int u = 0; u < TH_LB; u++) {
+            const int j = b * TH_LB + u;
+            dst[u] = j < nyp ? col[j * js] : Cx<T>{(T)0, (T)0};
+        }
+    };
+    // forward sweep
+    double cp = 0.0, dpx = 0.0, dpy = 0.0;
+    loadblk(0, cur);
+    for (int b = 0; b < nb; b++) {
+        if (b + 1 < nb) loadblk(b + 1, nxt);
+#pragma unroll
+        for (int u = 0; u < TH_LB; u++) {
+            const int j = b * TH_LB + u;
+            if (j < nyp) {
+                const double lo = j > 0 ? s_dl[j - 1] : 0.0;
+                const double r = thomas_pivot(Dj(j), lo, cp);
+                dpx = ((double)cur[u].x - lo * dpx) * r;
+                dpy = ((double)cur[u].y - lo * dpy) * r;
+                col[j * js] = Cx<T>{(T)dpx, (T)dpy};
+                cp = (j < nyp - 1) ? s_du[j] * r : 0.0;
+            }
+        }
+        ckpt[(int64_t)b * nmodes + q] = cp;
+#pragma unroll
+        for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
     }
-    double xr = dpx, xi = dpy;
-#pragma unroll 4
-    for (int jj = nyp - 2; jj >= 0; jj--) {
-        const double c = cscr[(int64_t)jj * nmodes + q];
-        const Cx<T> dv = col[jj * js];
-        xr = (double)dv.x - c * xr;
-        xi = (double)dv.y - c * xi;
-        col[jj * js] = Cx<T>{(T)xr, (T)xi};
+    // back substitution, block by block from the top
+    double xr = dpx, xi = dpy;  // row nyp-1
+    loadblk(nb - 1, cur);
+    double cnext = nb >= 2 ? ckpt[(int64_t)(nb - 2) * nmodes + q] : 0.0;
+    for (int b = nb - 1; b >= 0; b--) {
+        const double cstart = cnext;  // c' at the row before the block
+        if (b > 0) {
+            loadblk(b - 1, nxt);
+            cnext = b >= 2 ? ckpt[(int64_t)(b - 2) * nmodes + q] : 0.0;
+        }
+        double c[TH_LB];
+        double cprev = cstart;
+#pragma unroll
+        for (int u = 0; u < TH_LB; u++) {
+            const int j = b * TH_LB + u;
+            c[u] = 0.0;
+            if (j < nyp) {
+                const double lo = j > 0 ? s_dl[j - 1] : 0.0;
+                const double r = thomas_pivot(Dj(j), lo, cprev);
+                c[u] = (j < nyp - 1) ? s_du[j] * r : 0.0;
+                cprev = c[u];
+            }
+        }
+#pragma unroll
+        for (int u = TH_LB - 1; u >= 0; u--) {
+            const int j = b * TH_LB + u;
+            if (j < nyp - 1) {
+                xr = (double)cur[u].x - c[u] * xr;
+                xi = (double)cur[u].y - c[u] * xi;
+                col[j * js] = Cx<T>{(T)xr, (T)xi};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
     }
 }
 
@@ -657,8 +713,8 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
                    const double *dd, const double *dl, const double *du, hipStream_t st) {
     const int nmodes = d.mh * d.nzp;
     if (cscr && thomas_uses_scratch((int)sizeof(Cx<T>), nmodes)) {
-        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((nmodes + 255) / 256), dim3(256), 0, st, d, spec, cscr, kxx,
-                           kzz, dd, dl, du);
+        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((nmodes + 255) / 256), dim3(256), 3 * d.nyp * sizeof(double), st,
+                           d, spec, cscr, kxx, kzz, dd, dl, du);
         return;
     }
     int KZ, P;

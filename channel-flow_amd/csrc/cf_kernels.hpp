@@ -68,6 +68,7 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
 // the y solve form for a spectrum element of esize bytes needs an FP64
 // scratch of one double per spectrum element
 bool thomas_uses_scratch(int esize, int nmodes);
+size_t thomas_scratch_elems(const Dims &d);  // FP64 checkpoints of the lane-form y solve
 
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
