@@ -282,7 +282,7 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 // recomputes a block's c' from the previous block's checkpoint -- the same
 // operations on the same values, so the same c' -- and walks the block
 // backwards.  Two passes over the spectrum and 1/LB of a pass of FP64
-// scratch; no serial phase: the faster form when the spectrum is FP32.
+// scratch, no serial phase: the faster form whenever the modes fill the GPU.
 // ---------------------------------------------------------------------------
 constexpr int TH_LB = 16;
 size_t thomas_scratch_elems(const Dims &d) { return (size_t)((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp; }
@@ -703,9 +703,11 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
                        b, sign);
 }
 
-// lane form for FP32 spectra with enough modes to fill the GPU one lane each
+// lane form when there are enough modes to fill the GPU one lane each (both
+// precisions); the partitioned form below that
 bool thomas_uses_scratch(int esize, int nmodes) {
-    return env_int("CF_TH_LANE", (esize < 16 && nmodes >= 65536) ? 1 : 0) != 0;
+    (void)esize;
+    return env_int("CF_TH_LANE", nmodes >= 65536 ? 1 : 0) != 0;
 }
 
 template <typename T>
