@@ -302,8 +302,23 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     PM<R> pm;
     pm.set_ij(m, min(i, nx), min(j, ny));
     const R cnew = (R)(dt * a_new), cg = (R)(dt * a_g);
+    // substeps 2, 3 read G at the thread's own point: loaded one plane ahead
+    // so the load has a whole plane of arithmetic to arrive
+    const int64_t ou0 = (int64_t)(i - 1) + nx * (j - 1), ov0 = (int64_t)(i - 1) + nxg * (j - 1);
+    const bool vrow = j <= ny - 1;
+    R gu = 0, gv = 0, gw = 0;
+    auto gload = [&](int kk, R &xu, R &xv, R &xw) {
+        if (SUB != 1 && active && kk < kend) {
+            xu = (R)Gu[(int64_t)(kk - 1) * psu + ou0];
+            if (vrow) xv = (R)Gv[(int64_t)(kk - 1) * psv + ov0];
+            xw = (R)Gw[(int64_t)(kk - 1) * psw + ov0];
+        }
+    };
+    gload(kbeg, gu, gv, gw);
     for (int k = kbeg; k < kend; k++) {
         const int r0 = k - kbeg;
+        R gu_n = 0, gv_n = 0, gw_n = 0;
+        gload(k + 1, gu_n, gv_n, gw_n);
         if (active) {
             a.slot[0] = r0 & 3; a.slot[1] = (r0 + 1) & 3; a.slot[2] = (r0 + 2) & 3;
             R ru = 0, rv = 0, rw = 0;
@@ -324,11 +339,12 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
                 Wo[qw] = (S)(cw + cnew * rw);
                 Gw[qw] = (S)(cw + cg * rw);
             } else {
-                Uo[qu] = (S)((R)Gu[qu] + cnew * ru);
-                if (j <= ny - 1) Vo[qv] = (S)((R)Gv[qv] + cnew * rv);
-                Wo[qw] = (S)((R)Gw[qw] + cnew * rw);
+                Uo[qu] = (S)(gu + cnew * ru);
+                if (j <= ny - 1) Vo[qv] = (S)(gv + cnew * rv);
+                Wo[qw] = (S)(gw + cnew * rw);
             }
         }
+        gu = gu_n; gv = gv_n; gw = gw_n;
         // plane k+2 into the slot of plane k-2, then prefetch plane k+3
         if (k + 2 <= kend) {
             store((r0 + 3) & 3);
