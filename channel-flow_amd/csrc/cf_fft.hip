@@ -117,22 +117,34 @@ struct StepDiv {
 
 // One Stockham pass over `nlines` lines of length n held in LDS (line stride
 // ls).  MAXIT bounds the butterflies per thread: callers guarantee
-// nlines*n <= 8*WG.
-template <int R, int WG, typename T>
-__device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, int Ns,
-                                         const Cx<T> *__restrict__ tw, int sign) {
+// nlines*n <= 8*WG.  NC > 0 fixes n = NC, Ns = NSC and sign = SIGNC at
+// compile time, so that the index arithmetic divides by constants.
+template <int R, int WG, typename T, int NC = 0, int NSC = 1, int SIGNC = -1>
+__device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n_, int Ns_,
+                                         const Cx<T> *__restrict__ tw, int sign_) {
     constexpr int MAXIT = (8 + R - 1) / R;
+    const int n = NC ? NC : n_, Ns = NC ? NSC : Ns_, sign = NC ? SIGNC : sign_;
     const int nb = n / R, items = nb * nlines;
     const int stw = n / (Ns * R);
     const float inb = 1.0f / (float)nb, ins = 1.0f / (float)Ns;
+    auto split = [&](int it, int &line, int &jj, int &q, int &k) {
+        if constexpr (NC > 0) {
+            line = it / (NC / R);
+            jj = it - line * (NC / R);
+            q = jj / NSC;
+            k = jj - q * NSC;
+        } else {
+            small_divmod(it, nb, inb, line, jj);
+            small_divmod(jj, Ns, ins, q, k);
+        }
+    };
     Cx<T> v[MAXIT][R];
 #pragma unroll
     for (int c = 0; c < MAXIT; c++) {
         const int it = threadIdx.x + c * WG;
         if (it < items) {
             int line, jj, q, k;
-            small_divmod(it, nb, inb, line, jj);
-            small_divmod(jj, Ns, ins, q, k);
+            split(it, line, jj, q, k);
             const Cx<T> *src = buf + line * ls;
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -159,8 +171,7 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n, 
         const int it = threadIdx.x + c * WG;
         if (it < items) {
             int line, jj, q, k;
-            small_divmod(it, nb, inb, line, jj);
-            small_divmod(jj, Ns, ins, q, k);
+            split(it, line, jj, q, k);
             const int base = q * Ns * R + k;
             Cx<T> *dst = buf + line * ls;
 #pragma unroll
@@ -184,20 +195,47 @@ __device__ void lds_fft(Cx<T> *buf, int ls, int nlines, const FFTPlan &p, const 
     }
 }
 
+// The plan of a length-n transform (the host's make_plan): radix 8 first
+// when r8, then 4, 2 and 3.
+struct CPlan {
+    int nr;
+    int radix[16];
+};
+__host__ __device__ constexpr CPlan cplan(int n, bool r8) {
+    CPlan p{0, {}};
+    int m = n;
+    while (r8 && m % 8 == 0 && p.nr < 16) { p.radix[p.nr++] = 8; m /= 8; }
+    while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
+    while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
+    while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
+    return p;
+}
+
+// lds_fft for a length N and sign known at compile time
+template <int N, bool R8, int WG, int SIGN, int PASS = 0, int NS = 1, typename T>
+__device__ __forceinline__ void lds_fft_c(Cx<T> *buf, int ls, int nlines, const Cx<T> *tw) {
+    constexpr CPlan p = cplan(N, R8);
+    if constexpr (PASS < p.nr) {
+        constexpr int R = p.radix[PASS];
+        fft_pass<R, WG, T, N, NS, SIGN>(buf, ls, nlines, N, NS, tw, SIGN);
+        lds_fft_c<N, R8, WG, SIGN, PASS + 1, NS * R>(buf, ls, nlines, tw);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // div(u*) fused with the forward real FFT along x.
 // compute_pseudo_pressure_rhs (projection.f90:45-53) for i = 2..nxp+1 then
 // the x part of the 2-D DFT (projection.f90:84-86), as a length-M complex FFT
 // of z[m] = div[2m] + i div[2m+1] (M = nxp/2) and the standard split.
 // ---------------------------------------------------------------------------
-template <typename S, typename T, int WG>
+template <typename S, typename T, int WG, int MC>
 __global__ void __launch_bounds__(WG)
 k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const S *__restrict__ W, Cx<T> *__restrict__ spec, FFTPlan pM,
            const Cx<T> *__restrict__ twM, const Cx<T> *__restrict__ twN, int rb) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int M = pM.n, N = 2 * M, ls = fft_line<T>(M);
+    const int M = MC ? MC : pM.n, N = 2 * M, ls = fft_line<T>(M);
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
@@ -221,7 +259,8 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
     __syncthreads();
-    lds_fft<WG>(lds, ls, rb, pM, tw, -1);
+    if constexpr (MC > 0) lds_fft_c<MC, false, WG, -1>(lds, ls, rb, tw);
+    else lds_fft<WG>(lds, ls, rb, pM, tw, -1);
     const int mh = M + 1;
     // consecutive threads take consecutive rows (kz) of one kx: runs of rb
     StepDiv ek(threadIdx.x, WG, rb);  // (kx = q, row lr = r)
@@ -248,13 +287,13 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
 // the contiguous run spec[c * nzp .. c * nzp + nzp), and a workgroup's batch
 // of consecutive columns is one contiguous block of memory.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int NC, int SIGNC>
 __global__ void __launch_bounds__(512)
 k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ twZ, int bcols,
        int sign) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int n = pZ.n, ls = fft_line<T>(n) + 1;
+    const int n = NC ? NC : pZ.n, ls = fft_line<T>(n) + 1;
     const int64_t ncols = (int64_t)d.nyp * d.mh;
     const int64_t c0 = (int64_t)blockIdx.x * bcols;
     const int nb = (int)min((int64_t)bcols, ncols - c0);
@@ -266,7 +305,8 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
         lds[b * ls + fpad<T>(kz)] = base[e];
     }
     __syncthreads();
-    lds_fft<512>(lds, ls, nb, pZ, tw, sign);
+    if constexpr (NC > 0) lds_fft_c<NC, true, 512, SIGNC>(lds, ls, nb, tw);
+    else lds_fft<512>(lds, ls, nb, pZ, tw, sign);
     for (int e = threadIdx.x; e < nb * n; e += 512) {
         const int b = e / n, kz = e - b * n;
         base[e] = lds[b * ls + fpad<T>(kz)];
@@ -548,14 +588,14 @@ k_thomas(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ kxx, const
 // nxp*nzp, real part), writes phi and projects U on the same row
 // (projection.f90:208-211), phi(nx) being the periodic copy of phi(2).
 // ---------------------------------------------------------------------------
-template <typename S, typename T, typename P, int WG>
+template <typename S, typename T, typename P, int WG, int MC>
 __global__ void __launch_bounds__(WG)
 k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__ phi,
               S *__restrict__ U, FFTPlan pM, const Cx<T> *__restrict__ twM,
               const Cx<T> *__restrict__ twN, int rb, double norm) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
-    const int M = pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
+    const int M = MC ? MC : pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
     const int nrows = d.nyp * d.nzp;
     const int row0 = blockIdx.x * rb;
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
@@ -599,7 +639,8 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         if (k != 0 && k2 != k) row[fpad<T>(k2)] = z(B, A, k2);
     }
     __syncthreads();
-    lds_fft<WG>(lds, ls, rb, pM, tw, +1);
+    if constexpr (MC > 0) lds_fft_c<MC, false, WG, 1>(lds, ls, rb, tw);
+    else lds_fft<WG>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;
     const double rnorm = 1.0 / norm;
@@ -630,6 +671,14 @@ static int env_int(const char *name, int dflt) {
     const char *v = getenv(name);
     return v ? atoi(v) : dflt;
 }
+
+// Transform lengths compiled with their plan fixed (constant index
+// arithmetic): half lengths M = nxp/2 of the x transforms and z lengths nzp
+// of the grids 64..384; other lengths use the run-time plan.  env
+// CF_FFT_STATIC=0 selects the run-time plan for every length.
+#define CF_X_LENGTHS(F) F(32) F(48) F(64) F(96) F(128) F(192)
+#define CF_Z_LENGTHS(F) F(64) F(96) F(128) F(192) F(256) F(384)
+static bool fft_static_sizes() { return env_int("CF_FFT_STATIC", 1) != 0; }
 
 // threads per workgroup of the x transforms (env CF_FFT_XWG: 256 or 512)
 static int fft_x_wg(int esize) {
@@ -685,12 +734,24 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>);
-    if (fft_x_wg((int)sizeof(Cx<T>)) == 512)
-        hipLaunchKernelGGL((k_div_fftx<S, T, 512>), dim3((nrows + rb - 1) / rb), dim3(512), shm, st, d, m, U, V,
-                           W, spec, pM, twM, twN, rb);
-    else
-        hipLaunchKernelGGL((k_div_fftx<S, T, 256>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m, U, V,
-                           W, spec, pM, twM, twN, rb);
+    const dim3 grid((nrows + rb - 1) / rb);
+    if (fft_x_wg((int)sizeof(Cx<T>)) == 512) {
+        hipLaunchKernelGGL((k_div_fftx<S, T, 512, 0>), grid, dim3(512), shm, st, d, m, U, V, W, spec, pM, twM, twN,
+                           rb);
+        return;
+    }
+#define CF_DIVX(MC)                                                                                        \
+    case MC:                                                                                               \
+        hipLaunchKernelGGL((k_div_fftx<S, T, 256, MC>), grid, dim3(256), shm, st, d, m, U, V, W, spec, pM, \
+                           twM, twN, rb);                                                                  \
+        return;
+    switch (fft_static_sizes() ? pM.n : 0) {
+        CF_X_LENGTHS(CF_DIVX)
+        default:
+            hipLaunchKernelGGL((k_div_fftx<S, T, 256, 0>), grid, dim3(256), shm, st, d, m, U, V, W, spec, pM, twM,
+                               twN, rb);
+    }
+#undef CF_DIVX
 }
 
 template <typename T>
@@ -699,8 +760,20 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
     const int b = fft_cols_per_wg(pZ.n, (int)sizeof(Cx<T>));
     const int64_t ncols = (int64_t)d.nyp * d.mh;
     const size_t shm = ((size_t)b * (fft_line<T>(pZ.n) + 1) + pZ.n) * sizeof(Cx<T>);
-    hipLaunchKernelGGL((k_fftz<T>), dim3((unsigned)((ncols + b - 1) / b)), dim3(512), shm, st, d, spec, pZ, twZ,
-                       b, sign);
+    const dim3 grid((unsigned)((ncols + b - 1) / b));
+#define CF_FFTZ(NC)                                                                                           \
+    case NC:                                                                                                  \
+        if (sign < 0)                                                                                         \
+            hipLaunchKernelGGL((k_fftz<T, NC, -1>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign);     \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_fftz<T, NC, 1>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign);      \
+        return;
+    switch (fft_static_sizes() ? pZ.n : 0) {
+        CF_Z_LENGTHS(CF_FFTZ)
+        default:
+            hipLaunchKernelGGL((k_fftz<T, 0, 0>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign);
+    }
+#undef CF_FFTZ
 }
 
 // lane form when there are enough modes to fill the GPU one lane each (both
@@ -748,12 +821,24 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>);
-    if (fft_x_wg((int)sizeof(Cx<T>)) == 512)
-        hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512>), dim3((nrows + rb - 1) / rb), dim3(512), shm, st, d, m,
-                           spec, phi, U, pM, twM, twN, rb, norm);
-    else
-        hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256>), dim3((nrows + rb - 1) / rb), dim3(256), shm, st, d, m,
-                           spec, phi, U, pM, twM, twN, rb, norm);
+    const dim3 grid((nrows + rb - 1) / rb);
+    if (fft_x_wg((int)sizeof(Cx<T>)) == 512) {
+        hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512, 0>), grid, dim3(512), shm, st, d, m, spec, phi, U, pM, twM,
+                           twN, rb, norm);
+        return;
+    }
+#define CF_IFFTX(MC)                                                                                          \
+    case MC:                                                                                                  \
+        hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, MC>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM, \
+                           twM, twN, rb, norm);                                                               \
+        return;
+    switch (fft_static_sizes() ? pM.n : 0) {
+        CF_X_LENGTHS(CF_IFFTX)
+        default:
+            hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, 0>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM,
+                               twM, twN, rb, norm);
+    }
+#undef CF_IFFTX
 }
 
 template void launch_div_fftx<double, double>(const Dims &, const Metrics &, const double *,
