@@ -244,6 +244,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
     const float inz = 1.0f / (float)d.nzp;
     StepDiv el(threadIdx.x, WG, N);  // element (row lr = q, ip = r)
+#pragma unroll 4
     for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
         const int lr = el.q, ip = el.r;
         const int r = row0 + lr;
@@ -603,8 +604,28 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const float inz = 1.0f / (float)d.nzp;
     // consecutive threads read consecutive rows (kz) of one kx: runs of rb
     {
+        // all loads of the thread first (rb * mh <= 8 WG + rb, so 9 cover
+        // WG >= 256), then the LDS writes; a plain loop takes any remainder
+        constexpr int MAXL = 9;
+        T vr[MAXL], vi[MAXL];
         StepDiv ek(threadIdx.x, WG, rb);  // (kx = q, row lr = r)
-        for (int e = threadIdx.x; e < rb * mh; e += WG, ek.next()) {
+#pragma unroll
+        for (int c = 0; c < MAXL; c++, ek.next()) {
+            vr[c] = vi[c] = (T)0;
+            const int r = row0 + ek.r;
+            if (threadIdx.x + c * WG < rb * mh && r < nrows) {
+                int jp, kp;
+                small_divmod(r, d.nzp, inz, jp, kp);
+                const Cx<T> v = spec[((int64_t)jp * mh + ek.q) * d.nzp + kp];
+                vr[c] = v.x;
+                vi[c] = v.y;
+            }
+        }
+        StepDiv es(threadIdx.x, WG, rb);
+#pragma unroll
+        for (int c = 0; c < MAXL; c++, es.next())
+            if (threadIdx.x + c * WG < rb * mh) lds[es.r * ls + fpad<T>(es.q)] = Cx<T>{vr[c], vi[c]};
+        for (int e = threadIdx.x + MAXL * WG; e < rb * mh; e += WG, ek.next()) {
             const int kx = ek.q, lr = ek.r;
             const int r = row0 + lr;
             Cx<T> v = {(T)0, (T)0};
@@ -645,6 +666,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const int nx = d.nx, nyg = d.nyg;
     const double rnorm = 1.0 / norm;
     StepDiv el(threadIdx.x, WG, N);  // (row lr = q, ip = r)
+#pragma unroll 4
     for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
         const int lr = el.q, ip = el.r;
         const int r = row0 + lr;
