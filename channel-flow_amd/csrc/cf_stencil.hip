@@ -408,7 +408,7 @@ __device__ __forceinline__ int img_ctr(int i, int ng) {  // centres 1..ng, uniqu
 template <typename S>
 __device__ __forceinline__ void ghost_one(const Dims &d, S *__restrict__ F, int field, int wall_mode,
                                           const double *__restrict__ alpha, double dyg_bot, double dyg_top,
-                                          int64_t g) {
+                                          int g) {
     int n1, n2, n3;
     bool xf, zf, yf;
     if (field == 0) { n1 = d.nx; n2 = d.nyg; n3 = d.nzg; xf = true; yf = false; zf = false; }
@@ -417,24 +417,26 @@ __device__ __forceinline__ void ghost_one(const Dims &d, S *__restrict__ F, int 
     else { n1 = d.nxg; n2 = d.nyg; n3 = d.nzg; xf = false; yf = false; zf = false; }
     const int nxgh = xf ? 2 : 3;              // ghost columns in x
     const int nzgh = zf ? 2 : 3;              // ghost planes in z
-    const int64_t nY = (int64_t)n1 * 2 * n3;  // wall rows
-    const int64_t nZ = (int64_t)n1 * (n2 - 2) * nzgh;
-    const int64_t nX = (int64_t)nxgh * (n2 - 2) * (n3 - nzgh);
+    // 32-bit index arithmetic: a field has far fewer than 2^31 ghosts
+    const int nY = n1 * 2 * n3;  // wall rows
+    const int nZ = n1 * (n2 - 2) * nzgh;
+    const int nX = nxgh * (n2 - 2) * (n3 - nzgh);
     int i, j, k;
     if (g < nY) {
-        i = 1 + (int)(g % n1);
-        int w = (int)((g / n1) % 2);
-        k = 1 + (int)(g / (2 * (int64_t)n1));
-        j = w == 0 ? 1 : n2;
+        const int q = g / n1;
+        i = 1 + (g - q * n1);
+        k = 1 + (q >> 1);
+        j = (q & 1) == 0 ? 1 : n2;
     } else if ((g -= nY) < nZ) {
-        i = 1 + (int)(g % n1);
-        j = 2 + (int)((g / n1) % (n2 - 2));
-        int z = (int)(g / ((int64_t)n1 * (n2 - 2)));
+        const int q = g / n1, z = q / (n2 - 2);
+        i = 1 + (g - q * n1);
+        j = 2 + (q - z * (n2 - 2));
         k = zf ? (z == 0 ? 1 : n3) : (z == 0 ? 1 : n3 - 2 + z);
     } else if ((g -= nZ) < nX) {
-        int c = (int)(g % nxgh);
-        j = 2 + (int)((g / nxgh) % (n2 - 2));
-        k = 2 + (int)(g / ((int64_t)nxgh * (n2 - 2)));
+        const int q = g / nxgh, kq = q / (n2 - 2);
+        const int c = g - q * nxgh;
+        j = 2 + (q - kq * (n2 - 2));
+        k = 2 + kq;
         i = xf ? (c == 0 ? 1 : n1) : (c == 0 ? 1 : n1 - 2 + c);
     } else {
         return;
@@ -487,7 +489,7 @@ __global__ void k_ghosts(Dims d, GhostSet gs, double dyg_bot, double dyg_top) {
     int64_t beg = 0;
     for (int f = 0; f < gs.n; f++) {
         if (g < gs.end[f]) {
-            ghost_one<S>(d, (S *)gs.F[f], gs.field[f], gs.mode[f], gs.alpha[f], dyg_bot, dyg_top, g - beg);
+            ghost_one<S>(d, (S *)gs.F[f], gs.field[f], gs.mode[f], gs.alpha[f], dyg_bot, dyg_top, (int)(g - beg));
             return;
         }
         beg = gs.end[f];
