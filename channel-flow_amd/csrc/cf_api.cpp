@@ -298,7 +298,7 @@ int Ctx<S>::init() {
     wt.jjz = m_jjz; wt.wz = m_wz; wt.jjx = m_jjx; wt.wx = m_wx;
     // FFT plans and twiddles: x uses a length-M complex FFT plus the N-point split
     std::string err;
-    pM = make_plan(M, err, false);
+    pM = make_plan(M, err, FFT_X_RADIX8);
     pZ = make_plan(d.nzp, err, true);
     if (!err.empty()) { set_error(err); return -1; }
     auto twid = [](int n) {

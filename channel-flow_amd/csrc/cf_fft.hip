@@ -260,7 +260,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
     __syncthreads();
-    if constexpr (MC > 0) lds_fft_c<MC, false, WG, -1>(lds, ls, rb, tw);
+    if constexpr (MC > 0) lds_fft_c<MC, FFT_X_RADIX8, WG, -1>(lds, ls, rb, tw);
     else lds_fft<WG>(lds, ls, rb, pM, tw, -1);
     const int mh = M + 1;
     // consecutive threads take consecutive rows (kz) of one kx: runs of rb
@@ -660,7 +660,7 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         if (k != 0 && k2 != k) row[fpad<T>(k2)] = z(B, A, k2);
     }
     __syncthreads();
-    if constexpr (MC > 0) lds_fft_c<MC, false, WG, 1>(lds, ls, rb, tw);
+    if constexpr (MC > 0) lds_fft_c<MC, FFT_X_RADIX8, WG, 1>(lds, ls, rb, tw);
     else lds_fft<WG>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;

@@ -11,6 +11,8 @@ struct FFTPlan {
     int nr;
     int radix[16];
 };
+// radix 8 in the plan of the x transforms (the z transforms always use it)
+constexpr bool FFT_X_RADIX8 = true;
 
 // ---- stencils (cf_stencil.hip) ----
 template <typename S, typename R>
