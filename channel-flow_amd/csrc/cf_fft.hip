@@ -72,13 +72,16 @@ __device__ __forceinline__ void dft3(Cx<T> *a, int sign) {
     a[2] = csub(m, r);
 }
 
-// LDS lines are padded with one element after every 2^5 so that the
-// strided writes of the early Stockham passes (stride R) and the transposed
-// accesses spread over the LDS banks; fpad maps a line position to its slot.
-// (FP64 complex only: with 8-byte FP32 complex elements the unpadded
-// layout measured faster)
+// LDS lines are padded with one element after every 8 (FP64 complex) or 16
+// (FP32 complex) so that the strided writes of the early Stockham passes
+// (stride R) and the transposed accesses spread over the LDS banks; fpad
+// maps a line position to its slot.  (Measured against 1 per 32 and none:
+// the z transform ran 0.158 -> 0.150 ms in FP64 and 0.094 -> 0.072 ms in
+// FP32 at 384 points.)
 template <typename T>
-__host__ __device__ __forceinline__ int fpad(int e) { return sizeof(T) == 8 ? e + (e >> 5) : e; }
+__host__ __device__ __forceinline__ int fpad(int e) {
+    return sizeof(T) == 8 ? e + (e >> 3) : e + (e >> 4);
+}
 // slots of a padded line of n elements
 template <typename T>
 __host__ __device__ __forceinline__ int fft_line(int n) { return fpad<T>(n - 1) + 1; }
