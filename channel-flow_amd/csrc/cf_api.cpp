@@ -491,7 +491,9 @@ void Ctx<S>::step_launch() {
                             G[2], sc, ubar, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
                             s == 1 ? 1 : 2, stream);
         toc();
-        ghosts(Fn, 7);
+        // the divergence reads the x and z ghost faces at their periodic
+        // images, so the boundary conditions are applied once, after the
+        // projection (time_integration.f90 applies them before and after)
         poisson(Fn, s == 1 ? phi1 : phi23);
         ghosts(Fn, 7);
         cur = 1 - cur;

@@ -230,6 +230,9 @@ __device__ __forceinline__ void lds_fft_c(Cx<T> *buf, int ls, int nlines, const 
 // compute_pseudo_pressure_rhs (projection.f90:45-53) for i = 2..nxp+1 then
 // the x part of the 2-D DFT (projection.f90:84-86), as a length-M complex FFT
 // of z[m] = div[2m] + i div[2m+1] (M = nxp/2) and the standard split.
+// U(1) and W(k=1) are read at their periodic images U(nx-1), W(nz-1) (the
+// values apply_boundary_conditions gives them); V(j=1) is the wall row,
+// which only the boundary kernel writes (zero).
 // ---------------------------------------------------------------------------
 template <typename S, typename T, int WG, int MC>
 __global__ void __launch_bounds__(WG)
@@ -257,8 +260,12 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
             small_divmod(r, d.nzp, inz, jr, kr);
             const int j = 2 + jr, k = 2 + kr, i = 2 + ip;
             const int64_t qu = i3(nx, nyg, i, j, k), qv = i3(nxg, ny, i, j, k), qw = i3(nxg, nyg, i, j, k);
-            dv = ((T)U[qu] - (T)U[qu - 1]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)m.rdyf[j] +
-                 ((T)W[qw] - (T)W[qw - (int64_t)nxg * nyg]) * (T)m.rdzf[k];
+            // the x and z ghost faces are read at their periodic images, so the
+            // fields need no ghost fill between the RK update and this kernel
+            const int64_t qum = i == 2 ? qu + (nx - 3) : qu - 1;  // U(1) = U(nx-1)
+            const int64_t qwm = k == 2 ? qw + (int64_t)(d.nz - 3) * nxg * nyg : qw - (int64_t)nxg * nyg;  // W(:,:,1) = W(:,:,nz-1)
+            dv = ((T)U[qu] - (T)U[qum]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)m.rdyf[j] +
+                 ((T)W[qw] - (T)W[qwm]) * (T)m.rdzf[k];
         }
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
