@@ -703,6 +703,162 @@ void orc_projection_step(orc_t *o, int rk_step) {
                 w(i, j, k) = w(i, j, k) - (rp[IP(i, j, k + 1)] - rp[IP(i, j, k)]) / (ZG(k + 1) - ZG(k));
 }
 
+/* ======================================================================
+ * pressure.f90:21-341, compute_pressure: the actual pressure from
+ *   Laplacian(P) = div(rhs_u, rhs_v, rhs_w),
+ *   dP/dy = d/dy(nu dV/dy) at the walls (bc_1 bottom, bc_2 top).
+ * Optional outputs: rhs_p (2:nxg-1, 2:nyg-1, 2:nzg-1) and the two wall
+ * conditions (2:nxg-1, 2:nzg-1) each, for tests.  Ghosts the one-rank
+ * reference leaves unset (i = 1, nxg; k = 1, nzg) are copies of the
+ * neighbouring plane, as the pseudo-pressure save does (projection.f90:130-135).
+ * ====================================================================== */
+#define IB(i, k) I3(o->nxg - 2, 1, (i)-1, 1, (k)-1)
+void orc_compute_pressure(orc_t *o, double *Pout, double *rhs_p_out, double *bc1_out, double *bc2_out) {
+    const int nx = o->nx, ny = o->ny, nz = o->nz, nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
+    const double *x = o->x, *y = o->y, *z = o->z, *yg = o->yg;
+    const double *V = o->V, *avg = o->avg_nu_t;
+    const int nxp = o->nxp, nzp = o->nzp, nr = o->nr;
+    const size_t nb = (size_t)(nxg - 2) * (nzg - 2);
+    double *bc1 = dalloc(nb), *bc2 = dalloc(nb);
+    /* compute_pressure_BC (:256-341) */
+    {
+        const double dzeta = 1.0;
+        double g = (-3.0 * Yf(1) + 4.0 * Yf(2) - Yf(3)) / (2.0 * dzeta);
+        double g2 = (2.0 * Yf(1) - 5.0 * Yf(2) + 4.0 * Yf(3) - 1.0 * Yf(4)) / pow(dzeta, 2.0);
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) {
+                double nui = o->p.nu + 0.5 * (avg[0] + avg[1]);
+                double dnu = (avg[1] - avg[0]) / (YG(2) - YG(1));
+                double dV = (-3.0 * v(i, 1, k) + 4.0 * v(i, 2, k) - v(i, 3, k)) / (2.0 * dzeta) * 1.0 / g;
+                double ddV = (2.0 * v(i, 1, k) - 5.0 * v(i, 2, k) + 4.0 * v(i, 3, k) - 1.0 * v(i, 4, k)) / pow(dzeta, 2.0);
+                ddV = (ddV - dV * g2) / pow(g, 2.0);
+                bc1[IB(i, k)] = dnu * dV + nui * ddV;
+            }
+        g = (3.0 * Yf(ny) - 4.0 * Yf(ny - 1) + Yf(ny - 2)) / (2.0 * dzeta);
+        g2 = (2.0 * Yf(ny) - 5.0 * Yf(ny - 1) + 4.0 * Yf(ny - 2) - 1.0 * Yf(ny - 3)) / pow(dzeta, 2.0);
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) {
+                double nui = o->p.nu + 0.5 * (avg[nyg - 1] + avg[nyg - 2]);
+                double dnu = (avg[nyg - 1] - avg[nyg - 2]) / (YG(nyg) - YG(nyg - 1));
+                double dV = (3.0 * v(i, ny, k) - 4.0 * v(i, ny - 1, k) + v(i, ny - 2, k)) / (2.0 * dzeta) * 1.0 / g;
+                double ddV = (2.0 * v(i, ny, k) - 5.0 * v(i, ny - 1, k) + 4.0 * v(i, ny - 2, k) - 1.0 * v(i, ny - 3, k)) /
+                             pow(dzeta, 2.0);
+                ddV = (ddV - dV * g2) / pow(g, 2.0);
+                bc2[IB(i, k)] = dnu * dV + nui * ddV;
+            }
+    }
+    /* compute_pressure_rhs (:43-73): rhs_u, rhs_v, rhs_w on full arrays with
+       the x-periodic rhs_u, the wall rows of rhs_v set to the conditions and
+       the z-periodic rhs_w, then their divergence */
+    double *ruf = dalloc(o->nU), *rvf = dalloc(o->nV), *rwf = dalloc(o->nW);
+    {
+        double *ru = dalloc(o->nRu), *rv = dalloc(o->nRv), *rw = dalloc(o->nRw);
+        orc_compute_rhs(o, o->U, o->V, o->W, ru, rv, rw);
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nx - 1; i++) ruf[IU(i, j, k)] = ru[IRU(i, j, k)];
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= ny - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) rvf[IV(i, j, k)] = rv[IRV(i, j, k)];
+        for (int k = 2; k <= nz - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) rwf[IW(i, j, k)] = rw[IRW(i, j, k)];
+        free(ru); free(rv); free(rw);
+    }
+    periodic_x(o, ruf, nx, nyg, nzg, 1);
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int i = 2; i <= nxg - 1; i++) { rvf[IV(i, 1, k)] = bc1[IB(i, k)]; rvf[IV(i, ny, k)] = bc2[IB(i, k)]; }
+    periodic_z(rwf, nxg, nyg, nz, 1);
+    double *rp = o->rhs_p;
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++)
+                rp[IP(i, j, k)] = (ruf[IU(i, j, k)] - ruf[IU(i - 1, j, k)]) / (X(i) - X(i - 1)) +
+                                  (rvf[IV(i, j, k)] - rvf[IV(i, j - 1, k)]) / (Yf(j) - Yf(j - 1)) +
+                                  (rwf[IW(i, j, k)] - rwf[IW(i, j, k - 1)]) / (Z(k) - Z(k - 1));
+    free(ruf); free(rvf); free(rwf);
+    if (rhs_p_out)
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++)
+                    rhs_p_out[I3(nxg - 2, nyg - 2, i - 1, j - 1, k - 1)] = rp[IP(i, j, k)];
+    if (bc1_out) memcpy(bc1_out, bc1, nb * sizeof(double));
+    if (bc2_out) memcpy(bc2_out, bc2, nb * sizeof(double));
+    /* solve_poisson_equation_for_pressure (:94-151) */
+    const double cb1 = 1.0 / (Yf(2) - Yf(1)) / (YG(2) - YG(1));               /* coef_bc_1 (initialization.f90:364) */
+    const double cb2 = 1.0 / (Yf(nyg - 1) - Yf(nyg - 2)) / (YG(nyg) - YG(nyg - 1)); /* coef_bc_2 (:373) */
+    size_t npl = (size_t)nxp * nzp;
+    cplx *hat = (cplx *)malloc(npl * (nr + 2) * sizeof(cplx)); /* [j][kz][kx], then bc_1, bc_2 */
+    int mx = nxp > nzp ? nxp : nzp;
+#pragma omp parallel
+    {
+        cplx *w1 = (cplx *)malloc(mx * sizeof(cplx)), *w2 = (cplx *)malloc(mx * sizeof(cplx));
+#pragma omp for schedule(static)
+        for (int j = 2; j <= nyg + 1; j++) { /* the nr planes, then the two conditions */
+            cplx *pl = hat + (size_t)(j - 2) * npl;
+            for (int kk = 1; kk <= nzp; kk++)
+                for (int ii = 1; ii <= nxp; ii++) {
+                    double val = j <= nyg - 1 ? rp[IP(ii + 1, j, kk + 1)]
+                                              : (j == nyg ? bc1 : bc2)[IB(ii + 1, kk + 1)];
+                    pl[(size_t)(kk - 1) * nxp + (ii - 1)] = val;
+                }
+            fft2d_plane(nxp, nzp, pl, -1, w1, w2);
+        }
+        double *D = dalloc(nr), *DL = dalloc(nr), *DU = dalloc(nr), *br = dalloc(nr), *bi = dalloc(nr);
+#pragma omp for schedule(static)
+        for (int kz = 0; kz < nzp; kz++)
+            for (int kx = 0; kx < nxp; kx++) {
+                for (int q = 0; q < nr; q++) D[q] = o->dyy_d[q] + o->kxx[kx] + o->kzz[kz];
+                for (int q = 0; q < nr - 1; q++) { DL[q] = o->dyy_l[q]; DU[q] = o->dyy_u[q]; }
+                if (kx == 0 && kz == 0) D[0] = 3.0 / 2.0 * D[0];
+                for (int q = 0; q < nr; q++) {
+                    cplx h = hat[(size_t)q * npl + (size_t)kz * nxp + kx];
+                    br[q] = creal(h); bi[q] = cimag(h);
+                }
+                const cplx b1 = hat[(size_t)nr * npl + (size_t)kz * nxp + kx];
+                const cplx b2 = hat[(size_t)(nr + 1) * npl + (size_t)kz * nxp + kx];
+                const cplx r0 = (br[0] + I * bi[0]) + cb1 * (YG(2) - YG(1)) * b1;
+                const cplx rn = (br[nr - 1] + I * bi[nr - 1]) - cb2 * (YG(nyg) - YG(nyg - 1)) * b2;
+                br[0] = creal(r0); bi[0] = cimag(r0);
+                br[nr - 1] = creal(rn); bi[nr - 1] = cimag(rn);
+                orc_zgtsv(nr, DL, D, DU, br, bi);
+                for (int q = 0; q < nr; q++) hat[(size_t)q * npl + (size_t)kz * nxp + kx] = br[q] + I * bi[q];
+            }
+        free(D); free(DL); free(DU); free(br); free(bi);
+        const double norm = (double)(nxp * nzp);
+#pragma omp for schedule(static)
+        for (int j = 2; j <= nyg - 1; j++) {
+            cplx *pl = hat + (size_t)(j - 2) * npl;
+            fft2d_plane(nxp, nzp, pl, +1, w1, w2);
+            for (int kk = 1; kk <= nzp; kk++)
+                for (int ii = 1; ii <= nxp; ii++)
+                    Pout[IC(ii + 1, j, kk + 1)] = creal(pl[(size_t)(kk - 1) * nxp + (ii - 1)] / norm);
+        }
+        free(w1); free(w2);
+    }
+    free(hat);
+    /* ghost rows in y from the wall conditions (:155-158), then the periodic
+       copies (:177-193, intended one-rank semantics) */
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int i = 2; i <= nxg - 1; i++) {
+            Pout[IC(i, 1, k)] = Pout[IC(i, 2, k)] - (YG(2) - YG(1)) * bc1[IB(i, k)];
+            Pout[IC(i, nyg, k)] = Pout[IC(i, nyg - 1, k)] + (YG(nyg) - YG(nyg - 1)) * bc2[IB(i, k)];
+        }
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) Pout[IC(nxg - 1, j, k)] = Pout[IC(2, j, k)];
+    for (int j = 1; j <= nyg; j++)
+        for (int i = 2; i <= nxg - 1; i++) Pout[IC(i, j, nzg - 1)] = Pout[IC(i, j, 2)];
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) { Pout[IC(1, j, k)] = Pout[IC(2, j, k)]; Pout[IC(nxg, j, k)] = Pout[IC(nxg - 1, j, k)]; }
+    {
+        size_t pl = (size_t)nxg * nyg;
+        memcpy(Pout, Pout + pl, pl * sizeof(double));
+        memcpy(Pout + pl * (nzg - 1), Pout + pl * (nzg - 2), pl * sizeof(double));
+    }
+    free(bc1); free(bc2);
+}
+#undef IB
+
 /* projection.f90:234-256 (signed max, starting from 0) plus max |div| */
 void orc_check_divergence(const orc_t *o, double *max_signed, double *max_abs) {
     const int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;

@@ -78,6 +78,10 @@ double orc_mean_mass_flow_V(const orc_t *o, const double *V);
 void orc_eddy_viscosity(orc_t *o);
 void orc_wall_model(orc_t *o, int *n_nan, int *n_iter);
 void orc_check_divergence(const orc_t *o, double *max_signed, double *max_abs);
+/* pressure.f90:21-341 on the current state and avg_nu_t: P (nxg,nyg,nzg);
+   optional rhs_p (2:nxg-1,2:nyg-1,2:nzg-1) and wall conditions
+   bc_1, bc_2 (2:nxg-1,2:nzg-1) */
+void orc_compute_pressure(orc_t *o, double *P, double *rhs_p, double *bc_1, double *bc_2);
 /* out: Umean,Vmean,Wmean,U2mean,V2mean,W2mean,UVmean (ny each), then
    [utau, Retau, Qflow_x, Qflow_y] */
 void orc_statistics(const orc_t *o, double *out);

@@ -54,6 +54,7 @@ def lib():
             "orc_eddy_viscosity": (None, [v]),
             "orc_wall_model": (None, [v, _ip, _ip]),
             "orc_check_divergence": (None, [v, _dp, _dp]),
+            "orc_compute_pressure": (None, [v, _dp, _dp, _dp, _dp]),
             "orc_statistics": (None, [v, _dp]),
             "orc_eqwm": (None, [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                          ctypes.c_int, ctypes.c_double, _dp, _ip]),
@@ -201,6 +202,15 @@ class Oracle:
 
     def compute_eddy_viscosity(self):
         self.L.orc_eddy_viscosity(self.h)
+
+    def compute_pressure(self, details=False):
+        """pressure.f90 compute_pressure: P, and with details also rhs_p
+        (2:nxg-1,2:nyg-1,2:nzg-1) and the wall conditions (2:nxg-1,2:nzg-1)."""
+        Pa = _f(self.shape_c)
+        rp = _f((self.nxg - 2, self.nyg - 2, self.nzg - 2))
+        b1, b2 = _f((self.nxg - 2, self.nzg - 2)), _f((self.nxg - 2, self.nzg - 2))
+        self.L.orc_compute_pressure(self.h, P(Pa), P(rp), P(b1), P(b2))
+        return (Pa, rp, b1, b2) if details else Pa
 
     def compute_wall_model(self):
         a, b = ctypes.c_int(), ctypes.c_int()

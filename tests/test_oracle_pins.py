@@ -348,3 +348,60 @@ def test_wall_model_closed_form():
     # interior W points: average of the 4 surrounding U-point values
     exp = 0.25 * (azt[:-1, :-1] + azt[1:, :-1] + azt[:-1, 1:] + azt[1:, 1:])  # (nx-1, nzg-1) -> W (i, k)
     np.testing.assert_allclose(az[1:o.nx - 2, 0, 1:o.nz - 2], exp[1:o.nx - 2, 1:o.nz - 2], rtol=1e-12)
+
+
+# ------------------------------------------------------------- actual pressure
+def test_pressure_wall_condition_closed_form():
+    """pressure.f90 compute_pressure_BC on a uniform y grid: for
+    V = c (y - y1)(y - yN) the one-sided formulas are exact, so
+    d/dy(nu dV/dy) = 2 c nu at both walls (no eddy viscosity)."""
+    c = cfg_small(cells=(8, 12, 8), stretch=0.0)
+    o = Oracle(c)
+    U, V, W = (np.zeros(s, order="F") for s in (o.shape_u, o.shape_v, o.shape_w))
+    y = o.y
+    cc = 0.7
+    V[:] = (cc * (y - y[0]) * (y - y[-1]))[None, :, None]
+    o.set_state(U, V, W)
+    _, _, b1, b2 = o.compute_pressure(details=True)
+    np.testing.assert_allclose(b1, 2 * cc * c.nu, rtol=1e-9)
+    np.testing.assert_allclose(b2, 2 * cc * c.nu, rtol=1e-9)
+
+
+@pytest.mark.parametrize("les", [False, True])
+def test_pressure_solves_poisson_with_wall_conditions(les):
+    """The returned P satisfies the finite-volume Laplacian = rhs_p at every
+    interior cell (ghost rows from the wall conditions included), up to the
+    gauge row: the plane-mean residual is zero except at j = 2, where the
+    (0,0) mode's diagonal is scaled by 3/2 (residual -D(2)/2 <P(:,2,:)>)."""
+    c = cfg_small(cells=(16, 16, 12), les_model=1 if les else 0, dPdx=1e-3)
+    o = Oracle(c)
+    U, V, W = cases.initial_field(c)
+    o.initialise(U, V, W)
+    o.apply_boundary_conditions()
+    if les:
+        o.compute_eddy_viscosity()
+    Pa, rp, b1, b2 = o.compute_pressure(details=True)
+    g = o.grid()
+    x, y, z, yg = o.x, o.y, o.z, g["yg"]
+    dx, dz = x[1] - x[0], z[1] - z[0]
+    nxg, nyg, nzg = o.nxg, o.nyg, o.nzg
+    Pi = Pa[1:nxg - 2, :, 1:nzg - 2]            # unique cells i = 2..nxg-2, k = 2..nzg-2 (all j)
+    Lx = (np.roll(Pi, -1, 0) - 2 * Pi + np.roll(Pi, 1, 0)) / dx ** 2
+    Lz = (np.roll(Pi, -1, 2) - 2 * Pi + np.roll(Pi, 1, 2)) / dz ** 2
+    j = np.arange(1, nyg - 1)                   # 0-based rows of j = 2..nyg-1
+    dyf = (y[j] - y[j - 1])[None, :, None]
+    up = (Pi[:, j + 1, :] - Pi[:, j, :]) / (yg[j + 1] - yg[j])[None, :, None]
+    dn = (Pi[:, j, :] - Pi[:, j - 1, :]) / (yg[j] - yg[j - 1])[None, :, None]
+    L = Lx[:, j, :] + (up - dn) / dyf + Lz[:, j, :]
+    res = L - rp[:nxg - 3, :, :nzg - 3]
+    scale = np.abs(rp).max()
+    mean = res.mean(axis=(0, 2))
+    np.testing.assert_allclose(res - mean[None, :, None], 0.0, atol=1e-9 * scale)
+    np.testing.assert_allclose(mean[1:], 0.0, atol=1e-9 * scale)
+    b = -(1.0 / (y[1] - y[0])) * (1.0 / (yg[2] - yg[1]) + 1.0 / (yg[1] - yg[0]))
+    cdn = 1.0 / (y[1] - y[0]) / (yg[1] - yg[0])
+    assert abs(mean[0] - (-0.5 * (b + cdn) * Pi[:, 1, :].mean())) < 1e-9 * scale
+    # ghost rows carry the wall conditions, periodic copies the duplicates
+    np.testing.assert_allclose((Pa[1:-1, 1, 1:-1] - Pa[1:-1, 0, 1:-1]) / (yg[1] - yg[0]), b1, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(Pa[nxg - 2], Pa[1])
+    np.testing.assert_allclose(Pa[:, :, nzg - 2], Pa[:, :, 1])
