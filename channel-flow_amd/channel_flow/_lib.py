@@ -40,6 +40,7 @@ SIGNATURES = {
     "cf_set_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "cf_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "cf_get_pressure": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_compute_pressure": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_set_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_get_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_get_wall": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),

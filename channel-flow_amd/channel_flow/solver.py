@@ -109,6 +109,13 @@ class ChannelFlow:
         _lib.check(self.lib.cf_get_pressure(self.h, _lib.dptr(P)), "cf_get_pressure")
         return P
 
+    def compute_pressure(self):
+        """pressure.f90 compute_pressure: the actual pressure of the current
+        state (FP64, reference layout with ghosts)."""
+        P = _f(self.shape_c)
+        _lib.check(self.lib.cf_compute_pressure(self.h, _lib.dptr(P)), "cf_compute_pressure")
+        return P
+
     @property
     def nu_t(self):
         a = _f(self.shape_c)

@@ -729,6 +729,99 @@ int Ctx<S>::get_pressure(double *P) {
     return 0;
 }
 
+// compute_pressure (pressure.f90:21-341): the actual pressure of the current
+// state.  The right-hand sides of the momentum equations and their
+// divergence, the wall conditions dP/dy = d/dy(nu dV/dy) from one-sided
+// differences of V and the plane-averaged eddy viscosity, then the same
+// transforms and y solve as the projection, with the wall conditions as
+// sources of the first and last rows.  Ghosts: rows j = 1, nyg from the wall
+// conditions, duplicate planes by the periodic copies, and the ghost planes
+// the one-rank reference leaves unset (i = 1, nxg; k = 1, nzg) as copies of
+// their neighbours.  Output-only: temporaries are allocated per call.
+template <typename S>
+int Ctx<S>::compute_pressure(double *P) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    const int nx = d.nx, ny = d.ny, nz = d.nz, nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
+    const size_t nU = (size_t)nx * nyg * nzg, nV = (size_t)nxg * ny * nzg, nW = (size_t)nxg * nyg * nz;
+    const size_t nb = (size_t)(nxg - 2) * (nzg - 2), npl = (size_t)d.nxp * d.nzp;
+    const size_t ncell = npl * d.nyp;
+    const size_t ntot = nU + nV + nW + 2 * nb + 2 * npl + ncell + 2;
+    double *buf = nullptr;
+    HCK(hipMalloc(&buf, ntot * sizeof(double)));
+    double *Ru = buf, *Rv = Ru + nU, *Rw = Rv + nV, *bcs = Rw + nW, *ysrc = bcs + 2 * nb, *pcell = ysrc + 2 * npl,
+           *means = pcell + ncell;
+    hipError_t e = hipMemsetAsync(buf, 0, ntot * sizeof(double), stream);
+    S *const *F = st[cur];
+    // avg_nu_t at rows 1, 2, nyg-1, nyg (subgrid.f90:605-617)
+    double a1 = 0, a2 = 0, an1 = 0, an = 0;
+    const bool nut = h.prm.les_model == 1 || nut_user;
+    if (e == hipSuccess && nut) {
+        launch_plane_means<S>(d, NT, 2, nyg - 1, means, stream);
+        double m2[2];
+        e = hipMemcpyAsync(m2, means, sizeof m2, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        a2 = std::max(m2[0], 0.0);
+        an1 = std::max(m2[1], 0.0);
+        a1 = h.prm.dirichlet_nu_t == 1 ? -a2 : a2;
+        an = h.prm.dirichlet_nu_t == 1 ? -an1 : an1;
+    }
+    // one-sided metrics at the walls (pressure.f90:263-266, 302-305), dzeta = 1
+    const std::vector<double> &y = h.y1, &yg = h.yg1;  // 1-based
+    PressureBC c;
+    c.g_b = (-3.0 * y[1] + 4.0 * y[2] - y[3]) / 2.0;
+    c.g2_b = 2.0 * y[1] - 5.0 * y[2] + 4.0 * y[3] - 1.0 * y[4];
+    c.g_t = (3.0 * y[ny] - 4.0 * y[ny - 1] + y[ny - 2]) / 2.0;
+    c.g2_t = 2.0 * y[ny] - 5.0 * y[ny - 1] + 4.0 * y[ny - 2] - 1.0 * y[ny - 3];
+    c.nui_b = h.prm.nu + 0.5 * (a1 + a2);
+    c.dnu_b = (a2 - a1) / (yg[2] - yg[1]);
+    c.nui_t = h.prm.nu + 0.5 * (an + an1);
+    c.dnu_t = (an - an1) / (yg[nyg] - yg[nyg - 1]);
+    const double coef_bc_1 = 1.0 / (y[2] - y[1]) / (yg[2] - yg[1]);                  // initialization.f90:360-364
+    const double coef_bc_2 = 1.0 / (y[nyg - 1] - y[nyg - 2]) / (yg[nyg] - yg[nyg - 1]);  // :369-373
+    c.src_b = coef_bc_1 * (yg[2] - yg[1]);
+    c.src_t = -(coef_bc_2 * (yg[nyg] - yg[nyg - 1]));
+    if (e == hipSuccess) {
+        launch_rhs_only<S, S>(d, met, F[0], F[1], F[2], nut ? NT : nullptr, Ru, Rv, Rw, sc, ubar, h.prm.nu,
+                              h.prm.omega_z, stream, true);
+        launch_pressure_bc<S>(d, F[1], c, Rv, bcs, ysrc, stream);
+        launch_div_fftx<double, S>(d, met, Ru, Rv, Rw, spec, pM, twM, twN, stream, ysrc);
+        launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
+        launch_thomas<S>(d, spec, cscr, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
+        launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
+        launch_ifftx_proju<S, S, double>(d, met, spec, pcell, (S *)nullptr, pM, twM, twN, (double)(d.nxp * d.nzp),
+                                         stream);
+        e = hipGetLastError();
+    }
+    std::vector<double> pc(ncell), bh(2 * nb);
+    if (e == hipSuccess) e = hipMemcpyAsync(pc.data(), pcell, ncell * sizeof(double), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(bh.data(), bcs, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    hipFree(buf);
+    if (e != hipSuccess) { set_error(std::string("compute_pressure: ") + hipGetErrorString(e)); return -1; }
+    auto idx = [&](int i, int j, int k) { return i3(nxg, nyg, i, j, k); };
+    auto b = [&](int w, int i, int k) { return bh[(size_t)w * nb + (size_t)(k - 2) * (nxg - 2) + (i - 2)]; };
+    for (int k = 2; k <= nzg - 2; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 2; i++)
+                P[idx(i, j, k)] = pc[((size_t)(k - 2) * d.nyp + (j - 2)) * d.nxp + (i - 2)];
+    for (int k = 2; k <= nzg - 2; k++)
+        for (int i = 2; i <= nxg - 2; i++) {
+            P[idx(i, 1, k)] = P[idx(i, 2, k)] - (yg[2] - yg[1]) * b(0, i, k);
+            P[idx(i, nyg, k)] = P[idx(i, nyg - 1, k)] + (yg[nyg] - yg[nyg - 1]) * b(1, i, k);
+        }
+    for (int k = 2; k <= nzg - 2; k++)
+        for (int j = 1; j <= nyg; j++) P[idx(nxg - 1, j, k)] = P[idx(2, j, k)];
+    for (int j = 1; j <= nyg; j++)
+        for (int i = 2; i <= nxg - 1; i++) P[idx(i, j, nzg - 1)] = P[idx(i, j, 2)];
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) { P[idx(1, j, k)] = P[idx(2, j, k)]; P[idx(nxg, j, k)] = P[idx(nxg - 1, j, k)]; }
+    const size_t pl = (size_t)nxg * nyg;
+    std::memcpy(P, P + pl, pl * sizeof(double));
+    std::memcpy(P + pl * (nzg - 1), P + pl * (nzg - 2), pl * sizeof(double));
+    return 0;
+}
+
 template <typename S>
 int Ctx<S>::set_nu_t(const double *nt) {
     HCK(hipSetDevice(dev));
@@ -1082,6 +1175,11 @@ int cf_get_state(cf_ctx *c, double *U, double *V, double *W) {
     return c->impl->get_state(U, V, W);
 }
 int cf_get_pressure(cf_ctx *c, double *P) { CTX_CHECK(c); return c->impl->get_pressure(P); }
+int cf_compute_pressure(cf_ctx *c, double *P) {
+    CTX_CHECK(c);
+    if (!P) { set_error("cf_compute_pressure: null output"); return -1; }
+    return c->impl->compute_pressure(P);
+}
 int cf_set_nu_t(cf_ctx *c, const double *nt) { CTX_CHECK(c); return c->impl->set_nu_t(nt); }
 int cf_get_nu_t(cf_ctx *c, double *nt) { CTX_CHECK(c); return c->impl->get_nu_t(nt); }
 int cf_get_wall(cf_ctx *c, double *ax, double *az, double *ti) { CTX_CHECK(c); return c->impl->get_wall(ax, az, ti); }

@@ -238,7 +238,8 @@ template <typename S, typename T, int WG, int MC>
 __global__ void __launch_bounds__(WG)
 k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const S *__restrict__ W, Cx<T> *__restrict__ spec, FFTPlan pM,
-           const Cx<T> *__restrict__ twM, const Cx<T> *__restrict__ twN, int rb) {
+           const Cx<T> *__restrict__ twM, const Cx<T> *__restrict__ twN, int rb,
+           const double *__restrict__ ysrc) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, ls = fft_line<T>(M);
@@ -266,6 +267,10 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
             const int64_t qwm = k == 2 ? qw + (int64_t)(d.nz - 3) * nxg * nyg : qw - (int64_t)nxg * nyg;  // W(:,:,1) = W(:,:,nz-1)
             dv = ((T)U[qu] - (T)U[qum]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)m.rdyf[j] +
                  ((T)W[qw] - (T)W[qwm]) * (T)m.rdzf[k];
+            if (ysrc) {
+                if (j == 2) dv += (T)ysrc[kr * d.nxp + ip];
+                if (j == nyg - 1) dv += (T)ysrc[(d.nzp + kr) * d.nxp + ip];
+            }
         }
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
     }
@@ -687,9 +692,11 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         phi[((int64_t)kp * d.nyp + jp) * d.nxp + ip] = (P)ph;
         const int ipn = ip + 1 == N ? 0 : ip + 1;
         const double phn = (double)zr[2 * (lr * ls + fpad<T>(ipn >> 1)) + (ipn & 1)] * rnorm;
-        const int i = 2 + ip, j = 2 + jp, k = 2 + kp;
-        const int64_t qu = i3(nx, nyg, i, j, k);
-        U[qu] = (S)((double)U[qu] - (phn - ph) * m.rdxc[i]);
+        if (U) {
+            const int i = 2 + ip, j = 2 + jp, k = 2 + kp;
+            const int64_t qu = i3(nx, nyg, i, j, k);
+            U[qu] = (S)((double)U[qu] - (phn - ph) * m.rdxc[i]);
+        }
     }
 }
 
@@ -762,26 +769,26 @@ bool thomas_supported(const Dims &d) { return (d.nyp + 3) / 4 <= 512; }
 template <typename S, typename T>
 void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      Cx<T> *spec, const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN,
-                     hipStream_t st) {
+                     hipStream_t st, const double *ysrc) {
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>);
     const dim3 grid((nrows + rb - 1) / rb);
     if (fft_x_wg((int)sizeof(Cx<T>)) == 512) {
         hipLaunchKernelGGL((k_div_fftx<S, T, 512, 0>), grid, dim3(512), shm, st, d, m, U, V, W, spec, pM, twM, twN,
-                           rb);
+                           rb, ysrc);
         return;
     }
 #define CF_DIVX(MC)                                                                                        \
     case MC:                                                                                               \
         hipLaunchKernelGGL((k_div_fftx<S, T, 256, MC>), grid, dim3(256), shm, st, d, m, U, V, W, spec, pM, \
-                           twM, twN, rb);                                                                  \
+                           twM, twN, rb, ysrc);                                                            \
         return;
     switch (fft_static_sizes() ? pM.n : 0) {
         CF_X_LENGTHS(CF_DIVX)
         default:
             hipLaunchKernelGGL((k_div_fftx<S, T, 256, 0>), grid, dim3(256), shm, st, d, m, U, V, W, spec, pM, twM,
-                               twN, rb);
+                               twN, rb, ysrc);
     }
 #undef CF_DIVX
 }
@@ -876,10 +883,13 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
 template void launch_div_fftx<double, double>(const Dims &, const Metrics &, const double *,
                                               const double *, const double *, Cx<double> *,
                                               const FFTPlan &, const Cx<double> *,
-                                              const Cx<double> *, hipStream_t);
+                                              const Cx<double> *, hipStream_t, const double *);
 template void launch_div_fftx<float, float>(const Dims &, const Metrics &, const float *, const float *,
                                             const float *, Cx<float> *, const FFTPlan &,
-                                            const Cx<float> *, const Cx<float> *, hipStream_t);
+                                            const Cx<float> *, const Cx<float> *, hipStream_t, const double *);
+template void launch_div_fftx<double, float>(const Dims &, const Metrics &, const double *, const double *,
+                                            const double *, Cx<float> *, const FFTPlan &, const Cx<float> *,
+                                            const Cx<float> *, hipStream_t, const double *);
 template void launch_fftz<double>(const Dims &, Cx<double> *, const FFTPlan &, const Cx<double> *, int,
                                   hipStream_t);
 template void launch_fftz<float>(const Dims &, Cx<float> *, const FFTPlan &, const Cx<float> *, int,
@@ -896,5 +906,9 @@ template void launch_ifftx_proju<float, float, float>(const Dims &, const Metric
                                                       float *, float *, const FFTPlan &,
                                                       const Cx<float> *, const Cx<float> *, double,
                                                       hipStream_t);
+template void launch_ifftx_proju<float, float, double>(const Dims &, const Metrics &, const Cx<float> *,
+                                                       double *, float *, const FFTPlan &,
+                                                       const Cx<float> *, const Cx<float> *, double,
+                                                       hipStream_t);
 
 }  // namespace cf

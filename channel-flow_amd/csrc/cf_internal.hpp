@@ -61,6 +61,7 @@ struct CtxBase {
     virtual int set_state(const double *U, const double *V, const double *W) = 0;
     virtual int get_state(double *U, double *V, double *W) = 0;
     virtual int get_pressure(double *P) = 0;
+    virtual int compute_pressure(double *P) = 0;
     virtual int set_nu_t(const double *nt) = 0;
     virtual int get_nu_t(double *nt) = 0;
     virtual int get_wall(double *ax, double *az, double *ti) = 0;
@@ -167,6 +168,7 @@ struct Ctx : CtxBase {
     int set_state(const double *U, const double *V, const double *W) override;
     int get_state(double *U, double *V, double *W) override;
     int get_pressure(double *P) override;
+    int compute_pressure(double *P) override;
     int set_nu_t(const double *nt) override;
     int get_nu_t(double *nt) override;
     int get_wall(double *ax, double *az, double *ti) override;

@@ -23,7 +23,19 @@ void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, cons
 template <typename S, typename R>
 void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      const S *NT, double *Ru, double *Rv, double *Rw, const Scalars *sc,
-                     const double *ubar, double nu, double om, hipStream_t st);
+                     const double *ubar, double nu, double om, hipStream_t st, bool full = false);
+// actual pressure (pressure.f90): plane means of a centred field at two y
+// rows, and the wall conditions d/dy(nu dV/dy)
+template <typename S>
+void launch_plane_means(const Dims &d, const S *F, int j0, int j1, double *out, hipStream_t st);
+struct PressureBC {
+    double g_b, g2_b, g_t, g2_t;      // one-sided metric factors dy/dzeta, d2y/dzeta2 at the walls
+    double nui_b, dnu_b, nui_t, dnu_t;  // nu + avg nu_t at the wall face and its y derivative
+    double src_b, src_t;              // coef_bc_1 (yg2-yg1) and -coef_bc_2 (yg(nyg)-yg(nyg-1))
+};
+template <typename S>
+void launch_pressure_bc(const Dims &d, const S *V, const PressureBC &c, double *Rv, double *bcs, double *ysrc,
+                        hipStream_t st);
 template <typename S>
 void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *alpha,
                    double dyg_bot, double dyg_top, hipStream_t st);
@@ -58,10 +70,12 @@ void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st);
 
 // ---- Poisson chain (cf_fft.hip) ----
 int fft_rows_per_wg(int M, int elems_per_row, int esize);
+// ysrc (optional): 2 x nzp x nxp values added to the divergence of the first
+// and the last interior y row (the actual-pressure wall sources)
 template <typename S, typename T>
 void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      Cx<T> *spec, const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN,
-                     hipStream_t st);
+                     hipStream_t st, const double *ysrc = nullptr);
 int fft_cols_per_wg(int n, int esize);
 bool thomas_supported(const Dims &d);
 template <typename T>
@@ -75,6 +89,7 @@ size_t thomas_scratch_elems(const Dims &d);  // FP64 checkpoints of the lane-for
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st);
+// U null: phi only, no projection of U
 template <typename S, typename T, typename P>
 void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
                         const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,

@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(256)
 k_rhs_only(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const S *__restrict__ W, const S *__restrict__ NT, double *__restrict__ Ru,
            double *__restrict__ Rv, double *__restrict__ Rw, const Scalars *__restrict__ sc,
-           const double *__restrict__ ubar, double nu_d, double om_d) {
+           const double *__restrict__ ubar, double nu_d, double om_d, int full) {
     // covers i in 2..nxg-1, j in 2..nyg-1, k in 2..nzg-1 (union of the three
     // interior boxes, duplicates included, as the reference computes them)
     const int i = 2 + blockIdx.x * 64 + (threadIdx.x & 63);
@@ -378,6 +378,14 @@ k_rhs_only(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     pm.set_ij(m, i, j);
     pm.set_k(m, k);
     rhs_point<R, NUT, ROT>(a, pm, (R)nu_d, (R)om_d, (R)sc->dPdx, (R)sc->dPdz, du, dv, dw, ru, rv, rw);
+    // full: arrays in the field layouts (ghosts included, left untouched);
+    // otherwise the reference's interior-only rhs arrays
+    if (full) {
+        if (du) Ru[i3(d.nx, d.nyg, i, j, k)] = (double)ru;
+        if (dv) Rv[i3(d.nxg, d.ny, i, j, k)] = (double)rv;
+        if (dw) Rw[i3(d.nxg, d.nyg, i, j, k)] = (double)rw;
+        return;
+    }
     if (du) Ru[i3(d.nx - 2, d.nyg - 2, i - 1, j - 1, k - 1)] = (double)ru;
     if (dv) Rv[i3(d.nxg - 2, d.ny - 2, i - 1, j - 1, k - 1)] = (double)rv;
     if (dw) Rw[i3(d.nxg - 2, d.nyg - 2, i - 1, j - 1, k - 1)] = (double)rw;
@@ -928,12 +936,12 @@ void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, cons
 template <typename S, typename R>
 void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      const S *NT, double *Ru, double *Rv, double *Rw, const Scalars *sc,
-                     const double *ubar, double nu, double om, hipStream_t st) {
+                     const double *ubar, double nu, double om, hipStream_t st, bool full) {
     dim3 grid((d.nxg - 2 + 63) / 64, (d.nyg - 2 + 3) / 4, d.nzg - 2);
     const bool nut = NT != nullptr, rot = fabs(om) > 1e-6;
 #define L(NUTB, ROTB)                                                                              \
     hipLaunchKernelGGL((k_rhs_only<S, R, NUTB, ROTB>), grid, dim3(256), 0, st, d, m, U, V, W, NT, Ru, Rv, \
-                       Rw, sc, ubar, nu, om)
+                       Rw, sc, ubar, nu, om, full ? 1 : 0)
     if (nut) { if (rot) L(true, true); else L(true, false); }
     else { if (rot) L(false, true); else L(false, false); }
 #undef L
@@ -1035,6 +1043,72 @@ void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const
     hipLaunchKernelGGL((k_stats_planes<S>), dim3(d.ny), dim3(256), 0, st, d, m, U, V, W, ubar, out);
 }
 
+// ---------------------------------------------------------------------------
+// Actual pressure (pressure.f90:21-341), the pieces around the Poisson chain.
+// ---------------------------------------------------------------------------
+// mean of F(2:nxg-1, j, 2:nzg-1) for j = j0 (block 0) and j1 (block 1), in
+// FP64 with a fixed reduction order (subgrid.f90:605-607)
+template <typename S>
+__global__ void __launch_bounds__(256) k_plane_means(Dims d, const S *__restrict__ F, int j0, int j1,
+                                                    double *__restrict__ out) {
+    __shared__ double sh[8];
+    const int j = blockIdx.x == 0 ? j0 : j1;
+    const int ni = d.nxg - 2, n = ni * (d.nzg - 2);
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < n; q += 256) {
+        const int k = 2 + q / ni, i = 2 + q % ni;
+        acc += (double)F[i3(d.nxg, d.nyg, i, j, k)];
+    }
+    acc = block_reduce(acc, OpAdd(), sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc / (double)n;
+}
+
+// Wall conditions dP/dy = d/dy(nu dV/dy) (pressure.f90:256-341) at every
+// (i, k) of 2..nxg-1 x 2..nzg-1: bc goes to the wall rows of the rhs_v array
+// (compute_rhs_v_boundary, :240-250), to bcs (2 x (nxg-2) x (nzg-2)) and,
+// times the solver coefficients, to the source of the first and last rows of
+// the Poisson right-hand side at the unique cells (:137-139, applied before
+// the transforms: the DFT is linear).
+template <typename S>
+__global__ void k_pressure_bc(Dims d, const S *__restrict__ V, PressureBC c, double *__restrict__ Rv,
+                              double *__restrict__ bcs, double *__restrict__ ysrc) {
+    const int ni = d.nxg - 2, n = ni * (d.nzg - 2);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = 2 + q % ni, k = 2 + q / ni;
+    const int nxg = d.nxg, ny = d.ny;
+    auto v = [&](int j) { return (double)V[i3(nxg, ny, i, j, k)]; };
+    const double dzeta = 1.0;
+    double dV = (-3.0 * v(1) + 4.0 * v(2) - v(3)) / (2.0 * dzeta) * 1.0 / c.g_b;
+    double ddV = (2.0 * v(1) - 5.0 * v(2) + 4.0 * v(3) - 1.0 * v(4)) / (dzeta * dzeta);
+    ddV = (ddV - dV * c.g2_b) / (c.g_b * c.g_b);
+    const double b1 = c.dnu_b * dV + c.nui_b * ddV;
+    dV = (3.0 * v(ny) - 4.0 * v(ny - 1) + v(ny - 2)) / (2.0 * dzeta) * 1.0 / c.g_t;
+    ddV = (2.0 * v(ny) - 5.0 * v(ny - 1) + 4.0 * v(ny - 2) - 1.0 * v(ny - 3)) / (dzeta * dzeta);
+    ddV = (ddV - dV * c.g2_t) / (c.g_t * c.g_t);
+    const double b2 = c.dnu_t * dV + c.nui_t * ddV;
+    Rv[i3(nxg, ny, i, 1, k)] = b1;
+    Rv[i3(nxg, ny, i, ny, k)] = b2;
+    bcs[q] = b1;
+    bcs[n + q] = b2;
+    if (i <= d.nxp + 1 && k <= d.nzp + 1) {
+        const int u = (k - 2) * d.nxp + (i - 2);
+        ysrc[u] = c.src_b * b1;
+        ysrc[d.nxp * d.nzp + u] = c.src_t * b2;
+    }
+}
+
+template <typename S>
+void launch_plane_means(const Dims &d, const S *F, int j0, int j1, double *out, hipStream_t st) {
+    hipLaunchKernelGGL((k_plane_means<S>), dim3(2), dim3(256), 0, st, d, F, j0, j1, out);
+}
+template <typename S>
+void launch_pressure_bc(const Dims &d, const S *V, const PressureBC &c, double *Rv, double *bcs, double *ysrc,
+                        hipStream_t st) {
+    const int n = (d.nxg - 2) * (d.nzg - 2);
+    hipLaunchKernelGGL((k_pressure_bc<S>), dim3((n + 255) / 256), dim3(256), 0, st, d, V, c, Rv, bcs, ysrc);
+}
+
 #define INST(S, R)                                                                                    \
     template void launch_rhs_rk<S, R>(const Dims &, const Metrics &, const S *, const S *, const S *, \
                                       const S *, S *, S *, S *, S *, S *, S *, const Scalars *,       \
@@ -1042,7 +1116,8 @@ void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const
                                       hipStream_t);                                                   \
     template void launch_rhs_only<S, R>(const Dims &, const Metrics &, const S *, const S *,          \
                                         const S *, const S *, double *, double *, double *,          \
-                                        const Scalars *, const double *, double, double, hipStream_t);
+                                        const Scalars *, const double *, double, double, hipStream_t, \
+                                        bool);
 INST(double, double)
 INST(float, float)
 #undef INST
@@ -1064,7 +1139,10 @@ INST(float, float)
                                         const Scalars *, const double *, double, double, int,         \
                                         hipStream_t);                                                  \
     template void launch_stats<S>(const Dims &, const Metrics &, const S *, const S *, const S *,     \
-                                  const double *, double *, hipStream_t);
+                                  const double *, double *, hipStream_t);                              \
+    template void launch_plane_means<S>(const Dims &, const S *, int, int, double *, hipStream_t);     \
+    template void launch_pressure_bc<S>(const Dims &, const S *, const PressureBC &, double *, double *, \
+                                        double *, hipStream_t);
 INST1(double)
 INST1(float)
 #undef INST1

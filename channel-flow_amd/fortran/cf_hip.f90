@@ -55,6 +55,12 @@ Module cf_hip
        Real(c_double), Intent(Out) :: P(*)
        Integer(c_int) :: rc
      End Function cf_get_pressure
+     Function cf_compute_pressure(ctx, P) Bind(C, name='cf_compute_pressure') Result(rc)
+       Import :: c_int, c_double, c_ptr
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: P(*)
+       Integer(c_int) :: rc
+     End Function cf_compute_pressure
      Function cf_get_nu_t(ctx, a) Bind(C, name='cf_get_nu_t') Result(rc)
        Import :: c_int, c_double, c_ptr
        Type(c_ptr), Value :: ctx

@@ -3,6 +3,7 @@
 ! dummy arguments, each body a call into the HIP library.           !
 !   equations.f90:24,186,351   boundary_conditions.f90:20           !
 !   projection.f90:20,234      mass_flow.f90:25,103                 !
+!   pressure.f90:21                                                 !
 !   subgrid.f90:23             wallmodel.f90:24   eqwm.f90:90       !
 !   time_integration.f90:89,179                                     !
 !-------------------------------------------------------------------!
@@ -79,6 +80,16 @@ Contains
     max_divergence = ms
   End Subroutine check_divergence
 End Module projection
+
+Module pressure
+  Use global
+  Implicit None
+Contains
+  ! the actual pressure of the current device state into the module array P
+  Subroutine compute_pressure
+    Call cf_check(cf_compute_pressure(ctx, P), 'compute_pressure')
+  End Subroutine compute_pressure
+End Module pressure
 
 Module mass_flow
   Use global

@@ -65,6 +65,10 @@ int cf_destroy(cf_ctx *c);
 int cf_set_state(cf_ctx *c, const double *U, const double *V, const double *W);
 int cf_get_state(cf_ctx *c, double *U, double *V, double *W);
 int cf_get_pressure(cf_ctx *c, double *P);      /* projection.f90:129-137 */
+/* compute_pressure (pressure.f90:21-341): the actual pressure of the current
+   state, written to P (nxg,nyg,nzg) on the host.  Uses the current nu_t for
+   the plane-averaged eddy viscosity of the wall conditions. */
+int cf_compute_pressure(cf_ctx *c, double *P);
 int cf_set_nu_t(cf_ctx *c, const double *nu_t);
 int cf_get_nu_t(cf_ctx *c, double *nu_t);
 int cf_get_wall(cf_ctx *c, double *alpha_x, double *alpha_z, double *tau_init);

@@ -50,6 +50,23 @@ def test_mixed_tracks_fp64(nsteps):
     assert abs(g32.compute_mean_mass_flow_U() - g32.scalars["Qflow_x_0"]) < 1e-12
 
 
+def test_mixed_actual_pressure_tracks_fp64():
+    """compute_pressure in a mixed context (FP32 stencils and spectrum, FP64
+    y solve) against the FP64 context on the same state: 1e-3 of max |P|
+    (the right-hand sides are differences of FP32 velocities)."""
+    CF = solver()
+    g64 = CF(cfg_of("fp64", dPdx=2e-3))
+    g32 = CF(cfg_of("mixed", dPdx=2e-3))
+    U, V, W = cases.initial_field(g64.cfg)
+    g64.initialise(U, V, W)
+    g32.initialise(U, V, W)
+    g64.run(2, graph=True)
+    g32.run(2, graph=True)
+    p64, p32 = g64.compute_pressure(), g32.compute_pressure()
+    assert np.all(np.isfinite(p32))
+    assert relerr(p32, p64) < 1e-3, relerr(p32, p64)
+
+
 def test_mixed_deviation_storage_roundtrip():
     """set_state/get_state through the FP32 deviation storage lose less than
     FP32 rounding of the deviation."""

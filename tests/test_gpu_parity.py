@@ -200,3 +200,26 @@ def test_eqwm_batch_matches_oracle():
         t_o, i_o = orc_eqwm(u[q], nu[q], h[q], tin[q], ncv=30, stretch=1.1, tol=1e-8, max_iter=100)
         assert abs(tau[q] - t_o) <= 1e-6 * abs(t_o)
         assert abs(int(it[q]) - i_o) <= 1
+
+
+@pytest.mark.parametrize("les", [False, True])
+def test_actual_pressure_matches_oracle(les):
+    """compute_pressure (pressure.f90:21-341) on the same state: the wall
+    conditions from V and the plane-averaged eddy viscosity, the divergence
+    of the momentum right-hand sides, and the Poisson solve with wall
+    sources.  The GPU adds the wall sources before the transforms (the
+    reference after them), which changes rounding only: FP64 tolerance 1e-10."""
+    cfg = small_cfg(les_model=1 if les else 0, cs=0.1, dPdx=2e-3)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(2, graph=False)
+    o.run(2)
+    if les:
+        g.compute_eddy_viscosity()
+        o.compute_eddy_viscosity()
+    Pg = g.compute_pressure()
+    Po = o.compute_pressure()
+    assert np.all(np.isfinite(Pg))
+    assert relerr(Pg, Po) < 1e-10, relerr(Pg, Po)
