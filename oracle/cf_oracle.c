@@ -928,21 +928,17 @@ double orc_compute_dt(orc_t *o) {
  * subgrid.f90:891-949 compute_Sij and 564-619 constant Smagorinsky,
  * completed with the ghost sequence of subgrid.f90:521-536
  * ====================================================================== */
-void orc_eddy_viscosity(orc_t *o) {
-    const int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg, ny = o->ny;
-    double *nut = o->nu_t;
-    if (o->p.les_model != 1) { /* subgrid.f90:41-45 */
-        memset(nut, 0, o->nC * sizeof(double));
-        memset(o->avg_nu_t, 0, o->nyg * sizeof(double));
-        return;
-    }
+/* compute_Sij (subgrid.f90:891-949): Sij at cell centres (the off-diagonal
+   components averaged from the edges) and |S| = sqrt(2 Sij Sij).
+   S6 holds Sij_(2:nxg,2:nyg,2:nzg,6); Smag is a full (nxg,nyg,nzg) array
+   of which the interior 2..nxg-1 x 2..nyg-1 x 2..nzg-1 is written. */
+#define SIJ(i, j, k, c) S6[(size_t)(c - 1) * ns + I3(s1, s2, (i)-1, (j)-1, (k)-1)]
+static void compute_sij(const orc_t *o, const double *U, const double *V, const double *W, double *S6,
+                        double *Smag) {
+    const int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
     const double *x = o->x, *y = o->y, *z = o->z, *xg = o->xg, *yg = o->yg, *zg = o->zg;
-    const double *U = o->U, *V = o->V, *W = o->W;
-    /* Sij_(2:nxg,2:nyg,2:nzg,6) */
-    int s1 = nxg - 1, s2 = nyg - 1, s3 = nzg - 1;
-    size_t ns = (size_t)s1 * s2 * s3;
-    double *S = dalloc(6 * ns);
-#define SIJ(i, j, k, c) S[(size_t)(c - 1) * ns + I3(s1, s2, (i)-1, (j)-1, (k)-1)]
+    const int s1 = nxg - 1, s2 = nyg - 1, s3 = nzg - 1;
+    const size_t ns = (size_t)s1 * s2 * s3;
 #pragma omp parallel for schedule(static)
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
@@ -968,6 +964,34 @@ void orc_eddy_viscosity(orc_t *o) {
                 SIJ(i, j, k, 5) = 0.25 * (SIJ(i, j, k, 5) + SIJ(i + 1, j, k, 5) + SIJ(i, j, k + 1, 5) + SIJ(i + 1, j, k + 1, 5));
                 SIJ(i, j, k, 6) = 0.25 * (SIJ(i, j, k, 6) + SIJ(i, j + 1, k, 6) + SIJ(i, j, k + 1, 6) + SIJ(i, j + 1, k + 1, 6));
             }
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++) {
+                double sm = 0.0;
+                for (int c = 1; c <= 6; c++) {
+                    double e = SIJ(i, j, k, c);
+                    sm = sm + (c <= 3 ? 2.0 : 4.0) * e * e;
+                }
+                Smag[IC(i, j, k)] = sqrt(sm);
+            }
+}
+
+static void orc_dsm(orc_t *o);
+
+void orc_eddy_viscosity(orc_t *o) {
+    const int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg, ny = o->ny;
+    double *nut = o->nu_t;
+    if (o->p.les_model == 2) { orc_dsm(o); return; }
+    if (o->p.les_model != 1) { /* subgrid.f90:41-45 */
+        memset(nut, 0, o->nC * sizeof(double));
+        memset(o->avg_nu_t, 0, o->nyg * sizeof(double));
+        return;
+    }
+    const double *y = o->y, *yg = o->yg;
+    const int s1 = nxg - 1, s2 = nyg - 1, s3 = nzg - 1;
+    const size_t ns = (size_t)s1 * s2 * s3;
+    double *S6 = dalloc(6 * ns), *Smag = dalloc(o->nC);
+    compute_sij(o, o->U, o->V, o->W, S6, Smag);
     /* subgrid.f90:583-602 */
     double utau_t = sqrt(o->dPdx);
     for (int j = 2; j <= nyg - 1; j++) {
@@ -978,14 +1002,7 @@ void orc_eddy_viscosity(orc_t *o) {
         double Delta = pow(o->dx * o->dz * (Yf(j) - Yf(j - 1)), 1.0 / 3.0);
         double coef = sq(o->p.cs * Delta * f);
         for (int k = 2; k <= nzg - 1; k++)
-            for (int i = 2; i <= nxg - 1; i++) {
-                double s = 0.0;
-                for (int c = 1; c <= 6; c++) {
-                    double e = SIJ(i, j, k, c);
-                    s = s + (c <= 3 ? 2.0 : 4.0) * e * e;
-                }
-                nut[IC(i, j, k)] = coef * sqrt(s);
-            }
+            for (int i = 2; i <= nxg - 1; i++) nut[IC(i, j, k)] = coef * Smag[IC(i, j, k)];
     }
     /* subgrid.f90:605-617 */
     for (int j = 2; j <= nyg - 1; j++) {
@@ -1003,8 +1020,168 @@ void orc_eddy_viscosity(orc_t *o) {
     for (size_t q = 0; q < o->nC; q++) nut[q] = fmax(nut[q], 0.0);
     if (o->p.dirichlet_nu_t == 1) dirichlet_y(nut, nxg, nyg, nzg, 0);
     else neumann_y(nut, nxg, nyg, nzg);
-    free(S);
+    free(S6);
+    free(Smag);
+}
+
+/* ======================================================================
+ * Dynamic Smagorinsky, Lilly's form (subgrid.f90:376-538) with the
+ * filters filter_xzy (subgrid.f90:134-192): Simpson weights in x and z
+ * (1/36 1/9 1/36; 1/9 4/9 1/9) and (1/6 2/3 1/6) in y, one-sided
+ * (2/3 1/3) on the first and last row of the filtered y range.  Arrays are
+ * full (n1,n2,n3) with 1-based indices; the reference's sections map to
+ * index ranges given explicitly.
+ * ====================================================================== */
+/* Tf(I,J,K) for I in [i0,i1], K in [k0,k1], J in [j0,j1] (the filtered y
+   range: J = j0 and j1 are the one-sided rows), accumulated in the
+   reference's order (x offset outer, z offset inner) */
+static void filter_xzy_range(const double *T, int n1, int n2, double *Tf, int i0, int i1, int j0, int j1,
+                             int k0, int k1) {
+    static const double fil[3][3] = {{1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0},
+                                     {1.0 / 9.0, 4.0 / 9.0, 1.0 / 9.0},
+                                     {1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0}};
+#define TT(i, j, k) T[I3(n1, n2, i, j, k)]
+#pragma omp parallel for schedule(static)
+    for (int fk = k0; fk <= k1; fk++)
+        for (int fj = j0; fj <= j1; fj++)
+            for (int fi = i0; fi <= i1; fi++) {
+                double acc = 0.0;
+                for (int a = -1; a <= 1; a++)
+                    for (int c = -1; c <= 1; c++) {
+                        double t;
+                        if (fj == j0)
+                            t = 2.0 / 3.0 * TT(fi + a, fj, fk + c) + 1.0 / 3.0 * TT(fi + a, fj + 1, fk + c);
+                        else if (fj == j1)
+                            t = 1.0 / 3.0 * TT(fi + a, fj - 1, fk + c) + 2.0 / 3.0 * TT(fi + a, fj, fk + c);
+                        else
+                            t = 1.0 / 6.0 * TT(fi + a, fj - 1, fk + c) + 2.0 / 3.0 * TT(fi + a, fj, fk + c) +
+                                1.0 / 6.0 * TT(fi + a, fj + 1, fk + c);
+                        acc = acc + t * fil[a + 1][c + 1];
+                    }
+                Tf[I3(n1, n2, fi, fj, fk)] = acc;
+            }
+#undef TT
+}
+
+static void orc_dsm(orc_t *o) {
+    const int nx = o->nx, ny = o->ny, nz = o->nz, nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
+    const double *U = o->U, *V = o->V, *W = o->W;
+    const size_t nC = o->nC;
+    const int s1 = nxg - 1, s2 = nyg - 1, s3 = nzg - 1;
+    const size_t ns = (size_t)s1 * s2 * s3;
+    double *uc = dalloc(nC), *vc = dalloc(nC), *wc = dalloc(nC); /* term, term_1, term_2 */
+    double *tb = dalloc(6 * nC), *L = dalloc(6 * nC), *hv = dalloc(3 * nC), *M = dalloc(6 * nC);
+    double *S6 = dalloc(6 * ns), *Smag = dalloc(nC), *tmp = dalloc(nC), *tf = dalloc(nC);
+    /* Part 1: velocities at the cell centres (interpolate_x/y/z, faces to
+       centres), periodic in x (term) and z (term_2) */
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++)
+            for (int i = 2; i <= nxg - 1; i++) uc[IC(i, j, k)] = 0.5 * (u(i - 1, j, k) + u(i, j, k));
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 1; i <= nxg; i++) vc[IC(i, j, k)] = 0.5 * (v(i, j - 1, k) + v(i, j, k));
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 1; j <= nyg; j++)
+            for (int i = 1; i <= nxg; i++) wc[IC(i, j, k)] = 0.5 * (w(i, j, k - 1) + w(i, j, k));
+    periodic_x(o, uc, nxg, nyg, nzg, 0);
+    periodic_z(wc, nxg, nyg, nzg, 0);
+    /* products u_i u_j at the centres (2:nxg, 2:nyg-1, 2:nzg), periodic */
+    for (int c = 0; c < 6; c++) {
+        double *T = tb + (size_t)c * nC;
+        const double *A = c == 0 || c == 3 || c == 4 ? uc : (c == 1 || c == 5 ? vc : wc);
+        const double *B = c == 0 ? uc : (c == 1 || c == 3 ? vc : wc);
+        for (int k = 2; k <= nzg; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg; i++) T[IC(i, j, k)] = A[IC(i, j, k)] * B[IC(i, j, k)];
+        periodic_x(o, T, nxg, nyg, nzg, 0);
+        periodic_z(T, nxg, nyg, nzg, 0);
+        /* hat(ui uj) */
+        filter_xzy_range(T, nxg, nyg, L + (size_t)c * nC, 2, nxg - 1, 2, nyg - 1, 2, nzg - 1);
+    }
+    /* hat(ui) */
+    filter_xzy_range(uc, nxg, nyg, hv, 2, nxg - 1, 2, nyg - 1, 2, nzg - 1);
+    filter_xzy_range(vc, nxg, nyg, hv + nC, 2, nxg - 1, 2, nyg - 1, 2, nzg - 1);
+    filter_xzy_range(wc, nxg, nyg, hv + 2 * nC, 2, nxg - 1, 2, nyg - 1, 2, nzg - 1);
+    for (int c = 0; c < 6; c++) {
+        const double *A = hv + (size_t)(c == 0 || c == 3 || c == 4 ? 0 : (c == 1 || c == 5 ? 1 : 2)) * nC;
+        const double *B = hv + (size_t)(c == 0 ? 0 : (c == 1 || c == 3 ? 1 : 2)) * nC;
+        double *Lc = L + (size_t)c * nC;
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) Lc[IC(i, j, k)] = Lc[IC(i, j, k)] - A[IC(i, j, k)] * B[IC(i, j, k)];
+    }
+    /* Part 2: filtered velocities with their periodic ghosts */
+    double *Uf = dalloc(o->nU), *Vf = dalloc(o->nV), *Wf = dalloc(o->nW);
+    filter_xzy_range(U, nx, nyg, Uf, 2, nx - 1, 1, nyg, 2, nzg - 1);
+    filter_xzy_range(V, nxg, ny, Vf, 2, nxg - 1, 1, ny, 2, nzg - 1);
+    filter_xzy_range(W, nxg, nyg, Wf, 2, nxg - 1, 1, nyg, 2, nz - 1);
+    periodic_x(o, Uf, nx, nyg, nzg, 1);
+    periodic_z(Uf, nx, nyg, nzg, 0);
+    periodic_x(o, Vf, nxg, ny, nzg, 0);
+    periodic_z(Vf, nxg, ny, nzg, 0);
+    periodic_x(o, Wf, nxg, nyg, nz, 0);
+    periodic_z(Wf, nxg, nyg, nz, 1);
+    /* M = fil_size^2 |hat S| hat(Sij) - hat(|S| Sij), fil_size = 2 */
+    compute_sij(o, Uf, Vf, Wf, S6, Smag);
+#define SIJ(i, j, k, c) S6[(size_t)(c - 1) * ns + I3(s1, s2, (i)-1, (j)-1, (k)-1)]
+    const double fs2 = pow(2.0, 2.0);
+    for (int c = 1; c <= 6; c++)
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++)
+                    M[(size_t)(c - 1) * nC + IC(i, j, k)] = fs2 * Smag[IC(i, j, k)] * SIJ(i, j, k, c);
+    compute_sij(o, U, V, W, S6, Smag);
+    for (int c = 1; c <= 6; c++) {
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) tmp[IC(i, j, k)] = Smag[IC(i, j, k)] * SIJ(i, j, k, c);
+        periodic_x(o, tmp, nxg, nyg, nzg, 0);
+        periodic_z(tmp, nxg, nyg, nzg, 0);
+        filter_xzy_range(tmp, nxg, nyg, tf, 2, nxg - 1, 2, nyg - 1, 2, nzg - 1);
+        double *Mc = M + (size_t)(c - 1) * nC;
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nxg - 1; i++) Mc[IC(i, j, k)] = Mc[IC(i, j, k)] - tf[IC(i, j, k)];
+    }
 #undef SIJ
+    /* Part 3: plane averages <Lij Mij> / <Mij Mij>, nu_t = -0.5 avg |S| */
+    double *nut = o->nu_t, *avg = o->avg_nu_t;
+    memset(nut, 0, nC * sizeof(double));
+#define LL(c) L[(size_t)((c)-1) * nC + q]
+#define MM(c) M[(size_t)((c)-1) * nC + q]
+    for (int j = 2; j <= nyg - 1; j++) {
+        double lm = 0.0, mm = 0.0;
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) {
+                const size_t q = IC(i, j, k);
+                lm += LL(1) * MM(1) + LL(2) * MM(2) + LL(3) * MM(3) + 2.0 * (LL(4) * MM(4) + LL(5) * MM(5) + LL(6) * MM(6));
+            }
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) {
+                const size_t q = IC(i, j, k);
+                mm += MM(1) * MM(1) + MM(2) * MM(2) + MM(3) * MM(3) + 2.0 * (MM(4) * MM(4) + MM(5) * MM(5) + MM(6) * MM(6));
+            }
+        avg[j - 1] = lm / mm;
+    }
+#undef LL
+#undef MM
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++) nut[IC(i, j, k)] = -0.5 * avg[j - 1] * Smag[IC(i, j, k)];
+    periodic_x(o, nut, nxg, nyg, nzg, 0);
+    periodic_z(nut, nxg, nyg, nzg, 0);
+    for (size_t q = 0; q < nC; q++) nut[q] = fmax(nut[q], 0.0);
+    if (o->p.dirichlet_nu_t == 1) {
+        dirichlet_y(nut, nxg, nyg, nzg, 0);
+        avg[0] = -avg[1];
+        avg[nyg - 1] = -avg[nyg - 2];
+    } else {
+        neumann_y(nut, nxg, nyg, nzg);
+        avg[0] = avg[1];
+        avg[nyg - 1] = avg[nyg - 2];
+    }
+    free(uc); free(vc); free(wc); free(tb); free(L); free(hv); free(M);
+    free(S6); free(Smag); free(tmp); free(tf); free(Uf); free(Vf); free(Wf);
 }
 
 /* ======================================================================

@@ -150,6 +150,12 @@ class Oracle:
         self.L.orc_get_nu_t(self.h, P(a), None)
         return a
 
+    @property
+    def avg_nu_t(self):
+        a = np.zeros(self.nyg)
+        self.L.orc_get_nu_t(self.h, None, P(a))
+        return a
+
     def set_nu_t(self, a):
         a = np.asfortranarray(a, dtype=np.float64)
         self.L.orc_set_nu_t(self.h, P(a))

@@ -275,6 +275,125 @@ def test_smagorinsky_simple_shear():
     np.testing.assert_allclose(nt[2:-2, 3:-3, 2:-2], expect, rtol=1e-12)
 
 
+# ------------------------------------------------------ dynamic Smagorinsky
+def _hat_z(a):
+    """Simpson filter along one periodic direction: the x-z weights
+    (1/36 1/9 1/36; 1/9 4/9 1/9) on a field constant in x, and the y weights
+    on a field constant in y, both reduce to (1/6 2/3 1/6)."""
+    return np.roll(a, 1) / 6 + 2 * a / 3 + np.roll(a, -1) / 6
+
+
+def _dsm_z_only(uc, wf, dz):
+    """Lilly's dynamic coefficient for U = U(z) at the centres and W = W(z) on
+    the z faces (centre m between faces m-1 and m), V = 0, derived by hand:
+    the only nonzero strains are S33 (centres) and S13 (edge values
+    1/2 dU/dz, averaged to the centres); L and M reduce to their 13 and 33
+    components.  The plane sums run over centres 2..nzg-1 of the reference,
+    which counts the first centre twice (its periodic image at nzg-1)."""
+    wc = 0.5 * (np.roll(wf, 1) + wf)
+
+    def strain(u, w):
+        s33 = (w - np.roll(w, 1)) / dz
+        e = 0.5 * (u - np.roll(u, 1)) / dz
+        s13 = 0.5 * (e + np.roll(e, -1))
+        return s33, s13, np.sqrt(2 * s33 ** 2 + 4 * s13 ** 2)
+
+    L33 = _hat_z(wc * wc) - _hat_z(wc) ** 2
+    L13 = _hat_z(uc * wc) - _hat_z(uc) * _hat_z(wc)
+    s33, s13, S = strain(uc, wf)
+    f33, f13, Sf = strain(_hat_z(uc), _hat_z(wf))
+    M33 = 4 * Sf * f33 - _hat_z(S * s33)   # filter width ratio 2, squared
+    M13 = 4 * Sf * f13 - _hat_z(S * s13)
+    lm, mm = L33 * M33 + 2 * L13 * M13, M33 ** 2 + 2 * M13 ** 2
+    avg = (lm.sum() + lm[0]) / (mm.sum() + mm[0])
+    return avg, np.maximum(-0.5 * avg * S, 0.0)
+
+
+@pytest.mark.parametrize("dirichlet", [0, 1])
+def test_dynamic_smagorinsky_z_only_field(dirichlet):
+    """sgs_dynamic_Smagorinsky (subgrid.f90:376-538) on U(z), W(z): the 3-D
+    oracle agrees with the one-dimensional reduction above, row by row, and
+    the wall ghosts follow the Dirichlet (odd) or Neumann (even) rule."""
+    c = cfg_small(les_model=2, cells=(8, 10, 16), dirichlet_nu_t=dirichlet)
+    o = Oracle(c)
+    g = o.grid()
+    Lz = c.lengths[2]
+    U, V, W = fields(o, lambda x, y, z: 0.9 * np.sin(2 * np.pi * z / Lz + 0.3) + 0 * x * y, None,
+                     lambda x, y, z: 0.5 * np.cos(4 * np.pi * z / Lz + 1.1) + 0 * x * y)
+    o.set_state(U, V, W)
+    o.compute_eddy_viscosity()
+    nt, av = o.nu_t, o.avg_nu_t
+    N = o.nz - 2
+    avg, n1 = _dsm_z_only(U[0, 0, 1:1 + N], W[0, 0, 1:1 + N], o.z[1] - o.z[0])
+    assert avg < 0 and n1.max() > 0          # the model is active
+    np.testing.assert_allclose(av[1:-1], avg, rtol=1e-12)
+    np.testing.assert_allclose(nt[1:-1, 1:-1, 1:1 + N], np.broadcast_to(n1, nt[1:-1, 1:-1, 1:1 + N].shape),
+                               rtol=1e-11, atol=1e-18)
+    sgn = -1.0 if dirichlet else 1.0
+    np.testing.assert_array_equal(av[[0, -1]], sgn * av[[1, -2]])
+    np.testing.assert_array_equal(nt[:, 0, :], sgn * nt[:, 1, :])
+    np.testing.assert_array_equal(nt[:, -1, :], sgn * nt[:, -2, :])
+
+
+def _random_bc_state(o, seed):
+    rng = np.random.default_rng(seed)
+    U, V, W = (rng.standard_normal(s) for s in (o.shape_u, o.shape_v, o.shape_w))
+    o.set_state(U, V, W)
+    o.apply_boundary_conditions()
+    return o.get_state()
+
+
+def _dsm(o, U, V, W):
+    o.set_state(U, V, W)
+    o.compute_eddy_viscosity()
+    return o.nu_t, o.avg_nu_t
+
+
+def test_dynamic_smagorinsky_x_z_exchange():
+    """Exchanging the roles of x and z (U <-> W transposed, equal periodic
+    grids) transposes nu_t and keeps the plane averages: pins the pairing of
+    the 12/13/23 components, the x and z face offsets and the filter axes."""
+    c = cfg_small(les_model=2, cells=(12, 10, 12), lengths=(math.pi, 2.0, math.pi))
+    o = Oracle(c)
+    U, V, W = _random_bc_state(o, 3)
+    nt, av = _dsm(o, U, V, W)
+    t = (2, 1, 0)
+    nt2, av2 = _dsm(o, W.transpose(t), V.transpose(t), U.transpose(t))
+    assert (nt[1:-2, 1:-1, 1:-2] > 0).mean() > 0.2
+    np.testing.assert_allclose(av2, av, rtol=1e-12)
+    np.testing.assert_allclose(nt2, nt.transpose(t), rtol=1e-11, atol=1e-14 * np.abs(nt).max())
+
+
+def test_dynamic_smagorinsky_wall_mirror():
+    """Mirroring y about the channel centre (U, W reversed, V reversed and
+    negated, on a stretched grid symmetric by construction) reverses nu_t and
+    the averages: pins the one-sided filter rows, the V-to-centre
+    interpolation and the y offsets of the edge strains."""
+    c = cfg_small(les_model=2, cells=(12, 14, 8))
+    x, y, z = c.grid()
+    half = y[: (len(y) + 1) // 2]
+    y = np.concatenate([half, 2.0 - half[::-1][1:]]) if len(y) % 2 else np.concatenate([half, 2.0 - half[::-1]])
+    o = Oracle(c, x, y, z)
+    U, V, W = _random_bc_state(o, 5)
+    nt, av = _dsm(o, U, V, W)
+    nt2, av2 = _dsm(o, U[:, ::-1, :], -V[:, ::-1, :], W[:, ::-1, :])
+    np.testing.assert_allclose(av2, av[::-1], rtol=1e-9)
+    np.testing.assert_allclose(nt2, nt[:, ::-1, :], rtol=1e-9, atol=1e-12 * np.abs(nt).max())
+
+
+def test_dynamic_smagorinsky_galilean_invariance():
+    """Lij = hat(ui uj) - hat(ui) hat(uj) and Mij are unchanged by a uniform
+    velocity added to U, V and W: the filters preserve constants (one-sided
+    wall rows included) and the products pair the same components."""
+    c = cfg_small(les_model=2, cells=(12, 10, 8))
+    o = Oracle(c)
+    U, V, W = _random_bc_state(o, 7)
+    nt, av = _dsm(o, U, V, W)
+    nt2, av2 = _dsm(o, U + 0.7, V - 0.3, W + 0.45)
+    np.testing.assert_allclose(av2, av, rtol=1e-10)
+    np.testing.assert_allclose(nt2, nt, rtol=1e-10, atol=1e-12 * np.abs(nt).max())
+
+
 # -------------------------------------------------------------------- EQWM
 def test_eqwm_laminar_limit():
     """Tiny u: mixing length negligible, linear profile, tau = nu u / h."""
