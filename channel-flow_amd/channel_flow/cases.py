@@ -138,6 +138,16 @@ def reb_from_retau(retau):
 
 
 # ---- the five workloads ---------------------------------------------------
+def case_example():
+    """The reference's example input (input_parameters_test:1-37): 34x41x34
+    faces (32x40x32 cells), nu = 8.888...e-5, constant dPdx = 0.00138, the
+    dynamic Smagorinsky model with Neumann nu_t ghosts, no wall model,
+    CFL 0.5, 1000 steps.  The grid is this package's tanh 2.0 grid."""
+    return CaseConfig(name="example", cells=(32, 40, 32), stretch=2.0, nu=8.888888888888888e-05,
+                      cfl=0.5, dPdx=0.001379795599360, x_mass_cte=0, les_model=2, iwall_model=0,
+                      dirichlet_nu_t=0, steps=1000)
+
+
 def case_c1():
     """64^3 DNS, Re_b = 2800, tanh 2.0, dt = 2e-3, 100 steps."""
     return CaseConfig(name="dns64", cells=(64, 64, 64), stretch=2.0, nu=1 / 2800.0,

@@ -184,7 +184,7 @@ std::string Host::validate() const {
     if (d.nxp / 2 > 1024 || d.nzp > 2048) return "FFT length too large";
     if (!thomas_supported(d)) return "too many y cells for the y solve";
     if (p.precision != 0 && p.precision != 1) return "precision must be 0 (fp64) or 1 (mixed)";
-    if (p.les_model != 0 && p.les_model != 1) return "les_model must be 0 or 1";
+    if (p.les_model < 0 || p.les_model > 2) return "les_model must be 0, 1 or 2";
     if (p.iwall_model != 0 && p.iwall_model != 3) return "iwall_model must be 0 or 3";
     if (p.iwall_model == 3) {
         if (p.jref_eqwm < 2 || p.jref_eqwm > d.nyg - 1) return "jref_eqwm out of range";
@@ -271,6 +271,9 @@ int Ctx<S>::init() {
         A(&m_jjx, h.jjx.size()) || A(&m_wx, h.wx.size()))
         return -1;
     if (A(&twM, 2 * (size_t)M + 2) || A(&twN, (size_t)d.nxp + 1) || A(&twZ, 2 * (size_t)d.nzp + 2)) return -1;
+    if (h.prm.les_model == 2 && (A(&dsm.Q, 10 * nC) || A(&dsm.Fu, nU + nV + nW) ||
+                                 A(&dsm.part, dsm_part_elems(d)) || A(&dsm.avg, (size_t)d.nyg)))
+        return -1;
     if (A(&alpha_x, nwx) || A(&alpha_z, nwz) || A(&azt, nwx) || A(&tau_init, nwx)) return -1;
     if (dry) return 0;
 
@@ -280,6 +283,7 @@ int Ctx<S>::init() {
     if (!stream) HCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     else stream_ext = true;
     HCK(hipMemsetAsync(NT, 0, nC * sizeof(S), stream));
+    if (dsm.avg) HCK(hipMemsetAsync(dsm.avg, 0, d.nyg * sizeof(double), stream));
     for (int b = 0; b < 2; b++)
         for (int c = 0; c < 3; c++) HCK(hipMemsetAsync(st[b][c], 0, n3[c] * sizeof(S), stream));
     HCK(hipMemsetAsync(phi1, 0, nphi * sizeof(S), stream));
@@ -436,7 +440,13 @@ void Ctx<S>::poisson(S *const *F, S *phi) {
 template <typename S>
 void Ctx<S>::eddy_viscosity(S *const *F) {
     const Dims &d = h.d;
-    if (h.prm.les_model != 1) return;  // nu_t stays 0 (subgrid.f90:41-45)
+    if (h.prm.les_model == 0) return;  // nu_t stays 0 (subgrid.f90:41-45)
+    if (h.prm.les_model == 2) {
+        tic("dynamic_smagorinsky");
+        launch_dsm<S>(d, met, F[0], F[1], F[2], ubar, dsm, NT, h.prm.dirichlet_nu_t, stream);
+        toc();
+        return;
+    }
     tic("smagorinsky");
     launch_smagorinsky<S>(d, met, m_y, m_yg, m_delta, F[0], F[1], F[2], NT, sc, ubar, h.prm.cs, h.prm.nu,
                           h.prm.dirichlet_nu_t, stream);
@@ -459,7 +469,7 @@ void Ctx<S>::wall_model(S *const *F) {
     toc();
     if (surrogate) surrogate(this, F, &pred, &use);
     tic("wall_model");
-    launch_wall_model<S>(d, F[0], F[2], h.prm.les_model == 1 ? NT : nullptr, wt, tau_init, alpha_x, azt,
+    launch_wall_model<S>(d, F[0], F[2], h.prm.les_model != 0 ? NT : nullptr, wt, tau_init, alpha_x, azt,
                          alpha_z, sc, ep, h.prm.nu, hw, h.prm.jref_eqwm, h.prm.tau_mean_eqwm, pred, use,
                          ubar, stream);
     toc();
@@ -480,7 +490,7 @@ template <typename S>
 void Ctx<S>::step_launch() {
     const Dims &d = h.d;
     const double rk[3][3] = {{8.0 / 15.0, 0, 0}, {1.0 / 4.0, 5.0 / 12.0, 0}, {1.0 / 4.0, 0.0, 3.0 / 4.0}};
-    const bool nut = h.prm.les_model == 1 || nut_user;
+    const bool nut = h.prm.les_model != 0 || nut_user;
     for (int s = 1; s <= 3; s++) {
         S *const *Fc = st[cur];
         S *const *Fn = st[1 - cur];
@@ -755,8 +765,18 @@ int Ctx<S>::compute_pressure(double *P) {
     S *const *F = st[cur];
     // avg_nu_t at rows 1, 2, nyg-1, nyg (subgrid.f90:605-617)
     double a1 = 0, a2 = 0, an1 = 0, an = 0;
-    const bool nut = h.prm.les_model == 1 || nut_user;
-    if (e == hipSuccess && nut) {
+    const bool nut = h.prm.les_model != 0 || nut_user;
+    if (e == hipSuccess && h.prm.les_model == 2) {
+        // the dynamic model's avg_nu_t is its plane coefficient <LM>/<MM>
+        // of the last evaluation (subgrid.f90:505-508, 529-535)
+        std::vector<double> av(nyg);
+        e = hipMemcpyAsync(av.data(), dsm.avg, nyg * sizeof(double), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        a1 = av[0];
+        a2 = av[1];
+        an1 = av[nyg - 2];
+        an = av[nyg - 1];
+    } else if (e == hipSuccess && nut) {
         launch_plane_means<S>(d, NT, 2, nyg - 1, means, stream);
         double m2[2];
         e = hipMemcpyAsync(m2, means, sizeof m2, hipMemcpyDeviceToHost, stream);
@@ -1001,7 +1021,7 @@ int Ctx<S>::compute_rhs(const double *U, const double *V, const double *W, doubl
     const size_t nrw = (size_t)(d.nxg - 2) * (d.nyg - 2) * (d.nz - 2);
     double *dr = nullptr;
     HCK(hipMalloc(&dr, (nru + nrv + nrw) * sizeof(double)));
-    const bool nut = h.prm.les_model == 1 || nut_user;
+    const bool nut = h.prm.les_model != 0 || nut_user;
     launch_rhs_only<S, S>(d, met, T[0], T[1], T[2], nut ? NT : nullptr, dr, dr + nru, dr + nru + nrv, sc,
                           ubar, h.prm.nu, h.prm.omega_z, stream);
     hipError_t e = hipGetLastError();
@@ -1017,7 +1037,7 @@ int Ctx<S>::compute_rhs(const double *U, const double *V, const double *W, doubl
 template <typename S>
 int Ctx<S>::compute_eddy_viscosity() {
     HCK(hipSetDevice(dev));
-    if (h.prm.les_model != 1) {
+    if (h.prm.les_model == 0) {
         HCK(hipMemsetAsync(NT, 0, (size_t)h.d.nxg * h.d.nyg * h.d.nzg * sizeof(S), stream));
         if (nut_user) invalidate_graphs();
         nut_user = false;

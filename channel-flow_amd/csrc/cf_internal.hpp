@@ -118,6 +118,7 @@ struct Ctx : CtxBase {
     bool nut_user = false;
     Cx<S> *spec = nullptr;
     double *cscr = nullptr;  // y-solve scratch (lane form only)
+    DsmWork dsm{};           // dynamic Smagorinsky work arrays (les_model 2 only)
     S *phi1 = nullptr, *phi23 = nullptr;
     double *part = nullptr, *stats_out = nullptr;
     double *dtscr = nullptr;

@@ -14,6 +14,18 @@ struct FFTPlan {
 // radix 8 in the plan of the x transforms (the z transforms always use it)
 constexpr bool FFT_X_RADIX8 = true;
 
+// ---- dynamic Smagorinsky (cf_dsm.hip) ----
+struct DsmWork {
+    double *Q;     // 10 centre arrays (nxg*nyg*nzg each): centre velocities, |S|Sij, |S|
+    double *Fu;    // filtered U, V, W on their own grids, back to back
+    double *part;  // dsm_part_elems(d): per-wave plane partials of LM, MM
+    double *avg;   // nyg: <LM>/<MM> per row (rows 1 and nyg are ghosts)
+};
+size_t dsm_part_elems(const Dims &d);
+template <typename S>
+void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W, const double *ubar,
+                const DsmWork &w, S *NT, int dirichlet, hipStream_t st);
+
 // ---- stencils (cf_stencil.hip) ----
 template <typename S, typename R>
 void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,

@@ -39,7 +39,8 @@ typedef struct cf_params {
     double dPdx, dPdz;       /* initial forcing                                   */
     double omega_z;          /* Rossby_plus*utau/2 (input_output.f90:84)         */
     int x_mass_cte, y_mass_cte;
-    int les_model;           /* 0 none, 1 constant-coefficient Smagorinsky       */
+    int les_model;           /* 0 none, 1 constant-coefficient Smagorinsky,
+                                2 dynamic Smagorinsky (subgrid.f90:376-538)     */
     int iwall_model;         /* 0 no-slip, 3 equilibrium wall model               */
     int dirichlet_nu_t;
     int jref_eqwm;

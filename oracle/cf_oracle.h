@@ -32,7 +32,7 @@ typedef struct orc_params {
     double eqwm_tol;       /* eqwm.f90:109 hard-codes 1e-7                   */
     double tau_mean_eqwm;  /* fallback stress (initialization.f90:524)       */
     int x_mass_cte, y_mass_cte;
-    int les_model;         /* 0 none, 1 constant-coefficient Smagorinsky     */
+    int les_model;         /* 0 none, 1 constant-coefficient, 2 dynamic Smagorinsky */
     int iwall_model;       /* 0 no-slip, 3 equilibrium wall model            */
     int dirichlet_nu_t;
     int jref_eqwm;

@@ -160,6 +160,61 @@ def test_smagorinsky_matches():
     assert relerr(g.nu_t, o.nu_t) < 1e-12
 
 
+@pytest.mark.parametrize("cells,dirichlet", [((32, 24, 16), 1), ((36, 22, 18), 0)])
+def test_dynamic_smagorinsky_matches(cells, dirichlet):
+    """sgs_dynamic_Smagorinsky (subgrid.f90:376-538) on a stretched grid.
+    (36, 22, 18): x and z tiles and the y row chunks of the kernels are
+    ragged.  The plane sums are tree-ordered on the GPU: tolerance 1e-11."""
+    cfg = small_cfg(les_model=2, cells=cells, dirichlet_nu_t=dirichlet)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.compute_eddy_viscosity()
+    o.compute_eddy_viscosity()
+    ng, no = g.nu_t, o.nu_t
+    assert np.count_nonzero(no[1:-1, 1:-1, 1:-1]) > 0.1 * no[1:-1, 1:-1, 1:-1].size  # model active
+    assert relerr(ng, no) < 1e-11, relerr(ng, no)
+
+
+@pytest.mark.parametrize("wall", [0, 3])
+def test_dynamic_smagorinsky_steps_match(wall):
+    """Whole RK3 steps with the dynamic model (graph-replayed on the GPU).
+    Without a wall model the FP64 bar 1e-10 holds; with the equilibrium
+    wall model the iterative ODE solve sets the bar (1e-7, as for WMLES)."""
+    kw = dict(iwall_model=3, stretch=0.0, eqwm_ncv=30, eqwm_tol=1e-8, nu=1e-4, cfl=-1e-3,
+              dirichlet_nu_t=0) if wall else dict(dPdx=2e-3)
+    cfg = small_cfg(les_model=2, **kw)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(4, graph=True)
+    o.run(4)
+    tol = 1e-7 if wall else 1e-10
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert relerr(a, b) < tol, relerr(a, b)
+    assert relerr(g.nu_t, o.nu_t) < 100 * tol
+
+
+def test_reference_example_ten_steps():
+    """The reference's own example input (dynamic Smagorinsky, CFL 0.5,
+    constant dPdx): ten RK3 steps at the FP64 bar, divergence-free."""
+    cfg = cases.case_example()
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(10, graph=True)
+    o.run(10)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert relerr(a, b) < 1e-10, relerr(a, b)
+    assert relerr(g.nu_t, o.nu_t) < 1e-8
+    _, dmax = g.check_divergence()
+    assert dmax < 1e-10
+    assert abs(g.scalars["t"] - o.scalars["t"]) < 1e-12 * o.scalars["t"]
+
+
 def test_wall_model_matches():
     cfg = small_cfg(les_model=1, iwall_model=3, stretch=0.0, eqwm_ncv=30, eqwm_tol=1e-8, nu=1e-4)
     g, o = pair(cfg)
@@ -202,14 +257,14 @@ def test_eqwm_batch_matches_oracle():
         assert abs(int(it[q]) - i_o) <= 1
 
 
-@pytest.mark.parametrize("les", [False, True])
+@pytest.mark.parametrize("les", [0, 1, 2])
 def test_actual_pressure_matches_oracle(les):
     """compute_pressure (pressure.f90:21-341) on the same state: the wall
     conditions from V and the plane-averaged eddy viscosity, the divergence
     of the momentum right-hand sides, and the Poisson solve with wall
     sources.  The GPU adds the wall sources before the transforms (the
     reference after them), which changes rounding only: FP64 tolerance 1e-10."""
-    cfg = small_cfg(les_model=1 if les else 0, cs=0.1, dPdx=2e-3)
+    cfg = small_cfg(les_model=les, cs=0.1, dPdx=2e-3)
     g, o = pair(cfg)
     U, V, W = cases.initial_field(cfg)
     g.initialise(U, V, W)
