@@ -28,8 +28,6 @@
 // The reference sums the plane over centres 2..nxg-1 x 2..nzg-1, which holds
 // the periodic images nxg-1 (of 2) and nzg-1 (of 2): cells in the first
 // unique column or plane count twice (four times on both).
-#include <algorithm>
-
 #include "cf_kernels.hpp"
 
 namespace cf {
@@ -86,19 +84,17 @@ __device__ __forceinline__ double fil_xz(int a, int c) {
 // Q layout: 10 centre arrays (nxg, nyg, nzg), rows 2..nyg-1 written:
 //   0..2 centre velocities (interpolate_x/y/z, subgrid.f90:391-393)
 //   3..8 |S| Sij, 9 |S|                     (compute_Sij of the raw field)
+// Block 64 (x) by 4 (y) threads, grid (x tiles, y tiles, nzg).
 // ---------------------------------------------------------------------------
 template <typename S>
 __global__ void __launch_bounds__(DSM_WG)
 k_dsm_prep(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V, const S *__restrict__ W,
            const double *__restrict__ ubar, double *__restrict__ Q, int64_t nC) {
-    const int64_t ncell = (int64_t)d.nxg * (d.nyg - 2) * d.nzg;
-    const int64_t t = (int64_t)blockIdx.x * DSM_WG + threadIdx.x;
-    if (t >= ncell) return;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
-    const int i = 1 + (int)(t % nxg);
-    const int64_t r = t / nxg;
-    const int j = 2 + (int)(r % (nyg - 2));
-    const int k = 1 + (int)(r / (nyg - 2));
+    const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 2 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = 1 + blockIdx.z;
+    if (i > nxg || j > nyg - 1) return;
     const int ii = img_c(i, nxg), kk = img_c(k, d.nzg);
     auto u = [&](int I, int J, int K) {
         return (double)U[i3(nx, nyg, I, J, K)] + (ubar ? ubar[J] : 0.0);
@@ -117,29 +113,23 @@ k_dsm_prep(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V, 
 }
 
 // ---------------------------------------------------------------------------
-// Filtered velocities (subgrid.f90:428-443) on the full grids of U, V, W:
-// blockIdx.y selects the component.  Each point takes the filter at its
-// periodic image; all rows, one-sided y weights on the first and last.
+// Filtered velocities (subgrid.f90:428-443) on the full grid of one of U, V,
+// W (COMP 0, 1, 2).  Each point takes the filter at its periodic image; all
+// rows, one-sided y weights on the first and last.  Block 64 (x) by 4 (y),
+// grid (x tiles, y tiles, n3).
 // ---------------------------------------------------------------------------
-template <typename S>
+template <typename S, int COMP>
 __global__ void __launch_bounds__(DSM_WG)
-k_dsm_filter_vel(Dims d, const S *__restrict__ U, const S *__restrict__ V, const S *__restrict__ W,
-                 const double *__restrict__ ubar, double *__restrict__ Fu) {
-    const int comp = blockIdx.y;
-    const int n1 = comp == 0 ? d.nx : d.nxg, n2 = comp == 1 ? d.ny : d.nyg, n3 = comp == 2 ? d.nz : d.nzg;
-    const int64_t n = (int64_t)n1 * n2 * n3;
-    const int64_t t = (int64_t)blockIdx.x * DSM_WG + threadIdx.x;
-    if (t >= n) return;
-    const int I = 1 + (int)(t % n1);
-    const int64_t r = t / n1;
-    const int J = 1 + (int)(r % n2);
-    const int K = 1 + (int)(r / n2);
-    const int ii = comp == 0 ? img_f(I, n1) : img_c(I, n1);
-    const int kk = comp == 2 ? img_f(K, n3) : img_c(K, n3);
-    const S *F = comp == 0 ? U : (comp == 1 ? V : W);
-    const bool addu = comp == 0 && ubar != nullptr;
+k_dsm_filter_vel(Dims d, const S *__restrict__ F, const double *__restrict__ ubar, double *__restrict__ Fo) {
+    const int n1 = COMP == 0 ? d.nx : d.nxg, n2 = COMP == 1 ? d.ny : d.nyg, n3 = COMP == 2 ? d.nz : d.nzg;
+    const int I = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int K = 1 + blockIdx.z;
+    if (I > n1 || J > n2) return;
+    const int ii = COMP == 0 ? img_f(I, n1) : img_c(I, n1);
+    const int kk = COMP == 2 ? img_f(K, n3) : img_c(K, n3);
     auto f = [&](int a, int b, int c) {
-        return (double)F[i3(n1, n2, a, b, c)] + (addu ? ubar[b] : 0.0);
+        return (double)F[i3(n1, n2, a, b, c)] + ((COMP == 0 && ubar) ? ubar[b] : 0.0);
     };
     double acc = 0.0;
 #pragma unroll
@@ -154,36 +144,7 @@ k_dsm_filter_vel(Dims d, const S *__restrict__ U, const S *__restrict__ V, const
                      1.0 / 6.0 * f(ii + a, J + 1, kk + c);
             acc = acc + tt * fil_xz(a, c);
         }
-    const int64_t off = comp == 0 ? 0 : (comp == 1 ? (int64_t)d.nx * d.nyg * d.nzg
-                                                   : (int64_t)d.nx * d.nyg * d.nzg + (int64_t)d.nxg * d.ny * d.nzg);
-    Fu[off + t] = acc;
-}
-
-// x-z filtered sums of one row r at column (i, k): 3 velocities, 6 products
-// (uu vv ww uv uw vw), 6 |S|Sij
-__device__ __forceinline__ void dsm_row(const double *__restrict__ Q, int64_t nC, int nxg, int nyg, int i, int r,
-                                        int k, double F[15]) {
-#pragma unroll
-    for (int c = 0; c < 15; c++) F[c] = 0.0;
-#pragma unroll
-    for (int a = -1; a <= 1; a++)
-#pragma unroll
-        for (int c = -1; c <= 1; c++) {
-            const double f = fil_xz(a, c);
-            const int64_t q = i3(nxg, nyg, i + a, r, k + c);
-            const double u = Q[q], v = Q[nC + q], w = Q[2 * nC + q];
-            F[0] += f * u;
-            F[1] += f * v;
-            F[2] += f * w;
-            F[3] += f * (u * u);
-            F[4] += f * (v * v);
-            F[5] += f * (w * w);
-            F[6] += f * (u * v);
-            F[7] += f * (u * w);
-            F[8] += f * (v * w);
-#pragma unroll
-            for (int s = 0; s < 6; s++) F[9 + s] += f * Q[(3 + s) * nC + q];
-        }
+    Fo[i3(n1, n2, I, J, K)] = acc;
 }
 
 __device__ __forceinline__ double wave_sum(double x) {
@@ -192,22 +153,32 @@ __device__ __forceinline__ double wave_sum(double x) {
     return x;
 }
 
+// tile of k_dsm_lm: 64 columns in x by 4 in z, with a one-cell halo
+constexpr int LM_TX = 64, LM_TZ = 4, LM_HX = LM_TX + 2, LM_HZ = LM_TZ + 2, LM_NP = LM_HX * LM_HZ;
+constexpr int LM_NQ = 9;  // staged quantities: centre velocities and |S|Sij
+
 // ---------------------------------------------------------------------------
-// Lij, Mij and their plane products.  Block: 64 columns in x by 4 in z; one
-// thread marches DSM_LM_ROWS rows of its column (blockIdx.y selects the row
-// chunk).  part[(cw * nyp + j - 2) * 2 + {0, 1}] = wave sums of LM, MM, with
+// Lij, Mij and their plane products.  Block: 64 columns in x by 4 in z (one
+// wave per z column row); the block marches DSM_LM_ROWS rows (blockIdx.y
+// selects the row chunk).  Each row of the nine staged quantities is loaded
+// once per block into LDS with its halo; every thread forms its x-z filtered
+// sums (products included) from LDS and keeps the sums of the previous two
+// rows for the y combination.
+// part[(cw * nyp + j - 2) * 2 + {0, 1}] = wave sums of LM, MM, with
 // cw = (4 bz + wave) * tiles_x + bx.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(DSM_WG)
+__global__ void __launch_bounds__(DSM_WG) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_dsm_lm(Dims d, Metrics m, const double *__restrict__ Q, const double *__restrict__ Fu, double *__restrict__ part,
          int tiles_x) {
+    __shared__ double sq[LM_NQ][LM_NP];
     const int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg, nx = d.nx, ny = d.ny;
     const int64_t nC = (int64_t)nxg * nyg * nzg;
     const int bx = blockIdx.x % tiles_x, bz = blockIdx.x / tiles_x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = 2 + bx * 64 + lane, k = 2 + bz * 4 + wv;
+    const int i = 2 + bx * LM_TX + lane, k = 2 + bz * LM_TZ + wv;
+    const int ih = 1 + bx * LM_TX, kh = 1 + bz * LM_TZ;  // halo origin
     const bool act = i <= nxg - 2 && k <= nzg - 2;
-    const int cw = (4 * bz + wv) * tiles_x + bx;
+    const int cw = (LM_TZ * bz + wv) * tiles_x + bx;
     const int nyp = nyg - 2;
     const int j0 = 2 + blockIdx.y * DSM_LM_ROWS;
     const int j1 = min(j0 + DSM_LM_ROWS - 1, nyg - 1);
@@ -216,17 +187,59 @@ k_dsm_lm(Dims d, Metrics m, const double *__restrict__ Q, const double *__restri
     auto uf = [&](int I, int J, int K) { return Uf[i3(nx, nyg, I, J, K)]; };
     auto vf = [&](int I, int J, int K) { return Vf[i3(nxg, ny, I, J, K)]; };
     auto wf = [&](int I, int J, int K) { return Wf[i3(nxg, nyg, I, J, K)]; };
+    // stage row r (block-uniform) into LDS; points outside the arrays read 0
+    // and are never used by an active thread
+    auto stage = [&](int r) {
+        __syncthreads();
+        for (int p = threadIdx.x; p < LM_NQ * LM_NP; p += DSM_WG) {
+            const int qa = p / LM_NP, pt = p - qa * LM_NP;
+            const int pz = pt / LM_HX, px = pt - pz * LM_HX;
+            const int I = ih + px, K = kh + pz;
+            sq[qa][pt] = (I <= nxg && K <= nzg) ? Q[qa * nC + i3(nxg, nyg, I, r, K)] : 0.0;
+        }
+        __syncthreads();
+    };
+    // x-z filtered sums of the staged row: 3 velocities, 6 products
+    // (uu vv ww uv uw vw), 6 |S|Sij
+    auto rowsum = [&](double F[15]) {
+#pragma unroll
+        for (int c = 0; c < 15; c++) F[c] = 0.0;
+#pragma unroll
+        for (int a = -1; a <= 1; a++)
+#pragma unroll
+            for (int c = -1; c <= 1; c++) {
+                const double f = fil_xz(a, c);
+                const int pt = (wv + 1 + c) * LM_HX + lane + 1 + a;
+                const double u = sq[0][pt], v = sq[1][pt], w = sq[2][pt];
+                F[0] += f * u;
+                F[1] += f * v;
+                F[2] += f * w;
+                F[3] += f * (u * u);
+                F[4] += f * (v * v);
+                F[5] += f * (w * w);
+                F[6] += f * (u * v);
+                F[7] += f * (u * w);
+                F[8] += f * (v * w);
+#pragma unroll
+                for (int s = 0; s < 6; s++) F[9 + s] += f * sq[3 + s][pt];
+            }
+    };
     double Fm[15], F0[15], Fp[15];
 #pragma unroll
     for (int c = 0; c < 15; c++) { Fm[c] = 0.0; Fp[c] = 0.0; }
-    if (act) {
-        if (j0 > 2) dsm_row(Q, nC, nxg, nyg, i, j0 - 1, k, Fm);
-        dsm_row(Q, nC, nxg, nyg, i, j0, k, F0);
+    if (j0 > 2) {
+        stage(j0 - 1);
+        rowsum(Fm);
     }
+    stage(j0);
+    rowsum(F0);
     for (int j = j0; j <= j1; j++) {
+        if (j < nyg - 1) {
+            stage(j + 1);
+            rowsum(Fp);
+        }
         double lm = 0.0, mm = 0.0;
         if (act) {
-            if (j < nyg - 1) dsm_row(Q, nC, nxg, nyg, i, j + 1, k, Fp);
             const double am = j == 2 ? 0.0 : (j == nyg - 1 ? 1.0 / 3.0 : 1.0 / 6.0);
             const double a0 = 2.0 / 3.0;
             const double ap = j == nyg - 1 ? 0.0 : (j == 2 ? 1.0 / 3.0 : 1.0 / 6.0);
@@ -249,9 +262,9 @@ k_dsm_lm(Dims d, Metrics m, const double *__restrict__ Q, const double *__restri
             mm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + 2.0 * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
             lm *= wgt;
             mm *= wgt;
-#pragma unroll
-            for (int c = 0; c < 15; c++) { Fm[c] = F0[c]; F0[c] = Fp[c]; }
         }
+#pragma unroll
+        for (int c = 0; c < 15; c++) { Fm[c] = F0[c]; F0[c] = Fp[c]; }
         lm = wave_sum(lm);
         mm = wave_sum(mm);
         if (lane == 0) {
@@ -290,19 +303,17 @@ k_dsm_avg(int ncw, int nyp, const double *__restrict__ part, double *__restrict_
 template <typename S>
 __global__ void __launch_bounds__(DSM_WG)
 k_dsm_nut(Dims d, const double *__restrict__ Q, double *__restrict__ avg, S *__restrict__ NT, int dirichlet) {
-    const int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
-    const int64_t nC = (int64_t)nxg * nyg * nzg;
-    const int64_t t = (int64_t)blockIdx.x * DSM_WG + threadIdx.x;
-    if (t >= nC) return;
-    const int i = 1 + (int)(t % nxg);
-    const int64_t r = t / nxg;
-    const int j = 1 + (int)(r % nyg);
-    const int k = 1 + (int)(r / nyg);
+    const int nxg = d.nxg, nyg = d.nyg;
+    const int64_t nC = (int64_t)nxg * nyg * d.nzg;
+    const int i = 1 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 1 + blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = 1 + blockIdx.z;
+    if (i > nxg || j > nyg) return;
     const int jj = j == 1 ? 2 : (j == nyg ? nyg - 1 : j);
     const double sgn = (jj != j && dirichlet) ? -1.0 : 1.0;
     const double val = fmax(-0.5 * avg[jj - 1] * Q[9 * nC + i3(nxg, nyg, i, jj, k)], 0.0);
-    NT[t] = (S)(sgn * val);
-    if (t == 0) {
+    NT[i3(nxg, nyg, i, j, k)] = (S)(sgn * val);
+    if (i == 1 && j == 1 && k == 1) {
         avg[0] = dirichlet ? -avg[1] : avg[1];
         avg[nyg - 1] = dirichlet ? -avg[nyg - 2] : avg[nyg - 2];
     }
@@ -321,20 +332,19 @@ template <typename S>
 void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W, const double *ubar,
                 const DsmWork &w, S *NT, int dirichlet, hipStream_t st) {
     const int64_t nC = (int64_t)d.nxg * d.nyg * d.nzg;
-    const int64_t nprep = (int64_t)d.nxg * (d.nyg - 2) * d.nzg;
-    hipLaunchKernelGGL((k_dsm_prep<S>), dim3((unsigned)((nprep + DSM_WG - 1) / DSM_WG)), dim3(DSM_WG), 0, st, d, m,
-                       U, V, W, ubar, w.Q, nC);
-    const int64_t nmax = std::max((int64_t)d.nx * d.nyg * d.nzg,
-                                  std::max((int64_t)d.nxg * d.ny * d.nzg, (int64_t)d.nxg * d.nyg * d.nz));
-    hipLaunchKernelGGL((k_dsm_filter_vel<S>), dim3((unsigned)((nmax + DSM_WG - 1) / DSM_WG), 3), dim3(DSM_WG), 0, st,
-                       d, U, V, W, ubar, w.Fu);
+    auto g3 = [](int n1, int n2, int n3) { return dim3((n1 + 63) / 64, (n2 + 3) / 4, n3); };
+    hipLaunchKernelGGL((k_dsm_prep<S>), g3(d.nxg, d.nyg - 2, d.nzg), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q,
+                       nC);
+    double *Uf = w.Fu, *Vf = Uf + (int64_t)d.nx * d.nyg * d.nzg, *Wf = Vf + (int64_t)d.nxg * d.ny * d.nzg;
+    hipLaunchKernelGGL((k_dsm_filter_vel<S, 0>), g3(d.nx, d.nyg, d.nzg), dim3(DSM_WG), 0, st, d, U, ubar, Uf);
+    hipLaunchKernelGGL((k_dsm_filter_vel<S, 1>), g3(d.nxg, d.ny, d.nzg), dim3(DSM_WG), 0, st, d, V, ubar, Vf);
+    hipLaunchKernelGGL((k_dsm_filter_vel<S, 2>), g3(d.nxg, d.nyg, d.nz), dim3(DSM_WG), 0, st, d, W, ubar, Wf);
     const int tx = dsm_tiles_x(d), tz = dsm_tiles_z(d);
     const int nyp = d.nyg - 2;
     const int chunks = (nyp + DSM_LM_ROWS - 1) / DSM_LM_ROWS;
     hipLaunchKernelGGL(k_dsm_lm, dim3(tx * tz, chunks), dim3(DSM_WG), 0, st, d, m, w.Q, w.Fu, w.part, tx);
     hipLaunchKernelGGL(k_dsm_avg, dim3(nyp), dim3(DSM_WG), 0, st, tx * tz * 4, nyp, w.part, w.avg);
-    hipLaunchKernelGGL((k_dsm_nut<S>), dim3((unsigned)((nC + DSM_WG - 1) / DSM_WG)), dim3(DSM_WG), 0, st, d, w.Q,
-                       w.avg, NT, dirichlet);
+    hipLaunchKernelGGL((k_dsm_nut<S>), g3(d.nxg, d.nyg, d.nzg), dim3(DSM_WG), 0, st, d, w.Q, w.avg, NT, dirichlet);
 }
 
 template void launch_dsm<double>(const Dims &, const Metrics &, const double *, const double *, const double *,
