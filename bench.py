@@ -125,13 +125,17 @@ def cpu_baseline(cells, steps_cap_s=30.0):
 def other_configs(steps=50, warmup=5):
     """Step time of the other workloads (graph-replayed, inputs resident):
     64^3 DNS (FP64, fixed dt), 128x64x128 wall-modelled LES (Smagorinsky +
-    EQWM, mixed and FP64) and 256x96x256 wall-modelled LES at Re_tau 5200
-    (mixed, EQWM)."""
+    EQWM, mixed and FP64), 256x96x256 wall-modelled LES at Re_tau 5200
+    (mixed, EQWM), the reference's example input (32x40x32 cells, dynamic
+    Smagorinsky, FP64) and the 128x64x128 LES with the dynamic model (mixed)."""
     from channel_flow import cases
     from channel_flow.solver import ChannelFlow
     res = {}
+    dsm128 = cases.case_c3()
+    dsm128.les_model = 2
     todo = [("dns64_fp64", cases.case_c1(), "fp64"), ("wmles128_mixed", cases.case_c3(), "mixed"),
-            ("wmles128_fp64", cases.case_c3(), "fp64"), ("wmles256_mixed", cases.case_c4(), "mixed")]
+            ("wmles128_fp64", cases.case_c3(), "fp64"), ("wmles256_mixed", cases.case_c4(), "mixed"),
+            ("example_dsm_fp64", cases.case_example(), "fp64"), ("wmles128_dsm_mixed", dsm128, "mixed")]
     for name, cfg, prec in todo:
         cfg.precision = prec
         cfg.wall_surrogate = 0
@@ -214,6 +218,39 @@ def measured_copy_gbps(mib=1024, reps=10):
     return 2.0 * n * 8 / (ms / 1e3) / 1e9
 
 
+def dsm_steps(cells, steps, warmup=2):
+    """The dynamic Smagorinsky model (subgrid.f90:376-538) on the DNS
+    workload's grid: RK3 steps with les_model 2 (one model evaluation per
+    substep), graph-replayed, and the model's share from HIP events."""
+    from channel_flow import cases
+    from channel_flow.solver import ChannelFlow
+    cfg = cases.case_c2("fp64")
+    cfg.cells = cells
+    cfg.les_model = 2
+    g = ChannelFlow(cfg, device=0)
+    g.initialise(*cases.initial_field(cfg))
+    g.run(warmup, graph=True)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(steps, graph=True)
+    g.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    g.timing(True)
+    g.timing_reset()
+    g.run(steps, graph=False)
+    g.sync()
+    rep = g.timing_report()
+    g.close()
+    n = cells[0] * cells[1] * cells[2]
+    ev = rep.get("dynamic_smagorinsky", (None, 0))
+    log("dsm %s: %.3f ms/step, %.3f ms per model evaluation" % (cells, ms, ev[0] or float("nan")))
+    # algorithmic bytes of one evaluation: read U, V, W; write nu_t
+    ab = 4 * n * 8
+    return {"cells": list(cells), "ms_per_step": round(ms, 4), "mcell_per_s": round(n / ms / 1e3, 1),
+            "model_ms_per_eval": round(ev[0], 4) if ev[0] else None, "evals": ev[1],
+            "model_GBps_on_alg_bytes": round(ab / (ev[0] / 1e3) / 1e9, 1) if ev[0] else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +268,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB five times (counter calibration)")
     ap.add_argument("--copy-ceiling", action="store_true", help="only measure a 1 GiB device copy")
+    ap.add_argument("--dsm-only", action="store_true",
+                    help="only time RK3 steps with the dynamic Smagorinsky model on --cells")
     a = ap.parse_args()
     if a.copy_ceiling:
         print(json.dumps({"copy_GBps": measured_copy_gbps()}))
@@ -244,6 +283,9 @@ def main():
         sys.exit(2)
     from channel_flow import cases
     cells = tuple(int(c) for c in a.cells.split(","))
+    if a.dsm_only:
+        print(json.dumps({"dsm": dsm_steps(cells, a.steps, a.warmup)}))
+        return
     ncell = cells[0] * cells[1] * cells[2]
     mh_ratio = (cells[0] // 2 + 1) * 2.0 / cells[0]
 
@@ -316,6 +358,7 @@ def main():
         out["serving"] = serving_bench(a.serve_scale, a.serve_slots, a.serve_budget_gb)
     if not a.no_others:
         out["other_configs"] = other_configs()
+        out["other_configs"]["dns384_dsm_fp64"] = dsm_steps(cells, 5)
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cells)
