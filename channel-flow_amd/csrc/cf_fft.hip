@@ -330,18 +330,24 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 }
 
 // ---------------------------------------------------------------------------
-// Tridiagonal solve in y, one lane per (kx,kz) mode (the forward sweep and
-// back substitution of projection.f90:90-113 in FP64 arithmetic).  Rows go
-// in blocks of LB: the next block's spectrum values are loaded while the
-// current block is eliminated, d' is stored in the spectrum, and c' is kept
-// only at the last row of each block (a checkpoint).  The back substitution
+// Tridiagonal solve in y, two lanes per (kx,kz) mode: the real and the
+// imaginary part of the right-hand side share the matrix, so each lane runs
+// the forward sweep and back substitution of projection.f90:90-113 for one
+// of them (FP64 arithmetic), recomputing the same pivots.  Rows go in blocks
+// of LB: the next block's values are loaded while the current block is
+// eliminated, d' is stored in the spectrum, and c' is kept only at the last
+// row of each block (a checkpoint, one per lane).  The back substitution
 // recomputes a block's c' from the previous block's checkpoint -- the same
 // operations on the same values, so the same c' -- and walks the block
-// backwards.  Two passes over the spectrum and 1/LB of a pass of FP64
-// scratch, no serial phase: the faster form whenever the modes fill the GPU.
+// backwards.  Two passes over the spectrum and 2/LB of a pass of FP64
+// scratch, no serial phase: the faster form whenever the modes fill the GPU
+// (the sweep is latency bound, and two lanes per mode double the waves in
+// flight).
 // ---------------------------------------------------------------------------
 constexpr int TH_LB = 16;
-size_t thomas_scratch_elems(const Dims &d) { return (size_t)((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp; }
+size_t thomas_scratch_elems(const Dims &d) {
+    return (size_t)2 * ((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp;
+}
 
 // pivot reciprocal 1 / (D(j) - dl(j-1) c'(j-1)); c'(j) = du(j) times it
 __device__ __forceinline__ double thomas_pivot(double D, double lo, double cprev) { return 1.0 / (D - lo * cprev); }
@@ -357,28 +363,30 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
     for (int e = threadIdx.x; e < nyp; e += blockDim.x) { s_dd[e] = dd[e]; s_dl[e] = dl[e]; s_du[e] = du[e]; }
     __syncthreads();
     const int nmodes = d.mh * nzp;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nmodes) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;  // 2 q + part
+    if (t >= 2 * nmodes) return;
+    const int q = t >> 1;
     const int kx = q / nzp, kz = q - kx * nzp;
     const double kx2 = kxx[kx], kz2 = kzz[kz];
     const bool zm = kx == 0 && kz == 0;
-    const int64_t js = nmodes;
-    Cx<T> *col = spec + q;
+    const int64_t js = 2 * (int64_t)nmodes;          // row stride in T
+    T *col = reinterpret_cast<T *>(spec) + t;        // this lane's part of mode q, row 0
     const int nb = (nyp + TH_LB - 1) / TH_LB;
+    const int64_t cs = 2 * (int64_t)nmodes;          // checkpoint stride per block
     auto Dj = [&](int j) {
         const double D = (s_dd[j] + kx2) + kz2;
         return (j == 0 && zm) ? 1.5 * D : D;
     };
-    Cx<T> cur[TH_LB], nxt[TH_LB];
-    auto loadblk = [&](int b, Cx<T> *dst) {
+    T cur[TH_LB], nxt[TH_LB];
+    auto loadblk = [&](int b, T *dst) {
 #pragma unroll
         for (int u = 0; u < TH_LB; u++) {
             const int j = b * TH_LB + u;
-            dst[u] = j < nyp ? col[j * js] : Cx<T>{(T)0, (T)0};
+            dst[u] = j < nyp ? col[j * js] : (T)0;
         }
     };
     // forward sweep
-    double cp = 0.0, dpx = 0.0, dpy = 0.0;
+    double cp = 0.0, dp = 0.0;
     loadblk(0, cur);
     for (int b = 0; b < nb; b++) {
         if (b + 1 < nb) loadblk(b + 1, nxt);
@@ -388,25 +396,24 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
             if (j < nyp) {
                 const double lo = j > 0 ? s_dl[j - 1] : 0.0;
                 const double r = thomas_pivot(Dj(j), lo, cp);
-                dpx = ((double)cur[u].x - lo * dpx) * r;
-                dpy = ((double)cur[u].y - lo * dpy) * r;
-                col[j * js] = Cx<T>{(T)dpx, (T)dpy};
+                dp = ((double)cur[u] - lo * dp) * r;
+                col[j * js] = (T)dp;
                 cp = (j < nyp - 1) ? s_du[j] * r : 0.0;
             }
         }
-        ckpt[(int64_t)b * nmodes + q] = cp;
+        ckpt[(int64_t)b * cs + t] = cp;
 #pragma unroll
         for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
     }
     // back substitution, block by block from the top
-    double xr = dpx, xi = dpy;  // row nyp-1
+    double x = dp;  // row nyp-1
     loadblk(nb - 1, cur);
-    double cnext = nb >= 2 ? ckpt[(int64_t)(nb - 2) * nmodes + q] : 0.0;
+    double cnext = nb >= 2 ? ckpt[(int64_t)(nb - 2) * cs + t] : 0.0;
     for (int b = nb - 1; b >= 0; b--) {
         const double cstart = cnext;  // c' at the row before the block
         if (b > 0) {
             loadblk(b - 1, nxt);
-            cnext = b >= 2 ? ckpt[(int64_t)(b - 2) * nmodes + q] : 0.0;
+            cnext = b >= 2 ? ckpt[(int64_t)(b - 2) * cs + t] : 0.0;
         }
         double c[TH_LB];
         double cprev = cstart;
@@ -425,9 +432,8 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
         for (int u = TH_LB - 1; u >= 0; u--) {
             const int j = b * TH_LB + u;
             if (j < nyp - 1) {
-                xr = (double)cur[u].x - c[u] * xr;
-                xi = (double)cur[u].y - c[u] * xi;
-                col[j * js] = Cx<T>{(T)xr, (T)xi};
+                x = (double)cur[u] - c[u] * x;
+                col[j * js] = (T)x;
             }
         }
 #pragma unroll
@@ -827,8 +833,8 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
                    const double *dd, const double *dl, const double *du, hipStream_t st) {
     const int nmodes = d.mh * d.nzp;
     if (cscr && thomas_uses_scratch((int)sizeof(Cx<T>), nmodes)) {
-        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((nmodes + 255) / 256), dim3(256), 3 * d.nyp * sizeof(double), st,
-                           d, spec, cscr, kxx, kzz, dd, dl, du);
+        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((2 * nmodes + 255) / 256), dim3(256), 3 * d.nyp * sizeof(double),
+                           st, d, spec, cscr, kxx, kzz, dd, dl, du);
         return;
     }
     int KZ, P;

@@ -78,3 +78,19 @@ def test_fortran_driver_matches_python_host(tmp_path):
         assert np.max(np.abs(snap2[name] - a)) <= 1e-12 * np.max(np.abs(a))
     _, dmax = g.check_divergence()
     assert dmax < 1e-10
+
+
+def test_fortran_driver_runs_reference_example(tmp_path):
+    """The reference's example input (dynamic Smagorinsky, Neumann nu_t,
+    CFL 0.5, constant dPdx) through the Fortran driver: its snapshot equals
+    the Python host's run of the same case."""
+    from channel_flow.solver import ChannelFlow
+    cfg = cases.case_example()
+    out = run_driver(tmp_path, input_text(cfg, 5, 5, 5, 5, 1, "dummy", "ex"))
+    assert "step" in out
+    snap = read_snapshot(tmp_path / "ex.5")
+    g = ChannelFlow(cfg)
+    g.initialise(*cases.initial_field(cfg))
+    g.run(5, graph=True)
+    for name, a in zip("UVW", g.get_state()):
+        assert np.max(np.abs(snap[name] - a)) <= 1e-12 * np.max(np.abs(a))
