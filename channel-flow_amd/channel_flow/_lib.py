@@ -139,3 +139,11 @@ def iptr(a):
 
 def fptr(a):
     return a.ctypes.data_as(_fp)
+
+
+def precision_code(name):
+    """cf_params.precision of a case's precision name: 0 "fp64", 1 "mixed"."""
+    codes = {"fp64": 0, "mixed": 1}
+    if name not in codes:
+        raise ValueError("precision must be 'fp64' or 'mixed', not %r" % (name,))
+    return codes[name]

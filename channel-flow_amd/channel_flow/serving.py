@@ -20,7 +20,7 @@ def case_params(cfg: CaseConfig, device=0):
     x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in cfg.grid())
     p = _lib.CfParams()
     p.nx, p.ny, p.nz = len(x), len(y), len(z)
-    p.precision = 0 if cfg.precision == "fp64" else 1
+    p.precision = _lib.precision_code(cfg.precision)
     p.cfl, p.nu, p.dPdx, p.dPdz = cfg.cfl, cfg.nu, cfg.dPdx, cfg.dPdz
     p.omega_z = cfg.omega_z
     p.x_mass_cte, p.y_mass_cte = cfg.x_mass_cte, cfg.y_mass_cte

@@ -42,7 +42,7 @@ class ChannelFlow:
         self.nxg, self.nyg, self.nzg = nx + 1, ny + 1, nz + 1
         p = _lib.CfParams()
         p.nx, p.ny, p.nz = nx, ny, nz
-        p.precision = 0 if cfg.precision == "fp64" else 1
+        p.precision = _lib.precision_code(cfg.precision)
         p.cfl, p.nu, p.dPdx, p.dPdz = cfg.cfl, cfg.nu, cfg.dPdx, cfg.dPdz
         p.omega_z = cfg.omega_z
         p.x_mass_cte, p.y_mass_cte = cfg.x_mass_cte, cfg.y_mass_cte

@@ -182,6 +182,12 @@ std::string Host::validate() const {
     if (p.nx % 2 || p.nz % 2) return "nx and nz must be even (initialization.f90:54-55)";
     if ((int64_t)d.nxg * d.nyg * d.nzg > (int64_t)2000000000) return "grid too large";
     if (d.nxp / 2 > 1024 || d.nzp > 2048) return "FFT length too large";
+    {
+        std::string ferr;
+        make_plan(d.nxp / 2, ferr, FFT_X_RADIX8);
+        make_plan(d.nzp, ferr, true);
+        if (!ferr.empty()) return ferr;
+    }
     if (!thomas_supported(d)) return "too many y cells for the y solve";
     if (p.precision != 0 && p.precision != 1) return "precision must be 0 (fp64) or 1 (mixed)";
     if (p.les_model < 0 || p.les_model > 2) return "les_model must be 0, 1 or 2";

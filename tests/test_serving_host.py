@@ -46,3 +46,40 @@ def test_ensemble_is_seeded_and_mixed():
     assert all(200 <= n <= 2000 for _, n, _ in a)
     s = ensemble_workload(n=8, seed=5, steps_scale=0.1)
     assert all(20 <= n <= 200 for _, n, _ in s)
+
+
+def test_footprint_of_the_dynamic_model():
+    """les_model 2 adds the model's work arrays: 16 centre fields, the
+    filtered velocities and the plane partials."""
+    c = cases.case_example()
+    s = cases.case_example()
+    s.les_model = 1
+    NX, NY, NZ = c.cells
+    ncent = (NX + 3) * (NY + 2) * (NZ + 3)
+    assert footprint(c) - footprint(s) >= 8 * (16 * ncent) + _field_bytes(c, 8)
+
+
+def test_invalid_parameters_are_rejected():
+    """cf_create's checks, exercised through the dry footprint pass: each
+    bad parameter gives an error naming it."""
+    import pytest
+
+    def err(**kw):
+        c = cases.case_c1()
+        c.cells = (32, 24, 16)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        with pytest.raises(RuntimeError) as e:
+            footprint(c)
+        return str(e.value)
+
+    assert "les_model" in err(les_model=3)
+    assert "iwall_model" in err(iwall_model=2)
+    assert "even" in err(cells=(35, 24, 16))
+    assert "FFT" in err(cells=(30, 24, 14 * 5))
+    assert "nu" in err(nu=0.0)
+    assert "eqwm_ncv" in err(iwall_model=3, eqwm_ncv=500)
+    c = cases.case_c1()
+    c.precision = "fp16"
+    with pytest.raises(ValueError):
+        footprint(c)
