@@ -213,10 +213,11 @@ constexpr int LM_NQ = 9;  // staged quantities: centre velocities and |S|Sij
 // ---------------------------------------------------------------------------
 // Lij, Mij and their plane products.  Block: 64 columns in x by 4 in z (one
 // wave per z column row); the block marches DSM_LM_ROWS rows of one row
-// chunk.  Each row of the nine staged quantities is loaded
-// once per block into LDS with its halo; every thread forms its x-z filtered
-// sums (products included) from LDS and keeps the sums of the previous two
-// rows for the y combination.
+// chunk.  Each row of the nine staged quantities is loaded once per block
+// into LDS with its halo; the global loads of the next row are issued into
+// registers before the current row is computed and written to LDS after it.
+// Every thread forms its x-z filtered sums (products included) from LDS and
+// folds them into the streaming y combination below.
 // part[(cw * nyp + j - 2) * 2 + {0, 1}] = wave sums of LM, MM, with
 // cw = (4 bz + wave) * tiles_x + bx.
 // ---------------------------------------------------------------------------
@@ -239,13 +240,31 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
     const double wgt = (i == 2 ? 2.0 : 1.0) * (k == 2 ? 2.0 : 1.0);
     // stage row r (block-uniform) into LDS; points outside the arrays read 0
     // and are never used by an active thread
-    auto stage = [&](int r) {
+    constexpr int NPF = (LM_NQ * LM_NP + DSM_WG - 1) / DSM_WG;
+    double pf[NPF];
+    // global loads of row r into registers (block-uniform r)
+    auto fetch = [&](int r) {
+#pragma unroll
+        for (int u = 0; u < NPF; u++) {
+            const int p = threadIdx.x + u * DSM_WG;
+            pf[u] = 0.0;
+            if (p < LM_NQ * LM_NP) {
+                const int qa = p / LM_NP, pt = p - qa * LM_NP;
+                const int pz = pt / LM_HX, px = pt - pz * LM_HX;
+                const int I = ih + px, K = kh + pz;
+                if (I <= nxg && K <= nzg) pf[u] = Q[qa * nC + i3(nxg, nyg, I, r, K)];
+            }
+        }
+    };
+    auto commit = [&]() {
         __syncthreads();
-        for (int p = threadIdx.x; p < LM_NQ * LM_NP; p += DSM_WG) {
-            const int qa = p / LM_NP, pt = p - qa * LM_NP;
-            const int pz = pt / LM_HX, px = pt - pz * LM_HX;
-            const int I = ih + px, K = kh + pz;
-            sq[qa][pt] = (I <= nxg && K <= nzg) ? Q[qa * nC + i3(nxg, nyg, I, r, K)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < NPF; u++) {
+            const int p = threadIdx.x + u * DSM_WG;
+            if (p < LM_NQ * LM_NP) {
+                const int qa = p / LM_NP, pt = p - qa * LM_NP;
+                sq[qa][pt] = pf[u];
+            }
         }
         __syncthreads();
     };
@@ -262,9 +281,12 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
 #pragma unroll
     for (int c = 0; c < 15; c++) { P1[c] = 0.0; P2[c] = 0.0; }
     const int rs = j0 > 2 ? j0 - 1 : j0;
+    const int rlast = min(j1 + 1, nyg - 1);  // last row to stage
+    fetch(rs);
+    commit();
     for (int r = rs; r <= j1 + 1; r++) {
         const bool have = r <= nyg - 1;  // block-uniform
-        if (have) stage(r);
+        if (r + 1 <= rlast) fetch(r + 1);
         const int jd = r - 1;            // row completed by this one
         const bool use = act && jd >= j0 && jd <= j1;
         const double wap = use ? ap(jd) : 0.0, wam = am(r + 1);
@@ -305,6 +327,7 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
             lm += wc * (L * M);
             mm += wc * (M * M);
         }
+        if (r + 1 <= rlast) commit();
         if (jd >= j0 && jd <= j1) {  // block-uniform
             lm = use ? lm * wgt : 0.0;
             mm = use ? mm * wgt : 0.0;
