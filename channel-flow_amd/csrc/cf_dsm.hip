@@ -15,17 +15,17 @@
 // (the mixed path): Lij is a difference of nearly equal filtered products.
 //
 // Kernels, in launch order:
-//   k_dsm_prep       centre velocities, |S| and |S|Sij of the unfiltered
-//                    field at every centre (ghost columns take the values of
-//                    their periodic images)                     -> Q[0..9]
-//   k_dsm_filter_vel filtered U, V, W on their own grids, all rows  -> Fu
-//   k_dsm_fstrain    4 |hat S| hat(S)ij from Fu                 -> Q[10..15]
-//   k_dsm_lm         per cell Lij and Mij from the LDS-staged rows of
-//                    Q[0..8]; LM and MM summed over each wave's 64 cells of
-//                    a row                                      -> part
-//   k_dsm_avg        one block per row: fixed-order sum of the wave partials,
-//                    C(y) = LM/MM                               -> avg
-//   k_dsm_nut        nu_t at every centre, wall ghosts, avg ghosts
+//   k_dsm_strain<RAW>  centre velocities, |S| and |S|Sij of the field at
+//                      every centre (ghost columns and planes are copies of
+//                      their periodic images)              -> Q[0..9]
+//   k_dsm_filter_vel   filtered U, V, W on their own grids, all rows -> Fu
+//   k_dsm_strain<!RAW> 4 |hat S| hat(S)ij from Fu         -> Q[10..15]
+//   k_dsm_lm           per cell Lij and Mij from the LDS-staged rows of
+//                      Q[0..8]; LM and MM summed over each wave's 64 cells
+//                      of a row                            -> part
+//   k_dsm_avg          one block per row: fixed-order sum of the wave
+//                      partials, C(y) = LM/MM              -> avg
+//   k_dsm_nut          nu_t at every centre, wall ghosts, avg ghosts
 //
 // The reference sums the plane over centres 2..nxg-1 x 2..nzg-1, which holds
 // the periodic images nxg-1 (of 2) and nzg-1 (of 2): cells in the first
@@ -82,40 +82,6 @@ __device__ __forceinline__ double strain6(const Metrics &m, int i, int j, int k,
 // x-z Simpson weights fil(a, c), a, c in {-1, 0, 1}
 __device__ __forceinline__ double fil_xz(int a, int c) {
     return (a == 0 && c == 0) ? 4.0 / 9.0 : ((a == 0 || c == 0) ? 1.0 / 9.0 : 1.0 / 36.0);
-}
-
-// ---------------------------------------------------------------------------
-// Q layout: 10 centre arrays (nxg, nyg, nzg), rows 2..nyg-1 written:
-//   0..2 centre velocities (interpolate_x/y/z, subgrid.f90:391-393)
-//   3..8 |S| Sij, 9 |S|                     (compute_Sij of the raw field)
-// Block 64 (x) by 4 (y) threads over tiles (x, y, k), k fastest, so that
-// the blocks of one XCD sweep neighbouring k planes of one (x, y) tile.
-// ---------------------------------------------------------------------------
-template <typename S>
-__global__ void __launch_bounds__(DSM_WG)
-k_dsm_prep(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V, const S *__restrict__ W,
-           const double *__restrict__ ubar, double *__restrict__ Q, int64_t nC, int tiles_x, int ntiles) {
-    const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
-    const int t = xcd_remap(blockIdx.x, ntiles);
-    const int k = 1 + t % d.nzg, r = t / d.nzg;
-    const int i = 1 + (r % tiles_x) * 64 + (threadIdx.x & 63);
-    const int j = 2 + (r / tiles_x) * 4 + (threadIdx.x >> 6);
-    if (i > nxg || j > nyg - 1) return;
-    const int ii = img_c(i, nxg), kk = img_c(k, d.nzg);
-    auto u = [&](int I, int J, int K) {
-        return (double)U[i3(nx, nyg, I, J, K)] + (ubar ? ubar[J] : 0.0);
-    };
-    auto v = [&](int I, int J, int K) { return (double)V[i3(nxg, ny, I, J, K)]; };
-    auto w = [&](int I, int J, int K) { return (double)W[i3(nxg, nyg, I, J, K)]; };
-    double s[6];
-    const double sm = strain6(m, ii, j, kk, u, v, w, s);
-    const int64_t q = i3(nxg, nyg, i, j, k);
-    Q[q] = 0.5 * (u(ii - 1, j, kk) + u(ii, j, kk));
-    Q[nC + q] = 0.5 * (v(ii, j - 1, kk) + v(ii, j, kk));
-    Q[2 * nC + q] = 0.5 * (w(ii, j, kk - 1) + w(ii, j, kk));
-#pragma unroll
-    for (int c = 0; c < 6; c++) Q[(3 + c) * nC + q] = sm * s[c];
-    Q[9 * nC + q] = sm;
 }
 
 // ---------------------------------------------------------------------------
@@ -182,28 +148,118 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // ---------------------------------------------------------------------------
-// Strain of the filtered field (compute_Sij of Uf, Vf, Wf, subgrid.f90:446)
-// at the unique interior centres: T[c] = 4 |hat S| hat(S)c, the first term
-// of Mij (fil_size**2 = 4), stored in Q arrays 10..15.
+// compute_Sij (subgrid.f90:891-949) on a 2.5-D march: a block owns 64 x 4
+// unique centres in (x, y) and walks SK_PLANES z planes; U, V, W sit in an
+// LDS ring of three planes (z-1, z, z+1) with a one-cell x-y halo, the next
+// plane's loads issued into registers while the current plane is computed.
+// Every input value is read from HBM about once (plus the halo).
+//   RAW (the field, FP32 deviations + ubar or FP64): centre velocities,
+//     |S|Sij and |S| into Q[0..9] at the centre and at its periodic ghost
+//     copies (columns 1, nxg-1, nxg and planes 1, nzg-1, nzg).
+//   !RAW (the filtered field, FP64): 4 |hat S| hat(S)ij into Q[10..15] at
+//     the unique centres (fil_size**2 = 4, subgrid.f90:449-451).
+// Tiles (x, y, z chunk), z chunk fastest.
+// Q layout: 16 centre arrays (nxg, nyg, nzg), rows 2..nyg-1 written:
+//   0..2 centre velocities (interpolate_x/y/z, subgrid.f90:391-393)
+//   3..8 |S| Sij, 9 |S|       (the raw field; every centre)
+//   10..15 4 |hat S| hat(S)ij (the filtered field; unique centres only)
 // ---------------------------------------------------------------------------
+constexpr int SK_PLANES = 16;
+constexpr int SK_HX = 66, SK_HY = 6, SK_NP = SK_HX * SK_HY;
+
+template <typename T, bool RAW>
 __global__ void __launch_bounds__(DSM_WG)
-k_dsm_fstrain(Dims d, Metrics m, const double *__restrict__ Fu, double *__restrict__ Q, int64_t nC, int tiles_x,
-              int ntiles) {
-    const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nzp = d.nzg - 3;
+k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
+             const double *__restrict__ ubar, double *__restrict__ Q, int64_t nC, int tiles_x, int nchunks,
+             int ntiles) {
+    __shared__ double ring[3][3][SK_NP];  // [array][plane slot][point]
+    const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nz = d.nz, nzg = d.nzg;
     const int t = xcd_remap(blockIdx.x, ntiles);
-    const int k = 2 + t % nzp, r = t / nzp;
-    const int i = 2 + (r % tiles_x) * 64 + (threadIdx.x & 63);
-    const int j = 2 + (r / tiles_x) * 4 + (threadIdx.x >> 6);
-    if (i > nxg - 2 || j > nyg - 1) return;
-    const double *Uf = Fu, *Vf = Fu + (int64_t)nx * nyg * d.nzg, *Wf = Vf + (int64_t)nxg * ny * d.nzg;
-    auto uf = [&](int I, int J, int K) { return Uf[i3(nx, nyg, I, J, K)]; };
-    auto vf = [&](int I, int J, int K) { return Vf[i3(nxg, ny, I, J, K)]; };
-    auto wf = [&](int I, int J, int K) { return Wf[i3(nxg, nyg, I, J, K)]; };
-    double sf[6];
-    const double smf = strain6(m, i, j, k, uf, vf, wf, sf);
-    const int64_t q = i3(nxg, nyg, i, j, k);
+    const int ch = t % nchunks, r = t / nchunks;
+    const int bx = r % tiles_x, by = r / tiles_x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ih = 1 + bx * 64, jh = 1 + by * 4;  // halo origin
+    const int i = ih + 1 + lane, j = jh + 1 + wv;
+    const bool act = i <= nxg - 2 && j <= nyg - 1;
+    const int k0 = 2 + ch * SK_PLANES, k1 = min(k0 + SK_PLANES - 1, nzg - 2);
+    // cooperative load of plane K of the three arrays: 3 * SK_NP values
+    constexpr int NPF = (3 * SK_NP + DSM_WG - 1) / DSM_WG;
+    double pf[NPF];
+    auto fetch = [&](int K) {
 #pragma unroll
-    for (int c = 0; c < 6; c++) Q[(10 + c) * nC + q] = 4.0 * smf * sf[c];
+        for (int u = 0; u < NPF; u++) {
+            const int p = threadIdx.x + u * DSM_WG;
+            pf[u] = 0.0;
+            if (p < 3 * SK_NP) {
+                const int a = p / SK_NP, pt = p - a * SK_NP;
+                const int py = pt / SK_HX, px = pt - py * SK_HX;
+                const int I = ih + px, J = jh + py;
+                if (a == 0) {
+                    if (I <= nx && J <= nyg && K <= nzg)
+                        pf[u] = (double)U[i3(nx, nyg, I, J, K)] + ((RAW && ubar) ? ubar[J] : 0.0);
+                } else if (a == 1) {
+                    if (I <= nxg && J <= ny && K <= nzg) pf[u] = (double)V[i3(nxg, ny, I, J, K)];
+                } else {
+                    if (I <= nxg && J <= nyg && K <= nz) pf[u] = (double)W[i3(nxg, nyg, I, J, K)];
+                }
+            }
+        }
+    };
+    auto commit = [&](int K) {
+        __syncthreads();
+        const int sl = K % 3;
+#pragma unroll
+        for (int u = 0; u < NPF; u++) {
+            const int p = threadIdx.x + u * DSM_WG;
+            if (p < 3 * SK_NP) {
+                const int a = p / SK_NP, pt = p - a * SK_NP;
+                ring[a][sl][pt] = pf[u];
+            }
+        }
+        __syncthreads();
+    };
+    auto at = [&](int a, int I, int J, int K) { return ring[a][K % 3][(J - jh) * SK_HX + (I - ih)]; };
+    auto u = [&](int I, int J, int K) { return at(0, I, J, K); };
+    auto v = [&](int I, int J, int K) { return at(1, I, J, K); };
+    auto w = [&](int I, int J, int K) { return at(2, I, J, K); };
+    fetch(k0 - 1);
+    commit(k0 - 1);
+    fetch(k0);
+    commit(k0);
+    fetch(k0 + 1);
+    for (int k = k0; k <= k1; k++) {
+        commit(k + 1);  // ring holds k-1, k, k+1
+        if (k + 2 <= k1 + 1) fetch(k + 2);
+        if (act) {
+            double s[6];
+            const double sm = strain6(m, i, j, k, u, v, w, s);
+            if (RAW) {
+                const double cu = 0.5 * (u(i - 1, j, k) + u(i, j, k));
+                const double cv = 0.5 * (v(i, j - 1, k) + v(i, j, k));
+                const double cw = 0.5 * (w(i, j, k - 1) + w(i, j, k));
+                // the centre and its periodic ghost copies
+                const int ia[2] = {i, i == 2 ? nxg - 1 : (i == 3 ? nxg : (i == nxg - 2 ? 1 : 0))};
+                const int ka[2] = {k, k == 2 ? nzg - 1 : (k == 3 ? nzg : (k == nzg - 2 ? 1 : 0))};
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        if (ia[a] == 0 || ka[c] == 0) continue;
+                        const int64_t q = i3(nxg, nyg, ia[a], j, ka[c]);
+                        Q[q] = cu;
+                        Q[nC + q] = cv;
+                        Q[2 * nC + q] = cw;
+#pragma unroll
+                        for (int e = 0; e < 6; e++) Q[(3 + e) * nC + q] = sm * s[e];
+                        Q[9 * nC + q] = sm;
+                    }
+            } else {
+                const int64_t q = i3(nxg, nyg, i, j, k);
+#pragma unroll
+                for (int e = 0; e < 6; e++) Q[(10 + e) * nC + q] = 4.0 * sm * s[e];
+            }
+        }
+    }
 }
 
 // tile of k_dsm_lm: 64 columns in x by 4 in z, with a one-cell halo
@@ -402,10 +458,10 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
                 const DsmWork &w, S *NT, int dirichlet, hipStream_t st) {
     const int64_t nC = (int64_t)d.nxg * d.nyg * d.nzg;
     const int txc = (d.nxg + 63) / 64;
-    {
-        const int nt = txc * ((d.nyg - 2 + 3) / 4) * d.nzg;
-        hipLaunchKernelGGL((k_dsm_prep<S>), dim3(nt), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q, nC, txc, nt);
-    }
+    const int stx = (d.nxg - 3 + 63) / 64, sty = (d.nyg - 2 + 3) / 4, skc = (d.nzg - 3 + SK_PLANES - 1) / SK_PLANES;
+    const int snt = stx * sty * skc;
+    hipLaunchKernelGGL((k_dsm_strain<S, true>), dim3(snt), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q, nC, stx,
+                       skc, snt);
     double *Uf = w.Fu, *Vf = Uf + (int64_t)d.nx * d.nyg * d.nzg, *Wf = Vf + (int64_t)d.nxg * d.ny * d.nzg;
     const int nkc = (d.nzp + FV_KCH - 1) / FV_KCH;
     auto fv = [&](int n1, int n2) {
@@ -423,10 +479,8 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
         hipLaunchKernelGGL((k_dsm_filter_vel<S, 2>), dim3(g.second), dim3(DSM_WG), 0, st, d, W, ubar, Wf, g.first,
                            nkc, g.second);
     }
-    {
-        const int tx = (d.nxg - 3 + 63) / 64, nt = tx * ((d.nyg - 2 + 3) / 4) * (d.nzg - 3);
-        hipLaunchKernelGGL(k_dsm_fstrain, dim3(nt), dim3(DSM_WG), 0, st, d, m, w.Fu, w.Q, nC, tx, nt);
-    }
+    hipLaunchKernelGGL((k_dsm_strain<double, false>), dim3(snt), dim3(DSM_WG), 0, st, d, m, (const double *)Uf,
+                       (const double *)Vf, (const double *)Wf, (const double *)nullptr, w.Q, nC, stx, skc, snt);
     const int tx = dsm_tiles_x(d), tz = dsm_tiles_z(d);
     const int nyp = d.nyg - 2;
     const int chunks = (nyp + DSM_LM_ROWS - 1) / DSM_LM_ROWS;
