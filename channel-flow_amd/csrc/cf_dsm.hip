@@ -30,7 +30,7 @@
 // The reference sums the plane over centres 2..nxg-1 x 2..nzg-1, which holds
 // the periodic images nxg-1 (of 2) and nzg-1 (of 2): cells in the first
 // unique column or plane count twice (four times on both).
-#include <utility>
+#include <algorithm>
 
 #include "cf_kernels.hpp"
 
@@ -39,7 +39,10 @@ namespace cf {
 namespace {
 
 constexpr int DSM_WG = 256;
-constexpr int DSM_LM_ROWS = 16;  // rows per thread in k_dsm_lm (one halo row each side)
+// The marching kernels walk chunks of up to DSM_MAX_CHUNK planes or rows;
+// on small grids the chunks shorten until there are DSM_MIN_BLOCKS
+// workgroups (down to 2 rows or planes per chunk).
+constexpr int DSM_MAX_CHUNK = 16, DSM_MIN_BLOCKS = 2048;
 
 // periodic image of a centre index (ghosts 1, n-1, n of a centred axis of n
 // points: 1 <- n-2, n-1 <- 2, n <- 3) and of a face index (1 <- n-1, n <- 2)
@@ -89,16 +92,14 @@ __device__ __forceinline__ double fil_xz(int a, int c) {
 // W (COMP 0, 1, 2), all rows, one-sided y weights on the first and last.
 // The filter factors into (1/6 2/3 1/6) per axis: each thread owns one
 // (x, y) point, forms the x-y filtered value of every plane it passes and
-// marches FV_KCH unique z planes, so each output costs nine loads.  Ghost
+// marches a chunk of kch unique z planes, so each output costs nine loads.  Ghost
 // columns and planes take the values of their periodic images.  Tiles
 // (x, y, k chunk), k chunk fastest.
 // ---------------------------------------------------------------------------
-constexpr int FV_KCH = 16;
-
 template <typename S, int COMP>
 __global__ void __launch_bounds__(DSM_WG)
 k_dsm_filter_vel(Dims d, const S *__restrict__ F, const double *__restrict__ ubar, double *__restrict__ Fo,
-                 int tiles_x, int nchunks, int ntiles) {
+                 int tiles_x, int nchunks, int ntiles, int kch) {
     const int n1 = COMP == 0 ? d.nx : d.nxg, n2 = COMP == 1 ? d.ny : d.nyg, n3 = COMP == 2 ? d.nz : d.nzg;
     const int t = xcd_remap(blockIdx.x, ntiles);
     const int ch = t % nchunks, r = t / nchunks;
@@ -107,7 +108,7 @@ k_dsm_filter_vel(Dims d, const S *__restrict__ F, const double *__restrict__ uba
     if (I > n1 || J > n2) return;
     const int ii = COMP == 0 ? img_f(I, n1) : img_c(I, n1);
     const int nu = COMP == 2 ? n3 - 1 : n3 - 2;  // last unique plane
-    const int k0 = 2 + ch * FV_KCH, k1 = min(k0 + FV_KCH - 1, nu);
+    const int k0 = 2 + ch * kch, k1 = min(k0 + kch - 1, nu);
     const double wm = J == 1 ? 0.0 : (J == n2 ? 1.0 / 3.0 : 1.0 / 6.0);
     const double wp = J == n2 ? 0.0 : (J == 1 ? 1.0 / 3.0 : 1.0 / 6.0);
     const int jm = J > 1 ? J - 1 : J, jp = J < n2 ? J + 1 : J;
@@ -149,7 +150,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 // ---------------------------------------------------------------------------
 // compute_Sij (subgrid.f90:891-949) on a 2.5-D march: a block owns 64 x 4
-// unique centres in (x, y) and walks SK_PLANES z planes; U, V, W sit in an
+// unique centres in (x, y) and walks a chunk of `planes` z planes; U, V, W sit in an
 // LDS ring of three planes (z-1, z, z+1) with a one-cell x-y halo, the next
 // plane's loads issued into registers while the current plane is computed.
 // Every input value is read from HBM about once (plus the halo).
@@ -164,14 +165,13 @@ __device__ __forceinline__ double wave_sum(double x) {
 //   3..8 |S| Sij, 9 |S|       (the raw field; every centre)
 //   10..15 4 |hat S| hat(S)ij (the filtered field; unique centres only)
 // ---------------------------------------------------------------------------
-constexpr int SK_PLANES = 16;
 constexpr int SK_HX = 66, SK_HY = 6, SK_NP = SK_HX * SK_HY;
 
 template <typename T, bool RAW>
 __global__ void __launch_bounds__(DSM_WG)
 k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
              const double *__restrict__ ubar, double *__restrict__ Q, int64_t nC, int tiles_x, int nchunks,
-             int ntiles) {
+             int ntiles, int planes) {
     __shared__ double ring[3][3][SK_NP];  // [array][plane slot][point]
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nz = d.nz, nzg = d.nzg;
     const int t = xcd_remap(blockIdx.x, ntiles);
@@ -181,7 +181,7 @@ k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V
     const int ih = 1 + bx * 64, jh = 1 + by * 4;  // halo origin
     const int i = ih + 1 + lane, j = jh + 1 + wv;
     const bool act = i <= nxg - 2 && j <= nyg - 1;
-    const int k0 = 2 + ch * SK_PLANES, k1 = min(k0 + SK_PLANES - 1, nzg - 2);
+    const int k0 = 2 + ch * planes, k1 = min(k0 + planes - 1, nzg - 2);
     // cooperative load of plane K of the three arrays: 3 * SK_NP values
     constexpr int NPF = (3 * SK_NP + DSM_WG - 1) / DSM_WG;
     double pf[NPF];
@@ -268,7 +268,7 @@ constexpr int LM_NQ = 9;  // staged quantities: centre velocities and |S|Sij
 
 // ---------------------------------------------------------------------------
 // Lij, Mij and their plane products.  Block: 64 columns in x by 4 in z (one
-// wave per z column row); the block marches DSM_LM_ROWS rows of one row
+// wave per z column row); the block marches the `rows` rows of one row
 // chunk.  Each row of the nine staged quantities is loaded once per block
 // into LDS with its halo; the global loads of the next row are issued into
 // registers before the current row is computed and written to LDS after it.
@@ -278,7 +278,8 @@ constexpr int LM_NQ = 9;  // staged quantities: centre velocities and |S|Sij
 // cw = (4 bz + wave) * tiles_x + bx.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(DSM_WG)
-k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int tiles_x, int tiles_z, int nchunks) {
+k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int tiles_x, int tiles_z, int nchunks,
+         int rows) {
     __shared__ double sq[LM_NQ][LM_NP];
     const int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
     const int64_t nC = (int64_t)nxg * nyg * nzg;
@@ -291,8 +292,8 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
     const bool act = i <= nxg - 2 && k <= nzg - 2;
     const int cw = (LM_TZ * bz + wv) * tiles_x + bx;
     const int nyp = nyg - 2;
-    const int j0 = 2 + ch * DSM_LM_ROWS;
-    const int j1 = min(j0 + DSM_LM_ROWS - 1, nyg - 1);
+    const int j0 = 2 + ch * rows;
+    const int j1 = min(j0 + rows - 1, nyg - 1);
     const double wgt = (i == 2 ? 2.0 : 1.0) * (k == 2 ? 2.0 : 1.0);
     // stage row r (block-uniform) into LDS; points outside the arrays read 0
     // and are never used by an active thread
@@ -445,6 +446,11 @@ k_dsm_nut(Dims d, const double *__restrict__ Q, double *__restrict__ avg, S *__r
 }
 
 inline int dsm_tiles_x(const Dims &d) { return (d.nxg - 3 + 63) / 64; }
+// chunk length of a march over n planes or rows with `tiles` tiles per plane
+inline int dsm_chunk(int n, int tiles) {
+    const int64_t len = std::min<int64_t>(DSM_MAX_CHUNK, (int64_t)n * tiles / DSM_MIN_BLOCKS);
+    return (int)std::max<int64_t>(2, len);
+}
 inline int dsm_tiles_z(const Dims &d) { return (d.nzg - 3 + 3) / 4; }
 
 }  // namespace
@@ -458,33 +464,25 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
                 const DsmWork &w, S *NT, int dirichlet, hipStream_t st) {
     const int64_t nC = (int64_t)d.nxg * d.nyg * d.nzg;
     const int txc = (d.nxg + 63) / 64;
-    const int stx = (d.nxg - 3 + 63) / 64, sty = (d.nyg - 2 + 3) / 4, skc = (d.nzg - 3 + SK_PLANES - 1) / SK_PLANES;
-    const int snt = stx * sty * skc;
+    const int stx = (d.nxg - 3 + 63) / 64, sty = (d.nyg - 2 + 3) / 4;
+    const int spl = dsm_chunk(d.nzg - 3, stx * sty), skc = (d.nzg - 3 + spl - 1) / spl, snt = stx * sty * skc;
     hipLaunchKernelGGL((k_dsm_strain<S, true>), dim3(snt), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q, nC, stx,
-                       skc, snt);
+                       skc, snt, spl);
     double *Uf = w.Fu, *Vf = Uf + (int64_t)d.nx * d.nyg * d.nzg, *Wf = Vf + (int64_t)d.nxg * d.ny * d.nzg;
-    const int nkc = (d.nzp + FV_KCH - 1) / FV_KCH;
-    auto fv = [&](int n1, int n2) {
-        const int tx = (n1 + 63) / 64, nt = tx * ((n2 + 3) / 4) * nkc;
-        return std::make_pair(tx, nt);
+    auto fv = [&](auto kern, const S *F, double *Fo, int n1, int n2) {
+        const int tx = (n1 + 63) / 64, tiles = tx * ((n2 + 3) / 4);
+        const int kch = dsm_chunk(d.nzp, tiles), nkc = (d.nzp + kch - 1) / kch;
+        hipLaunchKernelGGL(kern, dim3(tiles * nkc), dim3(DSM_WG), 0, st, d, F, ubar, Fo, tx, nkc, tiles * nkc, kch);
     };
-    {
-        auto g = fv(d.nx, d.nyg);
-        hipLaunchKernelGGL((k_dsm_filter_vel<S, 0>), dim3(g.second), dim3(DSM_WG), 0, st, d, U, ubar, Uf, g.first,
-                           nkc, g.second);
-        g = fv(d.nxg, d.ny);
-        hipLaunchKernelGGL((k_dsm_filter_vel<S, 1>), dim3(g.second), dim3(DSM_WG), 0, st, d, V, ubar, Vf, g.first,
-                           nkc, g.second);
-        g = fv(d.nxg, d.nyg);
-        hipLaunchKernelGGL((k_dsm_filter_vel<S, 2>), dim3(g.second), dim3(DSM_WG), 0, st, d, W, ubar, Wf, g.first,
-                           nkc, g.second);
-    }
+    fv(k_dsm_filter_vel<S, 0>, U, Uf, d.nx, d.nyg);
+    fv(k_dsm_filter_vel<S, 1>, V, Vf, d.nxg, d.ny);
+    fv(k_dsm_filter_vel<S, 2>, W, Wf, d.nxg, d.nyg);
     hipLaunchKernelGGL((k_dsm_strain<double, false>), dim3(snt), dim3(DSM_WG), 0, st, d, m, (const double *)Uf,
-                       (const double *)Vf, (const double *)Wf, (const double *)nullptr, w.Q, nC, stx, skc, snt);
+                       (const double *)Vf, (const double *)Wf, (const double *)nullptr, w.Q, nC, stx, skc, snt, spl);
     const int tx = dsm_tiles_x(d), tz = dsm_tiles_z(d);
     const int nyp = d.nyg - 2;
-    const int chunks = (nyp + DSM_LM_ROWS - 1) / DSM_LM_ROWS;
-    hipLaunchKernelGGL(k_dsm_lm, dim3(tx * tz * chunks), dim3(DSM_WG), 0, st, d, w.Q, w.part, tx, tz, chunks);
+    const int rows = dsm_chunk(nyp, tx * tz), chunks = (nyp + rows - 1) / rows;
+    hipLaunchKernelGGL(k_dsm_lm, dim3(tx * tz * chunks), dim3(DSM_WG), 0, st, d, w.Q, w.part, tx, tz, chunks, rows);
     hipLaunchKernelGGL(k_dsm_avg, dim3(nyp), dim3(DSM_WG), 0, st, tx * tz * 4, nyp, w.part, w.avg);
     {
         const int nt = txc * ((d.nyg + 3) / 4) * d.nzg;
