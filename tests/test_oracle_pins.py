@@ -275,6 +275,65 @@ def test_smagorinsky_simple_shear():
     np.testing.assert_allclose(nt[2:-2, 3:-3, 2:-2], expect, rtol=1e-12)
 
 
+@pytest.mark.parametrize("dirichlet", [0, 1])
+def test_smagorinsky_ghost_sequence(dirichlet):
+    """The completed Smagorinsky (subgrid.f90:564-619) with the dynamic
+    model's ghost sequence (subgrid.f90:521-536): periodic copies in x and
+    z, clipping, then odd (Dirichlet) or even (Neumann) wall rows; the
+    plane average is clipped and gets the same wall rule."""
+    c = cfg_small(les_model=1, cs=0.1, cells=(12, 10, 8), dirichlet_nu_t=dirichlet)
+    o = Oracle(c)
+    rng = np.random.default_rng(11)
+    o.set_state(*(rng.standard_normal(s) for s in (o.shape_u, o.shape_v, o.shape_w)))
+    o.apply_boundary_conditions()
+    o.compute_eddy_viscosity()
+    nt, av = o.nu_t, o.avg_nu_t
+    # centres: 1 <- nxg-2, nxg-1 <- 2, nxg <- 3 (0-based -3, 1, 2), the same in z
+    for ax in (0, 2):
+        a = np.moveaxis(nt, ax, 0)
+        np.testing.assert_array_equal(a[0], a[-3])
+        np.testing.assert_array_equal(a[-2], a[1])
+        np.testing.assert_array_equal(a[-1], a[2])
+    assert np.all(nt[:, 1:-1, :] >= 0)
+    sgn = -1.0 if dirichlet else 1.0
+    np.testing.assert_array_equal(nt[:, 0, :], sgn * nt[:, 1, :])
+    np.testing.assert_array_equal(nt[:, -1, :], sgn * nt[:, -2, :])
+    # plane mean over the reference's range 2..nxg-1 x 2..nzg-1, clipped
+    np.testing.assert_allclose(av[1:-1], np.maximum(nt[1:-1, 1:-1, 1:-1].mean(axis=(0, 2)), 0.0), rtol=1e-14)
+    np.testing.assert_array_equal(av[[0, -1]], sgn * av[[1, -2]])
+
+
+def test_pressure_output_scaling():
+    """P saved at the first substep (projection.f90:129-137) is phi/(dt a11):
+    the velocity correction of the same projection equals dt a11 grad P, so a
+    wrong scale or a shifted P fails.  Ghosts are copies of the neighbours."""
+    c = cfg_small(cells=(12, 10, 8))
+    o = Oracle(c)
+    rng = np.random.default_rng(12)
+    o.set_state(*(rng.standard_normal(s) for s in (o.shape_u, o.shape_v, o.shape_w)))
+    o.apply_boundary_conditions()
+    dt = 3.7e-3
+    o.set_scalars(dt=dt)
+    U0, V0, _ = o.get_state()
+    o.compute_projection_step(rk_step=1)
+    U1, V1, _ = o.get_state()
+    P = o.P
+    g = o.grid()
+    a11 = 8.0 / 15.0
+    xg, yg = g["xg"], g["yg"]
+    # U faces i = 2..nx-1, V faces j = 2..ny-1 (1-based); interior j, k
+    du = (U0 - U1)[1:-1, 1:-1, 1:-1] * (xg[2:-1] - xg[1:-2])[:, None, None]
+    np.testing.assert_allclose(du, dt * a11 * (P[2:-1, 1:-1, 1:-1] - P[1:-2, 1:-1, 1:-1]), rtol=1e-9,
+                               atol=1e-12 * np.abs(du).max())
+    dv = (V0 - V1)[1:-1, 1:-1, 1:-1] * (yg[2:-1] - yg[1:-2])[None, :, None]
+    np.testing.assert_allclose(dv, dt * a11 * (P[1:-1, 2:-1, 1:-1] - P[1:-1, 1:-2, 1:-1]), rtol=1e-9,
+                               atol=1e-12 * np.abs(dv).max())
+    for ax in (0, 1, 2):
+        a = np.moveaxis(P, ax, 0)
+        np.testing.assert_array_equal(a[0], a[1])
+        np.testing.assert_array_equal(a[-1], a[-2])
+
+
 # ------------------------------------------------------ dynamic Smagorinsky
 def _hat_z(a):
     """Simpson filter along one periodic direction: the x-z weights
