@@ -257,6 +257,21 @@ def test_eqwm_batch_matches_oracle():
         assert abs(int(it[q]) - i_o) <= 1
 
 
+def test_saved_pressure_matches_oracle():
+    """P saved at the first substep of the last step (projection.f90:129-137):
+    phi/(dt a11) with neighbour-copy ghosts, after three steps."""
+    cfg = small_cfg(dPdx=2e-3)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(3, graph=True)
+    o.run(3)
+    Pg, Po = g.P, o.P
+    assert np.all(np.isfinite(Pg))
+    assert relerr(Pg, Po) < 1e-10, relerr(Pg, Po)
+
+
 @pytest.mark.parametrize("les", [0, 1, 2])
 def test_actual_pressure_matches_oracle(les):
     """compute_pressure (pressure.f90:21-341) on the same state: the wall
