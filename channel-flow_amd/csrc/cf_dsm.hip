@@ -42,7 +42,7 @@ constexpr int DSM_WG = 256;
 // The marching kernels walk chunks of up to DSM_MAX_CHUNK planes or rows;
 // on small grids the chunks shorten until there are DSM_MIN_BLOCKS
 // workgroups (down to 2 rows or planes per chunk).
-constexpr int DSM_MAX_CHUNK = 16, DSM_MIN_BLOCKS = 2048;
+constexpr int DSM_MAX_CHUNK = 16, DSM_MIN_BLOCKS = 512;
 
 // periodic image of a centre index (ghosts 1, n-1, n of a centred axis of n
 // points: 1 <- n-2, n-1 <- 2, n <- 3) and of a face index (1 <- n-1, n <- 2)
