@@ -150,9 +150,10 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 // ---------------------------------------------------------------------------
 // compute_Sij (subgrid.f90:891-949) on a 2.5-D march: a block owns 64 x 4
-// unique centres in (x, y) and walks a chunk of `planes` z planes; U, V, W sit in an
-// LDS ring of three planes (z-1, z, z+1) with a one-cell x-y halo, the next
-// plane's loads issued into registers while the current plane is computed.
+// unique centres in (x, y) and walks a chunk of `planes` z planes; U, V, W
+// sit in an LDS ring of three planes (z-1, z, z+1) with a one-cell x-y halo,
+// the next plane's loads issued into registers while the current plane is
+// computed.
 // Every input value is read from HBM about once (plus the halo).
 //   RAW (the field, FP32 deviations + ubar or FP64): centre velocities,
 //     |S|Sij and |S| into Q[0..9] at the centre and at its periodic ghost
