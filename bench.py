@@ -218,6 +218,41 @@ def measured_copy_gbps(mib=1024, reps=10):
     return 2.0 * n * 8 / (ms / 1e3) / 1e9
 
 
+def beside_step_timings(g, cells, reps=3):
+    """The rows beside the step, on the FP64 context of the main run: the
+    actual pressure (compute_pressure, pressure.f90:21-341) and one plane
+    statistics accumulation (statistics.f90:22-171) from HIP events on the
+    solver's stream (device time, excluding the host copy of P), and the
+    snapshot writer's host rate (input_output.f90:393-544 layout)."""
+    import tempfile
+    from channel_flow.snapshot import write_snapshot
+    g.timing(True)
+    g.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P = g.compute_pressure()
+    t_p = (time.perf_counter() - t0) / reps
+    for _ in range(reps):
+        g.stats_accumulate()
+    g.sync()
+    rep = g.timing_report()
+    g.timing(False)
+    U, V, W = g.get_state()
+    nbytes = 8 * (U.size + V.size + W.size + P.size)
+    with tempfile.TemporaryDirectory() as td:
+        t0 = time.perf_counter()
+        write_snapshot(os.path.join(td, "snap"), g.x, g.y, g.z, U, V, W, P=P)
+        t_w = time.perf_counter() - t0
+    n = cells[0] * cells[1] * cells[2]
+    cp = rep.get("compute_pressure", (None, 0))[0]
+    sa = rep.get("stats_accumulate", (None, 0))[0]
+    return {"compute_pressure_device_ms": round(cp, 4) if cp else None,
+            "compute_pressure_call_s": round(t_p, 3),
+            "compute_pressure_mcell_per_s": round(n / cp / 1e3, 1) if cp else None,
+            "stats_accumulate_device_ms": round(sa, 4) if sa else None,
+            "snapshot_write_GBps": round(nbytes / t_w / 1e9, 2)}
+
+
 def dsm_steps(cells, steps, warmup=2):
     """The dynamic Smagorinsky model (subgrid.f90:376-538) on the DNS
     workload's grid: RK3 steps with les_model 2 (one model evaluation per
@@ -334,6 +369,8 @@ def main():
                                "what": "1 GiB device copy, read+write bytes / HIP-event time"}
     except Exception as e:
         log("copy ceiling not measured: %s" % e)
+    if not a.no_others:
+        out["beside_step_fp64"] = beside_step_timings(g64, cells)
     if not a.no_mixed:
         U64, V64, W64 = g64.get_state()
         s64 = g64.compute_statistics()

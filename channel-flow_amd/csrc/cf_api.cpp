@@ -264,7 +264,7 @@ int Ctx<S>::init() {
     if (thomas_uses_scratch((int)sizeof(Cx<S>), d.mh * d.nzp) && A(&cscr, thomas_scratch_elems(d))) return -1;
     if (A(&NT, nC) || A(&spec, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
         A(&part, npart) || A(&dtscr, dt_scratch_doubles(d)) || A(&sc, 1) ||
-        (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, 7 * d.ny + 8) || A(&stats_acc, 7 * d.ny + 8))
+        (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, stats_out_doubles(d)) || A(&stats_acc, 7 * d.ny + 8))
         return -1;
     if (A(&m_rdxf, h.rdxf.size()) || A(&m_rdxc, h.rdxc.size()) || A(&m_rdyf, h.rdyf.size()) ||
         A(&m_rdyc, h.rdyc.size()) || A(&m_rdzf, h.rdzf.size()) || A(&m_rdzc, h.rdzc.size()) ||
@@ -808,6 +808,7 @@ int Ctx<S>::compute_pressure(double *P) {
     c.src_b = coef_bc_1 * (yg[2] - yg[1]);
     c.src_t = -(coef_bc_2 * (yg[nyg] - yg[nyg - 1]));
     if (e == hipSuccess) {
+        tic("compute_pressure");
         launch_rhs_only<S, S>(d, met, F[0], F[1], F[2], nut ? NT : nullptr, Ru, Rv, Rw, sc, ubar, h.prm.nu,
                               h.prm.omega_z, stream, true);
         launch_pressure_bc<S>(d, F[1], c, Rv, bcs, ysrc, stream);
@@ -817,6 +818,7 @@ int Ctx<S>::compute_pressure(double *P) {
         launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
         launch_ifftx_proju<S, S, double>(d, met, spec, pcell, (S *)nullptr, pM, twM, twN, (double)(d.nxp * d.nzp),
                                          stream);
+        toc();
         e = hipGetLastError();
     }
     std::vector<double> pc(ncell), bh(2 * nb);
@@ -974,8 +976,10 @@ template <typename S>
 int Ctx<S>::stats_accumulate() {
     HCK(hipSetDevice(dev));
     const Dims &d = h.d;
+    tic("stats_accumulate");
     launch_stats<S>(d, met, st[cur][0], st[cur][1], st[cur][2], ubar, stats_out, stream);
     launch_stats_accum(stats_out, stats_acc, 7 * d.ny, stream);
+    toc();
     HCK(hipGetLastError());
     return 0;
 }

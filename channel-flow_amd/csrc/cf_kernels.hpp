@@ -80,6 +80,7 @@ void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const
                   const double *ubar, double *out, hipStream_t st);
 
 void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st);
+size_t stats_out_doubles(const Dims &d);  // the `out` buffer of launch_stats (results + block partials)
 
 // ---- Poisson chain (cf_fft.hip) ----
 int fft_rows_per_wg(int M, int elems_per_row, int esize);

@@ -858,24 +858,25 @@ k_smagorinsky(Dims d, Metrics m, const double *__restrict__ y, const double *__r
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_stats_planes(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
-               const S *__restrict__ W, const double *__restrict__ ubar, double *__restrict__ out) {
+               const S *__restrict__ W, const double *__restrict__ ubar, double *__restrict__ part) {
     __shared__ double sh[8];
     const int jj = 1 + blockIdx.x;
+    const int nb = gridDim.y, q0 = threadIdx.x + blockIdx.y * blockDim.x, qs = blockDim.x * nb;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny, nz = d.nz, nzg = d.nzg;
     const double a = m.w0[jj], b = m.w1[jj];
     const double ua = ubar ? ubar[jj] : 0.0, ubb = ubar ? ubar[jj + 1] : 0.0;
     double su = 0, su2 = 0, sw = 0, sw2 = 0, sv = 0, sv2 = 0, suv = 0;
-    for (int q = threadIdx.x; q < (nx - 2) * (nzg - 2); q += blockDim.x) {
+    for (int q = q0; q < (nx - 2) * (nzg - 2); q += qs) {
         const int i = 1 + q % (nx - 2), k = 1 + q / (nx - 2);
         const double t = a * ((double)U[i3(nx, nyg, i, jj, k)] + ua) + b * ((double)U[i3(nx, nyg, i, jj + 1, k)] + ubb);
         su += t; su2 += t * t;
     }
-    for (int q = threadIdx.x; q < (nxg - 2) * (nz - 2); q += blockDim.x) {
+    for (int q = q0; q < (nxg - 2) * (nz - 2); q += qs) {
         const int i = 1 + q % (nxg - 2), k = 1 + q / (nxg - 2);
         const double t = a * (double)W[i3(nxg, nyg, i, jj, k)] + b * (double)W[i3(nxg, nyg, i, jj + 1, k)];
         sw += t; sw2 += t * t;
     }
-    for (int q = threadIdx.x; q < (nxg - 2) * (nzg - 2); q += blockDim.x) {
+    for (int q = q0; q < (nxg - 2) * (nzg - 2); q += qs) {
         const int i = 2 + q % (nxg - 2), k = 2 + q / (nxg - 2);
         const double vv = (double)V[i3(nxg, ny, i, jj, k)];
         sv += vv; sv2 += vv * vv;
@@ -887,8 +888,17 @@ k_stats_planes(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__
     double vals[7] = {su, sv, sw, su2, sv2, sw2, suv};
     for (int c = 0; c < 7; c++) {
         double r = block_reduce(vals[c], OpAdd(), sh);
-        if (threadIdx.x == 0) out[c * ny + (jj - 1)] = r;
+        if (threadIdx.x == 0) part[((size_t)c * ny + (jj - 1)) * nb + blockIdx.y] = r;
     }
+}
+
+// out[e] = sum of the STATS_NB block partials of statistic e, in block order
+__global__ void k_stats_fold(const double *__restrict__ part, double *__restrict__ out, int n, int nb) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double r = 0.0;
+    for (int b = 0; b < nb; b++) r += part[(size_t)e * nb + b];
+    out[e] = r;
 }
 
 // time-window accumulation of the plane statistics: acc[0..n) += out, acc[n] += 1
@@ -1037,10 +1047,16 @@ void launch_smagorinsky(const Dims &d, const Metrics &m, const double *y, const 
                        ubar, cs, nu, damping, tx, ty, nt);
 }
 
+// each plane is split over STATS_NB workgroups, so the launch fills the GPU
+constexpr int STATS_NB = 8;
+size_t stats_out_doubles(const Dims &d) { return (size_t)7 * d.ny * (STATS_NB + 1) + 8; }
+
 template <typename S>
 void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                   const double *ubar, double *out, hipStream_t st) {
-    hipLaunchKernelGGL((k_stats_planes<S>), dim3(d.ny), dim3(256), 0, st, d, m, U, V, W, ubar, out);
+    double *part = out + 7 * (size_t)d.ny + 8;  // the tail of the stats_out_doubles() buffer
+    hipLaunchKernelGGL((k_stats_planes<S>), dim3(d.ny, STATS_NB), dim3(256), 0, st, d, m, U, V, W, ubar, part);
+    hipLaunchKernelGGL(k_stats_fold, dim3((7 * d.ny + 255) / 256), dim3(256), 0, st, part, out, 7 * d.ny, STATS_NB);
 }
 
 // ---------------------------------------------------------------------------
