@@ -110,3 +110,30 @@ def test_cancel_leaves_others_identical(alone):
         _check_same(sv, ids, alone, skip=(victim,))
         with pytest.raises(RuntimeError):
             sv.result(ids[victim])
+
+
+def test_dynamic_model_cases_in_the_mix():
+    """Cases with the dynamic Smagorinsky model (its work arrays carved from
+    the serving arena) beside Smagorinsky and DNS cases: each equals the
+    same case run alone."""
+    from channel_flow.solver import ChannelFlow
+    ex = cases.case_example()
+    ex.name = "example"
+    dsm = ensemble_case((48, 32, 48), 900.0, True, name="dsm")
+    dsm.les_model = 2
+    wl = [(ex, 12, 0.0), (dsm, 10, 0.0), _workload()[0]]
+    alone = []
+    for cfg, n, _ in wl:
+        g = ChannelFlow(cfg)
+        g.initialise(*cases.initial_field(cfg))
+        g.run(n, graph=True)
+        alone.append(g.get_state())
+        g.close()
+    with Server(4 << 30, max_active=3, depth=2) as sv:
+        ids = [sv.submit(cfg, n, arrival=t) for cfg, n, t in wl]
+        sv.start()
+        assert sv.wait(120.0)
+        for q, cid in enumerate(ids):
+            U, V, W, _ = sv.result(cid)
+            for a, b in zip((U, V, W), alone[q]):
+                assert np.array_equal(a, b), q
