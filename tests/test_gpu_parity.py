@@ -114,6 +114,26 @@ def test_ten_rk3_steps(cells):
     assert abs(g.scalars["dPdx"] - o.scalars["dPdx"]) * dt < 1e-13
 
 
+@pytest.mark.parametrize("cells", [(128, 24, 96), (64, 20, 384), (384, 12, 64)])
+def test_ten_rk3_steps_large_grid_kernels(cells, monkeypatch):
+    """The kernels the 384x256x384 workload runs, on grids the oracle steps
+    in seconds: x and z transform lengths compiled with fixed plans (half
+    length 64, 32 or 192; z length 96, 384 or 64) and the lane form of the y solve,
+    forced here (it is chosen by default from 65,536 modes up)."""
+    monkeypatch.setenv("CF_TH_LANE", "1")
+    cfg = small_cfg(cells=cells)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.run(10, graph=True)
+    o.run(10)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert relerr(a, b) < 1e-10, relerr(a, b)
+    _, dmax = g.check_divergence()
+    assert dmax < 1e-10
+
+
 def test_graph_replay_matches_eager():
     cfg = small_cfg()
     CF = _solver()
