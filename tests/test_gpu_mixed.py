@@ -50,6 +50,26 @@ def test_mixed_tracks_fp64(nsteps):
     assert abs(g32.compute_mean_mass_flow_U() - g32.scalars["Qflow_x_0"]) < 1e-12
 
 
+def test_mixed_lane_solve_and_fixed_plans_track_fp64(monkeypatch):
+    """The mixed path with the kernels of the 384x256x384 workload (half
+    length 192, z length 64 fixed-plan transforms, lane y solve forced):
+    ten steps within the mixed tolerances of the FP64 path."""
+    monkeypatch.setenv("CF_TH_LANE", "1")
+    CF = solver()
+    g64 = CF(cfg_of("fp64", cells=(384, 24, 64)))
+    g32 = CF(cfg_of("mixed", cells=(384, 24, 64)))
+    U, V, W = cases.initial_field(g64.cfg)
+    g64.initialise(U, V, W)
+    g32.initialise(U, V, W)
+    g64.run(10, graph=True)
+    g32.run(10, graph=True)
+    a, b = g32.get_state(), g64.get_state()
+    assert relerr(a[0], b[0]) < 1e-4
+    assert np.max(np.abs(a[1] - b[1])) < 1e-4 * np.max(np.abs(b[0]))
+    _, div = g32.check_divergence()
+    assert div < 1e-5
+
+
 def test_mixed_actual_pressure_tracks_fp64():
     """compute_pressure in a mixed context (FP32 stencils and spectrum, FP64
     y solve) against the FP64 context on the same state: 1e-3 of max |P|
