@@ -43,6 +43,7 @@ SIGNATURES = {
     "cf_compute_pressure": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_set_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_get_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_get_avg_nu_t": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_get_wall": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     "cf_set_tau_init": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_set_scalars": (ctypes.c_int, [ctypes.c_void_p, _dp]),

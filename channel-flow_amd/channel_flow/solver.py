@@ -122,6 +122,13 @@ class ChannelFlow:
         _lib.check(self.lib.cf_get_nu_t(self.h, _lib.dptr(a)), "cf_get_nu_t")
         return a
 
+    @property
+    def avg_nu_t(self):
+        """avg_nu_t(1,nyg,1) of the last eddy-viscosity evaluation."""
+        a = np.zeros(self.nyg)
+        _lib.check(self.lib.cf_get_avg_nu_t(self.h, _lib.dptr(a)), "cf_get_avg_nu_t")
+        return a
+
     def set_nu_t(self, a):
         a = np.asfortranarray(a, dtype=np.float64)
         _lib.check(self.lib.cf_set_nu_t(self.h, _lib.dptr(a)), "cf_set_nu_t")

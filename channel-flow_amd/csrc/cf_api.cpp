@@ -868,6 +868,35 @@ int Ctx<S>::get_nu_t(double *nt) {
     const Dims &d = h.d;
     return get(nt, NT, (size_t)d.nxg * d.nyg * d.nzg);
 }
+// avg_nu_t (global.f90:180): the dynamic model's plane coefficient
+// <LM>/<MM> (subgrid.f90:505-535), or the clipped plane mean of nu_t of the
+// completed Smagorinsky (subgrid.f90:605-617), wall rows by the nu_t rule
+template <typename S>
+int Ctx<S>::get_avg_nu_t(double *avg) {
+    HCK(hipSetDevice(dev));
+    const Dims &d = h.d;
+    const int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
+    std::fill(avg, avg + nyg, 0.0);
+    if (h.prm.les_model == 2 && !nut_user) {
+        HCK(hipStreamSynchronize(stream));
+        HCK(hipMemcpy(avg, dsm.avg, nyg * sizeof(double), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    if (h.prm.les_model == 0 && !nut_user) return 0;
+    std::vector<double> nt((size_t)nxg * nyg * nzg);
+    if (get_nu_t(nt.data())) return -1;
+    for (int j = 2; j <= nyg - 1; j++) {
+        double sum = 0.0;
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int i = 2; i <= nxg - 1; i++) sum += nt[i3(nxg, nyg, i, j, k)];
+        avg[j - 1] = std::max(sum / (double)((nxg - 2) * (nzg - 2)), 0.0);
+    }
+    const double sg = h.prm.dirichlet_nu_t == 1 ? -1.0 : 1.0;
+    avg[0] = sg * avg[1];
+    avg[nyg - 1] = sg * avg[nyg - 2];
+    return 0;
+}
+
 template <typename S>
 int Ctx<S>::get_wall(double *ax, double *az, double *ti) {
     HCK(hipSetDevice(dev));
@@ -1212,6 +1241,11 @@ int cf_compute_pressure(cf_ctx *c, double *P) {
 }
 int cf_set_nu_t(cf_ctx *c, const double *nt) { CTX_CHECK(c); return c->impl->set_nu_t(nt); }
 int cf_get_nu_t(cf_ctx *c, double *nt) { CTX_CHECK(c); return c->impl->get_nu_t(nt); }
+int cf_get_avg_nu_t(cf_ctx *c, double *avg) {
+    CTX_CHECK(c);
+    if (!avg) { set_error("cf_get_avg_nu_t: null output"); return -1; }
+    return c->impl->get_avg_nu_t(avg);
+}
 int cf_get_wall(cf_ctx *c, double *ax, double *az, double *ti) { CTX_CHECK(c); return c->impl->get_wall(ax, az, ti); }
 int cf_set_tau_init(cf_ctx *c, const double *ti) { CTX_CHECK(c); return c->impl->set_tau_init(ti); }
 int cf_set_scalars(cf_ctx *c, const double *s) { CTX_CHECK(c); return c->impl->set_scalars(s); }

@@ -64,6 +64,7 @@ struct CtxBase {
     virtual int compute_pressure(double *P) = 0;
     virtual int set_nu_t(const double *nt) = 0;
     virtual int get_nu_t(double *nt) = 0;
+    virtual int get_avg_nu_t(double *avg) = 0;
     virtual int get_wall(double *ax, double *az, double *ti) = 0;
     virtual int set_tau_init(const double *ti) = 0;
     virtual int set_scalars(const double *s) = 0;
@@ -172,6 +173,7 @@ struct Ctx : CtxBase {
     int compute_pressure(double *P) override;
     int set_nu_t(const double *nt) override;
     int get_nu_t(double *nt) override;
+    int get_avg_nu_t(double *avg) override;
     int get_wall(double *ax, double *az, double *ti) override;
     int set_tau_init(const double *ti) override;
     int set_scalars(const double *s) override;

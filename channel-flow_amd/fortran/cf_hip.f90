@@ -67,6 +67,12 @@ Module cf_hip
        Real(c_double), Intent(Out) :: a(*)
        Integer(c_int) :: rc
      End Function cf_get_nu_t
+     Function cf_get_avg_nu_t(ctx, a) Bind(C, name='cf_get_avg_nu_t') Result(rc)
+       Import :: c_int, c_double, c_ptr
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: a(*)
+       Integer(c_int) :: rc
+     End Function cf_get_avg_nu_t
      Function cf_set_nu_t(ctx, a) Bind(C, name='cf_set_nu_t') Result(rc)
        Import :: c_int, c_double, c_ptr
        Type(c_ptr), Value :: ctx

@@ -123,12 +123,9 @@ Contains
     Real(Int64), Dimension(nxg,nyg,nz), Intent(In) :: W_
     Real(Int64), Dimension(1,nyg,1), Intent(Out) :: avg_nu_t_
     Real(Int64), Dimension(nxg,nyg,nzg), Intent(Out) :: nu_t_
-    Integer :: j
     Call cf_check(cf_compute_eddy_viscosity(ctx), 'compute_eddy_viscosity')
     Call cf_check(cf_get_nu_t(ctx, nu_t_), 'cf_get_nu_t')
-    Do j = 1, nyg
-       avg_nu_t_(1,j,1) = Sum(nu_t_(2:nxg-1,j,2:nzg-1))/Real((nxg-2)*(nzg-2),8)
-    End Do
+    Call cf_check(cf_get_avg_nu_t(ctx, avg_nu_t_), 'cf_get_avg_nu_t')
   End Subroutine compute_eddy_viscosity
 End Module subgrid
 

@@ -72,6 +72,11 @@ int cf_get_pressure(cf_ctx *c, double *P);      /* projection.f90:129-137 */
 int cf_compute_pressure(cf_ctx *c, double *P);
 int cf_set_nu_t(cf_ctx *c, const double *nu_t);
 int cf_get_nu_t(cf_ctx *c, double *nu_t);
+/* avg_nu_t(1,nyg,1) of global.f90:180 after the last eddy-viscosity
+   evaluation: the dynamic model's <LM>/<MM> per row (subgrid.f90:505-535) or
+   the clipped plane mean of nu_t (subgrid.f90:605-617); wall rows by the nu_t
+   ghost rule; zeros without a model.  nyg values. */
+int cf_get_avg_nu_t(cf_ctx *c, double *avg);
 int cf_get_wall(cf_ctx *c, double *alpha_x, double *alpha_z, double *tau_init);
 int cf_set_tau_init(cf_ctx *c, const double *tau_init);
 int cf_set_scalars(cf_ctx *c, const double *s);

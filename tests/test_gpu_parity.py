@@ -197,6 +197,23 @@ def test_dynamic_smagorinsky_matches(cells, dirichlet):
     assert relerr(ng, no) < 1e-11, relerr(ng, no)
 
 
+@pytest.mark.parametrize("les,dirichlet", [(1, 0), (1, 1), (2, 0), (2, 1)])
+def test_avg_nu_t_matches_oracle(les, dirichlet):
+    """avg_nu_t after an eddy-viscosity evaluation (cf_get_avg_nu_t): the
+    clipped plane mean (Smagorinsky) or the dynamic coefficient, wall rows
+    by the nu_t rule."""
+    cfg = small_cfg(les_model=les, cs=0.1, dirichlet_nu_t=dirichlet)
+    g, o = pair(cfg)
+    U, V, W = cases.initial_field(cfg)
+    g.initialise(U, V, W)
+    o.initialise(U, V, W)
+    g.compute_eddy_viscosity()
+    o.compute_eddy_viscosity()
+    ag, ao = g.avg_nu_t, o.avg_nu_t
+    assert np.any(ao != 0)
+    assert relerr(ag, ao) < 1e-11, relerr(ag, ao)
+
+
 @pytest.mark.parametrize("wall", [0, 3])
 def test_dynamic_smagorinsky_steps_match(wall):
     """Whole RK3 steps with the dynamic model (graph-replayed on the GPU).
