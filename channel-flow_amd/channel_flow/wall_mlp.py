@@ -6,7 +6,8 @@ of ``compute_equilibrium_wall_model`` (wallmodel.f90:591-685); wall points
 whose inputs leave the box of the training features are solved by the ODE.
 
 The reference has no learned wall model; this module is an addition whose
-judge is the EQWM kernel (and through it the flang-compiled eqwm.f90).
+errors are measured against the EQWM kernel (and through it the
+flang-compiled eqwm.f90).
 """
 import ctypes
 

@@ -7,8 +7,10 @@
  * Every routine follows the reference operation by operation for the points
  * it computes (same association order of sums and products, divisions kept
  * as divisions), so that its results track the reference at round-off level.
- * Single-rank MPI semantics are the intended ones (plain periodic copies,
- * SURVEY appendix A1/A2): periodic ghosts are copies of interior planes.
+ * Single-rank MPI semantics are the intended ones: periodic ghosts are plain
+ * copies of interior planes (the one-rank self-exchanges of
+ * boundary_conditions.f90:156-202 and projection.f90:150-160 would truncate
+ * or drop them).
  *
  * Compile with -ffp-contract=off so that no multiply-add is fused.
  */

@@ -27,7 +27,7 @@ def main():
             st = float(rng.choice([1.05, 1.10, 1.15]))
             tau, it = ref_eqwm(u, nu, h, tin, ncv=ncv, stretch=st)
             rows.append(dict(u=u, nu=nu, h=h, tau_in=tin, ncv=ncv, stretch=st, tau=tau, iter=it))
-    # the survey's probe point and a laminar point
+    # a turbulent probe point (u = 1, nu = 1e-4, h = 0.1) and a laminar point
     for (u, nu, h, tin, ncv) in [(1.0, 1e-4, 0.1, 1e-3, 30), (1e-3, 1e-3, 0.1, 1e-6, 30)]:
         tau, it = ref_eqwm(u, nu, h, tin, ncv=ncv, stretch=1.1)
         rows.append(dict(u=u, nu=nu, h=h, tau_in=tin, ncv=ncv, stretch=1.1, tau=tau, iter=it))
