@@ -25,7 +25,7 @@ extern "C" {
 typedef struct orc_params {
     double cfl;            /* <0: fixed dt = -cfl (time_integration.f90:223) */
     double nu;
-    double dPdx, dPdz;     /* forcing carried between steps (A10)            */
+    double dPdx, dPdz;     /* forcing carried between steps                  */
     double omega_z;        /* spanwise rotation (equations.f90:169)          */
     double cs;             /* Smagorinsky coefficient (subgrid.f90:580)      */
     double eqwm_stretch;   /* wallmodel.f90:621 passes 1.10                  */
