@@ -97,7 +97,7 @@ def cpu_baseline(cells, steps_cap_s=30.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
     from channel_flow import cases
-    cfg = cases.case_c2()
+    cfg = cases.case_dns384()
     cfg.cells = cells
     nth = int(os.environ.get("CF_CPU_THREADS", min(os.cpu_count() or 1, 16)))
     o = Oracle(cfg, nthreads=nth)
@@ -131,10 +131,10 @@ def other_configs(steps=50, warmup=5):
     from channel_flow import cases
     from channel_flow.solver import ChannelFlow
     res = {}
-    dsm128 = cases.case_c3()
+    dsm128 = cases.case_wmles128()
     dsm128.les_model = 2
-    todo = [("dns64_fp64", cases.case_c1(), "fp64"), ("wmles128_mixed", cases.case_c3(), "mixed"),
-            ("wmles128_fp64", cases.case_c3(), "fp64"), ("wmles256_mixed", cases.case_c4(), "mixed"),
+    todo = [("dns64_fp64", cases.case_dns64(), "fp64"), ("wmles128_mixed", cases.case_wmles128(), "mixed"),
+            ("wmles128_fp64", cases.case_wmles128(), "fp64"), ("wmles256_mixed", cases.case_wmles256(), "mixed"),
             ("example_dsm_fp64", cases.case_example(), "fp64"), ("wmles128_dsm_mixed", dsm128, "mixed")]
     for name, cfg, prec in todo:
         cfg.precision = prec
@@ -259,7 +259,7 @@ def dsm_steps(cells, steps, warmup=2):
     substep), graph-replayed, and the model's share from HIP events."""
     from channel_flow import cases
     from channel_flow.solver import ChannelFlow
-    cfg = cases.case_c2("fp64")
+    cfg = cases.case_dns384("fp64")
     cfg.cells = cells
     cfg.les_model = 2
     g = ChannelFlow(cfg, device=0)
@@ -324,7 +324,7 @@ def main():
     ncell = cells[0] * cells[1] * cells[2]
     mh_ratio = (cells[0] // 2 + 1) * 2.0 / cells[0]
 
-    cfg = cases.case_c2("fp64")
+    cfg = cases.case_dns384("fp64")
     cfg.cells = cells
     g64, ms64, ms64_ev, rep64, div64 = run_precision(cfg, a.steps, a.warmup, cases)
     value = ncell / (ms64 / 1e3) / 1e6
@@ -375,7 +375,7 @@ def main():
         U64, V64, W64 = g64.get_state()
         s64 = g64.compute_statistics()
         g64.close()
-        cfgm = cases.case_c2("mixed")
+        cfgm = cases.case_dns384("mixed")
         cfgm.cells = cells
         gm, msm, msm_ev, repm, divm = run_precision(cfgm, a.steps, a.warmup, cases)
         Um, Vm, Wm = gm.get_state()

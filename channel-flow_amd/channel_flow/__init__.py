@@ -1,9 +1,9 @@
 """Host package of the MI355X channel-flow time-step library."""
 from .cases import (CaseConfig, make_grid, centres, initial_field, counter_uniform,
-                    case_c1, case_c2, case_c3, case_c4)
+                    case_example, case_dns64, case_dns384, case_wmles128, case_wmles256)
 
 __all__ = ["CaseConfig", "make_grid", "centres", "initial_field", "counter_uniform",
-           "case_c1", "case_c2", "case_c3", "case_c4"]
+           "case_example", "case_dns64", "case_dns384", "case_wmles128", "case_wmles256"]
 
 
 def solver():

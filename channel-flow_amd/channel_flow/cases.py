@@ -148,20 +148,20 @@ def case_example():
                       dirichlet_nu_t=0, steps=1000)
 
 
-def case_c1():
+def case_dns64():
     """64^3 DNS, Re_b = 2800, tanh 2.0, dt = 2e-3, 100 steps."""
     return CaseConfig(name="dns64", cells=(64, 64, 64), stretch=2.0, nu=1 / 2800.0,
                       cfl=-2e-3, dPdx=(retau_from_reb(2800) / 2800.0) ** 2, steps=100)
 
 
-def case_c2(precision="fp64"):
+def case_dns384(precision="fp64"):
     """384x256x384 DNS, Re_b = 10000, tanh 2.2, CFL 0.5, 1000 steps."""
     return CaseConfig(name="dns384", cells=(384, 256, 384), stretch=2.2, nu=1 / 10000.0,
                       cfl=0.5, dPdx=(retau_from_reb(10000) / 10000.0) ** 2, steps=1000,
                       precision=precision)
 
 
-def case_c3():
+def case_wmles128():
     """128x64x128 wall-modelled LES at Re_tau = 2000, uniform y, Smagorinsky
     Cs = 0.1, EQWM at h = 0.1 with 30 cells, tol 1e-8."""
     nu = 1.0 / reb_from_retau(2000.0)
@@ -171,10 +171,10 @@ def case_c3():
                       eqwm_max_iter=100, steps=5000, precision="mixed")
 
 
-def case_c4():
+def case_wmles256():
     """256x96x256 wall-modelled LES at Re_tau = 5200 with the learned wall model."""
     nu = 1.0 / reb_from_retau(5200.0)
-    c = case_c3()
+    c = case_wmles128()
     c.name = "wmles256_mlp"
     c.cells = (256, 96, 256)
     c.nu = nu

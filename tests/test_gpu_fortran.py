@@ -54,7 +54,7 @@ def run_driver(tmp_path, text):
 
 def test_fortran_driver_matches_python_host(tmp_path):
     from channel_flow.solver import ChannelFlow
-    cfg = cases.case_c1()
+    cfg = cases.case_dns64()
     cfg.cells = (32, 24, 16)
     out = run_driver(tmp_path, input_text(cfg, 6, 6, 3, 3, 1, "dummy", "run"))
     assert "step" in out

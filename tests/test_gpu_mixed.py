@@ -16,7 +16,7 @@ def solver():
 
 
 def cfg_of(prec, **kw):
-    c = cases.case_c1()
+    c = cases.case_dns64()
     c.cells = kw.pop("cells", (64, 48, 32))
     c.precision = prec
     for k, v in kw.items():
@@ -145,17 +145,17 @@ def _window(g, steps, start, every):
     return g.stats_window()
 
 
-@pytest.mark.parametrize("case,steps,start", [("c1", 1200, 200), ("c3", 800, 200)])
+@pytest.mark.parametrize("case,steps,start", [("dns64", 1200, 200), ("wmles128", 800, 200)])
 def test_window_statistics_within_one_percent(case, steps, start):
     """Mean profile and Reynolds stresses averaged over a window of steps:
     the mixed path within 1% of the FP64 path (relative to each quantity's
     maximum over y), both accumulated in FP64 on the device."""
     CF = solver()
-    if case == "c1":
+    if case == "dns64":
         mk = lambda p: cfg_of(p, cells=(64, 64, 64))
     else:
         def mk(p):
-            c = cases.case_c3()
+            c = cases.case_wmles128()
             c.precision = p
             return c
     g64, g32 = CF(mk("fp64")), CF(mk("mixed"))

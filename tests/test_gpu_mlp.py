@@ -80,7 +80,7 @@ def test_forward_matches_oracle_and_hull_flags():
 
 @pytest.fixture(scope="module")
 def trained():
-    """The c4 training recipe: 2M EQWM samples over Re_tau 1e3-1e4 (seeded),
+    """The 256x96x256 LES training recipe: 2M EQWM samples over Re_tau 1e3-1e4 (seeded),
     ncv = 30, stretch 1.10, tol 1e-8; Adam, batch 4096."""
     from channel_flow.wall_mlp import WallMLP, eqwm_samples
     t0 = time.time()
@@ -138,7 +138,7 @@ def test_hull_fallback_is_exact_ode(trained):
     gives exactly the ODE result, and counts every point as a fallback."""
     from channel_flow.wall_mlp import WallMLP
     m, _ = trained
-    cfg = cases.case_c3()
+    cfg = cases.case_wmles128()
     cfg.cells = (64, 32, 64)
     cfg.precision = "fp64"
     g = _wall_ctx(cfg)
@@ -164,7 +164,7 @@ def test_hull_fallback_is_exact_ode(trained):
     nf = g.wall_fallbacks()
     tau_m = g.wall_state()[2]
     rel = np.abs(tau_m - ref[2]) / ref[2]
-    print("\n[mlp] c3 wall: %d of %d points fall back; |tau err| median %.3e max %.3e" %
+    print("\n[mlp] wmles128 wall: %d of %d points fall back; |tau err| median %.3e max %.3e" %
           (nf, npts, np.median(rel), rel.max()))
     assert nf < 0.05 * npts
     assert np.median(rel) < 1e-2
@@ -178,7 +178,7 @@ def test_gain_on_c4_wall(trained):
     """What the network gains over the ODE on the 256x96x256 case: wall-model
     time per call (same inputs, same kernel launch path) and the stress error."""
     m, _ = trained
-    cfg = cases.case_c4()
+    cfg = cases.case_wmles256()
     g = _wall_ctx(cfg)
     ti0 = g.wall_state()[2].copy()
     reps = 20
@@ -201,7 +201,7 @@ def test_gain_on_c4_wall(trained):
     t_ode, tau_ode, _ = timed(None)
     t_mlp, tau_mlp, nf = timed(m)
     rel = np.abs(tau_mlp - tau_ode) / tau_ode
-    print("\n[mlp] c4 wall (%d points): ODE %.4f ms, MLP %.4f ms (x%.2f), fallbacks %d, "
+    print("\n[mlp] wmles256 wall (%d points): ODE %.4f ms, MLP %.4f ms (x%.2f), fallbacks %d, "
           "|tau err| median %.3e p99 %.3e" % (tau_ode.size, t_ode, t_mlp, t_ode / t_mlp, nf,
                                               np.median(rel), np.percentile(rel, 99)))
     assert t_mlp < t_ode
@@ -216,7 +216,7 @@ def test_wmles_runs_with_surrogate_track_eqwm(trained):
     faster with the network."""
     from channel_flow.solver import ChannelFlow
     m, _ = trained
-    cfg = cases.case_c4()
+    cfg = cases.case_wmles256()
     steps, start, every = 300, 100, 5
     out = {}
     for name, model in (("ode", None), ("mlp", m)):
@@ -242,7 +242,7 @@ def test_wmles_runs_with_surrogate_track_eqwm(trained):
     (wo, mso, _), (wm, msm, nf) = out["ode"], out["mlp"]
     err = float(np.max(np.abs(wm["Umean"] - wo["Umean"])) / np.max(np.abs(wo["Umean"])))
     err_uu = float(np.max(np.abs(wm["uu"] - wo["uu"])) / np.max(np.abs(wo["uu"])))
-    print("\n[mlp] c4 run, steps %d-%d: mean-profile err %.3e, uu err %.3e; step %.3f ms (ODE) "
+    print("\n[mlp] wmles256 run, steps %d-%d: mean-profile err %.3e, uu err %.3e; step %.3f ms (ODE) "
           "vs %.3f ms (MLP), last-step fallbacks %d" % (start + 1, steps, err, err_uu, mso, msm, nf))
     assert err < 1e-2
     assert msm < mso  # the surrogate makes the whole step faster

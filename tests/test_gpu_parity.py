@@ -16,7 +16,7 @@ def _solver():
 
 
 def small_cfg(**kw):
-    cfg = cases.case_c1()
+    cfg = cases.case_dns64()
     cfg.cells = kw.pop("cells", (32, 24, 16))
     for k, v in kw.items():
         setattr(cfg, k, v)

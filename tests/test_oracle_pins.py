@@ -15,7 +15,7 @@ from oracle_lib import Oracle, lib, P, eqwm
 
 
 def cfg_small(**kw):
-    c = cases.case_c1()
+    c = cases.case_dns64()
     c.cells = kw.pop("cells", (16, 12, 8))
     c.x_mass_cte = kw.pop("x_mass_cte", 0)
     c.dPdx = kw.pop("dPdx", 0.0)

@@ -13,21 +13,21 @@ def _field_bytes(cfg, esize):
 
 
 def test_footprint_covers_the_state_and_scales():
-    c = cases.case_c1()
+    c = cases.case_dns64()
     fp = footprint(c)
     # two ping-pong copies of (U, V, W) plus the RK3 accumulators: at least 3x one state
     assert fp >= 3 * _field_bytes(c, 8)
     assert fp % 256 == 0
-    c2 = cases.case_c1()
-    c2.cells = (128, 128, 128)
-    assert 7.0 < footprint(c2) / fp < 9.0  # 8x the cells
-    m = cases.case_c1()
+    big = cases.case_dns64()
+    big.cells = (128, 128, 128)
+    assert 7.0 < footprint(big) / fp < 9.0  # 8x the cells
+    m = cases.case_dns64()
     m.precision = "mixed"
     assert footprint(m) < 0.7 * fp
 
 
 def test_footprint_of_large_cases():
-    c = cases.case_c2()
+    c = cases.case_dns384()
     fp = footprint(c)
     assert 3 * _field_bytes(c, 8) < fp < 20 * _field_bytes(c, 8) / 3
     assert fp < 6e9
@@ -65,7 +65,7 @@ def test_invalid_parameters_are_rejected():
     import pytest
 
     def err(**kw):
-        c = cases.case_c1()
+        c = cases.case_dns64()
         c.cells = (32, 24, 16)
         for k, v in kw.items():
             setattr(c, k, v)
@@ -79,7 +79,7 @@ def test_invalid_parameters_are_rejected():
     assert "FFT" in err(cells=(30, 24, 14 * 5))
     assert "nu" in err(nu=0.0)
     assert "eqwm_ncv" in err(iwall_model=3, eqwm_ncv=500)
-    c = cases.case_c1()
+    c = cases.case_dns64()
     c.precision = "fp16"
     with pytest.raises(ValueError):
         footprint(c)

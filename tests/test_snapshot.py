@@ -6,7 +6,7 @@ from channel_flow.snapshot import read_snapshot, write_snapshot
 
 
 def test_snapshot_roundtrip(tmp_path):
-    cfg = cases.case_c1()
+    cfg = cases.case_dns64()
     cfg.cells = (8, 6, 4)
     x, y, z = cfg.grid()
     U, V, W = cases.initial_field(cfg)
