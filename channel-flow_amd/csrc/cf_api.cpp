@@ -277,7 +277,7 @@ int Ctx<S>::init() {
         A(&m_jjx, h.jjx.size()) || A(&m_wx, h.wx.size()))
         return -1;
     if (A(&twM, 2 * (size_t)M + 2) || A(&twN, (size_t)d.nxp + 1) || A(&twZ, 2 * (size_t)d.nzp + 2)) return -1;
-    if (h.prm.les_model == 2 && (A(&dsm.Q, 16 * nC) || A(&dsm.Fu, nU + nV + nW) ||
+    if (h.prm.les_model == 2 && (A(&dsm.Q, 10 * nC) || A(&dsm.Fu, nU + nV + nW) ||
                                  A(&dsm.part, dsm_part_elems(d)) || A(&dsm.avg, (size_t)d.nyg)))
         return -1;
     if (A(&alpha_x, nwx) || A(&alpha_z, nwz) || A(&azt, nwx) || A(&tau_init, nwx)) return -1;

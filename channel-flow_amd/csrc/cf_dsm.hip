@@ -15,13 +15,13 @@
 // (the mixed path): Lij is a difference of nearly equal filtered products.
 //
 // Kernels, in launch order:
-//   k_dsm_strain<RAW>  centre velocities, |S| and |S|Sij of the field at
+//   k_dsm_strain       centre velocities, |S| and |S|Sij of the field at
 //                      every centre (ghost columns and planes are copies of
 //                      their periodic images)              -> Q[0..9]
 //   k_dsm_filter_vel   filtered U, V, W on their own grids, all rows -> Fu
-//   k_dsm_strain<!RAW> 4 |hat S| hat(S)ij from Fu         -> Q[10..15]
 //   k_dsm_lm           per cell Lij and Mij from the LDS-staged rows of
-//                      Q[0..8]; LM and MM summed over each wave's 64 cells
+//                      Q[0..8] and of Fu (4 |hat S| hat(S)ij formed in
+//                      place); LM and MM summed over each wave's 64 cells
 //                      of a row                            -> part
 //   k_dsm_avg          one block per row: fixed-order sum of the wave
 //                      partials, C(y) = LM/MM              -> avg
@@ -90,55 +90,101 @@ __device__ __forceinline__ double fil_xz(int a, int c) {
 // ---------------------------------------------------------------------------
 // Filtered velocities (subgrid.f90:428-443) on the full grid of one of U, V,
 // W (COMP 0, 1, 2), all rows, one-sided y weights on the first and last.
-// The filter factors into (1/6 2/3 1/6) per axis: each thread owns one
-// (x, y) point, forms the x-y filtered value of every plane it passes and
-// marches a chunk of kch unique z planes, so each output costs nine loads.  Ghost
-// columns and planes take the values of their periodic images.  Tiles
-// (x, y, k chunk), k chunk fastest.
+// The filter factors into (1/6 2/3 1/6) per axis.  A block owns 64 unique
+// columns by 4 rows and marches a chunk of kch unique z planes: each plane's
+// 66 x 6 tile (one-cell x-y halo, actual neighbours) is staged in LDS with
+// the next plane's loads in flight; each thread forms the x-y filtered value
+// of its point from LDS and combines three planes in z in registers.  The
+// value of a unique column or plane is also written at its periodic ghost
+// copies.  Tiles (x, y, k chunk), k chunk fastest.
 // ---------------------------------------------------------------------------
+constexpr int FV_HX = 66, FV_HY = 6, FV_NP = FV_HX * FV_HY;
+
 template <typename S, int COMP>
 __global__ void __launch_bounds__(DSM_WG)
 k_dsm_filter_vel(Dims d, const S *__restrict__ F, const double *__restrict__ ubar, double *__restrict__ Fo,
                  int tiles_x, int nchunks, int ntiles, int kch) {
+    __shared__ double tile[2][FV_NP];  // planes K and K+1 (slot K & 1)
     const int n1 = COMP == 0 ? d.nx : d.nxg, n2 = COMP == 1 ? d.ny : d.nyg, n3 = COMP == 2 ? d.nz : d.nzg;
     const int t = xcd_remap(blockIdx.x, ntiles);
     const int ch = t % nchunks, r = t / nchunks;
-    const int I = 1 + (r % tiles_x) * 64 + (threadIdx.x & 63);
-    const int J = 1 + (r / tiles_x) * 4 + (threadIdx.x >> 6);
-    if (I > n1 || J > n2) return;
-    const int ii = COMP == 0 ? img_f(I, n1) : img_c(I, n1);
+    const int bx = r % tiles_x, by = r / tiles_x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ih = 1 + bx * 64, jh = by * 4;  // halo origin (column, row)
+    const int I = ih + 1 + lane, J = jh + 1 + wv;
+    const int iu = COMP == 0 ? n1 - 1 : n1 - 2;  // last unique column
+    const bool act = I <= iu && J <= n2;
     const int nu = COMP == 2 ? n3 - 1 : n3 - 2;  // last unique plane
     const int k0 = 2 + ch * kch, k1 = min(k0 + kch - 1, nu);
     const double wm = J == 1 ? 0.0 : (J == n2 ? 1.0 / 3.0 : 1.0 / 6.0);
     const double wp = J == n2 ? 0.0 : (J == 1 ? 1.0 / 3.0 : 1.0 / 6.0);
-    const int jm = J > 1 ? J - 1 : J, jp = J < n2 ? J + 1 : J;
-    auto f = [&](int a, int b, int c) {
-        return (double)F[i3(n1, n2, a, b, c)] + ((COMP == 0 && ubar) ? ubar[b] : 0.0);
+    const int rm = (J > 1 ? J - 1 : J) - jh, r0 = J - jh, rp = (J < n2 ? J + 1 : J) - jh;  // tile rows
+    constexpr int NPF = (FV_NP + DSM_WG - 1) / DSM_WG;
+    double pf[NPF];
+    auto fetch = [&](int K) {
+#pragma unroll
+        for (int u = 0; u < NPF; u++) {
+            const int p = threadIdx.x + u * DSM_WG;
+            pf[u] = 0.0;
+            if (p < FV_NP) {
+                const int py = p / FV_HX, px = p - py * FV_HX;
+                const int Ia = ih + px, Ja = jh + py;
+                if (Ia <= n1 && Ja >= 1 && Ja <= n2)
+                    pf[u] = (double)F[i3(n1, n2, Ia, Ja, K)] + ((COMP == 0 && ubar) ? ubar[Ja] : 0.0);
+            }
+        }
+    };
+    // both slots are read between two barriers at most once each, so one
+    // barrier after each write suffices
+    auto commit = [&](int K) {
+#pragma unroll
+        for (int u = 0; u < NPF; u++) {
+            const int p = threadIdx.x + u * DSM_WG;
+            if (p < FV_NP) tile[K & 1][p] = pf[u];
+        }
+        __syncthreads();
     };
     auto X = [&](int K) {
+        const double *tp = tile[K & 1];
         double acc = 0.0;
 #pragma unroll
         for (int a = -1; a <= 1; a++) {
-            const double y = wm * f(ii + a, jm, K) + 2.0 / 3.0 * f(ii + a, J, K) + wp * f(ii + a, jp, K);
+            const int c = lane + 1 + a;
+            const double y = wm * tp[rm * FV_HX + c] + 2.0 / 3.0 * tp[r0 * FV_HX + c] + wp * tp[rp * FV_HX + c];
             acc = acc + (a == 0 ? 2.0 / 3.0 : 1.0 / 6.0) * y;
         }
         return acc;
     };
-    double xm = X(k0 - 1), x0 = X(k0);
+    // the column and its periodic ghost copy (faces: 1 <- n1-1, n1 <- 2;
+    // centres: 1 <- n1-2, n1-1 <- 2, n1 <- 3)
+    const int ig = COMP == 0 ? (I == 2 ? n1 : (I == n1 - 1 ? 1 : 0))
+                             : (I == 2 ? n1 - 1 : (I == 3 ? n1 : (I == n1 - 2 ? 1 : 0)));
+    fetch(k0 - 1);
+    commit(k0 - 1);
+    fetch(k0);
+    double xm = act ? X(k0 - 1) : 0.0;
+    commit(k0);
+    fetch(k0 + 1);
+    double x0 = act ? X(k0) : 0.0;
     for (int K = k0; K <= k1; K++) {
-        const double xp = X(K + 1);
-        const double out = 1.0 / 6.0 * xm + 2.0 / 3.0 * x0 + 1.0 / 6.0 * xp;
-        Fo[i3(n1, n2, I, J, K)] = out;
-        if (COMP == 2) {  // faces: 1 <- n3-1, n3 <- 2
-            if (K == 2) Fo[i3(n1, n2, I, J, n3)] = out;
-            if (K == n3 - 1) Fo[i3(n1, n2, I, J, 1)] = out;
-        } else {  // centres: 1 <- n3-2, n3-1 <- 2, n3 <- 3
-            if (K == 2) Fo[i3(n1, n2, I, J, n3 - 1)] = out;
-            if (K == 3) Fo[i3(n1, n2, I, J, n3)] = out;
-            if (K == n3 - 2) Fo[i3(n1, n2, I, J, 1)] = out;
+        commit(K + 1);
+        if (K + 2 <= k1 + 1) fetch(K + 2);
+        if (act) {
+            const double xp = X(K + 1);
+            const double out = 1.0 / 6.0 * xm + 2.0 / 3.0 * x0 + 1.0 / 6.0 * xp;
+            // the plane and its periodic ghost copy (faces: 1 <- n3-1,
+            // n3 <- 2; centres: 1 <- n3-2, n3-1 <- 2, n3 <- 3)
+            const int kg = COMP == 2 ? (K == 2 ? n3 : (K == n3 - 1 ? 1 : 0))
+                                     : (K == 2 ? n3 - 1 : (K == 3 ? n3 : (K == n3 - 2 ? 1 : 0)));
+            Fo[i3(n1, n2, I, J, K)] = out;
+            if (ig) Fo[i3(n1, n2, ig, J, K)] = out;
+            if (kg) {
+                Fo[i3(n1, n2, I, J, kg)] = out;
+                if (ig) Fo[i3(n1, n2, ig, J, kg)] = out;
+            }
+            xm = x0;
+            x0 = xp;
         }
-        xm = x0;
-        x0 = xp;
     }
 }
 
@@ -151,24 +197,20 @@ __device__ __forceinline__ double wave_sum(double x) {
 // ---------------------------------------------------------------------------
 // compute_Sij (subgrid.f90:891-949) on a 2.5-D march: a block owns 64 x 4
 // unique centres in (x, y) and walks a chunk of `planes` z planes; U, V, W
-// sit in an LDS ring of three planes (z-1, z, z+1) with a one-cell x-y halo,
-// the next plane's loads issued into registers while the current plane is
-// computed.
-// Every input value is read from HBM about once (plus the halo).
-//   RAW (the field, FP32 deviations + ubar or FP64): centre velocities,
-//     |S|Sij and |S| into Q[0..9] at the centre and at its periodic ghost
-//     copies (columns 1, nxg-1, nxg and planes 1, nzg-1, nzg).
-//   !RAW (the filtered field, FP64): 4 |hat S| hat(S)ij into Q[10..15] at
-//     the unique centres (fil_size**2 = 4, subgrid.f90:449-451).
+// (FP32 deviations + ubar, or FP64) sit in an LDS ring of three planes (z-1,
+// z, z+1) with a one-cell x-y halo, the next plane's loads issued into
+// registers while the current plane is computed.  Every input value is read
+// from HBM about once (plus the halo).  Centre velocities, |S|Sij and |S| go
+// to Q[0..9] at the centre and at its periodic ghost copies (columns 1,
+// nxg-1, nxg and planes 1, nzg-1, nzg).
 // Tiles (x, y, z chunk), z chunk fastest.
-// Q layout: 16 centre arrays (nxg, nyg, nzg), rows 2..nyg-1 written:
+// Q layout: 10 centre arrays (nxg, nyg, nzg), rows 2..nyg-1 written:
 //   0..2 centre velocities (interpolate_x/y/z, subgrid.f90:391-393)
-//   3..8 |S| Sij, 9 |S|       (the raw field; every centre)
-//   10..15 4 |hat S| hat(S)ij (the filtered field; unique centres only)
+//   3..8 |S| Sij, 9 |S|
 // ---------------------------------------------------------------------------
 constexpr int SK_HX = 66, SK_HY = 6, SK_NP = SK_HX * SK_HY;
 
-template <typename T, bool RAW>
+template <typename T>
 __global__ void __launch_bounds__(DSM_WG)
 k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V, const T *__restrict__ W,
              const double *__restrict__ ubar, double *__restrict__ Q, int64_t nC, int tiles_x, int nchunks,
@@ -197,7 +239,7 @@ k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V
                 const int I = ih + px, J = jh + py;
                 if (a == 0) {
                     if (I <= nx && J <= nyg && K <= nzg)
-                        pf[u] = (double)U[i3(nx, nyg, I, J, K)] + ((RAW && ubar) ? ubar[J] : 0.0);
+                        pf[u] = (double)U[i3(nx, nyg, I, J, K)] + (ubar ? ubar[J] : 0.0);
                 } else if (a == 1) {
                     if (I <= nxg && J <= ny && K <= nzg) pf[u] = (double)V[i3(nxg, ny, I, J, K)];
                 } else {
@@ -234,31 +276,25 @@ k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V
         if (act) {
             double s[6];
             const double sm = strain6(m, i, j, k, u, v, w, s);
-            if (RAW) {
-                const double cu = 0.5 * (u(i - 1, j, k) + u(i, j, k));
-                const double cv = 0.5 * (v(i, j - 1, k) + v(i, j, k));
-                const double cw = 0.5 * (w(i, j, k - 1) + w(i, j, k));
-                // the centre and its periodic ghost copies
-                const int ia[2] = {i, i == 2 ? nxg - 1 : (i == 3 ? nxg : (i == nxg - 2 ? 1 : 0))};
-                const int ka[2] = {k, k == 2 ? nzg - 1 : (k == 3 ? nzg : (k == nzg - 2 ? 1 : 0))};
+            const double cu = 0.5 * (u(i - 1, j, k) + u(i, j, k));
+            const double cv = 0.5 * (v(i, j - 1, k) + v(i, j, k));
+            const double cw = 0.5 * (w(i, j, k - 1) + w(i, j, k));
+            // the centre and its periodic ghost copies
+            const int ia[2] = {i, i == 2 ? nxg - 1 : (i == 3 ? nxg : (i == nxg - 2 ? 1 : 0))};
+            const int ka[2] = {k, k == 2 ? nzg - 1 : (k == 3 ? nzg : (k == nzg - 2 ? 1 : 0))};
 #pragma unroll
-                for (int a = 0; a < 2; a++)
+            for (int a = 0; a < 2; a++)
 #pragma unroll
-                    for (int c = 0; c < 2; c++) {
-                        if (ia[a] == 0 || ka[c] == 0) continue;
-                        const int64_t q = i3(nxg, nyg, ia[a], j, ka[c]);
-                        Q[q] = cu;
-                        Q[nC + q] = cv;
-                        Q[2 * nC + q] = cw;
+                for (int c = 0; c < 2; c++) {
+                    if (ia[a] == 0 || ka[c] == 0) continue;
+                    const int64_t q = i3(nxg, nyg, ia[a], j, ka[c]);
+                    Q[q] = cu;
+                    Q[nC + q] = cv;
+                    Q[2 * nC + q] = cw;
 #pragma unroll
-                        for (int e = 0; e < 6; e++) Q[(3 + e) * nC + q] = sm * s[e];
-                        Q[9 * nC + q] = sm;
-                    }
-            } else {
-                const int64_t q = i3(nxg, nyg, i, j, k);
-#pragma unroll
-                for (int e = 0; e < 6; e++) Q[(10 + e) * nC + q] = 4.0 * sm * s[e];
-            }
+                    for (int e = 0; e < 6; e++) Q[(3 + e) * nC + q] = sm * s[e];
+                    Q[9 * nC + q] = sm;
+                }
         }
     }
 }
@@ -270,18 +306,23 @@ constexpr int LM_NQ = 9;  // staged quantities: centre velocities and |S|Sij
 // ---------------------------------------------------------------------------
 // Lij, Mij and their plane products.  Block: 64 columns in x by 4 in z (one
 // wave per z column row); the block marches the `rows` rows of one row
-// chunk.  Each row of the nine staged quantities is loaded once per block
-// into LDS with its halo; the global loads of the next row are issued into
-// registers before the current row is computed and written to LDS after it.
-// Every thread forms its x-z filtered sums (products included) from LDS and
-// folds them into the streaming y combination below.
+// chunk.  Each row of the nine staged quantities, and of the filtered U, V,
+// W, is loaded once per block into LDS with its halo; the global loads of
+// the next row are issued into registers before the current row is computed
+// and written to LDS after it.  Every thread forms its x-z filtered sums
+// (products included) from LDS and folds them into the streaming y
+// combination below.  The filtered fields sit in a ring of three rows, from
+// which compute_Sij gives 4 |hat S| hat(S)ij of the completed row in place
+// (fil_size**2 = 4, subgrid.f90:449-451).
 // part[(cw * nyp + j - 2) * 2 + {0, 1}] = wave sums of LM, MM, with
 // cw = (4 bz + wave) * tiles_x + bx.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(DSM_WG)
-k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int tiles_x, int tiles_z, int nchunks,
-         int rows) {
+k_dsm_lm(Dims d, Metrics m, const double *__restrict__ Q, const double *__restrict__ Uf,
+         const double *__restrict__ Vf, const double *__restrict__ Wf, double *__restrict__ part, int tiles_x,
+         int tiles_z, int nchunks, int rows) {
     __shared__ double sq[LM_NQ][LM_NP];
+    __shared__ double fr[3][3][LM_NP];  // filtered U, V, W: ring of rows r-2, r-1, r
     const int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
     const int64_t nC = (int64_t)nxg * nyg * nzg;
     // tiles (z, row chunk, x), z fastest: neighbouring z tiles share halo planes
@@ -299,33 +340,75 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
     // stage row r (block-uniform) into LDS; points outside the arrays read 0
     // and are never used by an active thread
     constexpr int NPF = (LM_NQ * LM_NP + DSM_WG - 1) / DSM_WG;
-    double pf[NPF];
-    // global loads of row r into registers (block-uniform r)
-    auto fetch = [&](int r) {
+    constexpr int NPG = (3 * LM_NP + DSM_WG - 1) / DSM_WG;
+    double pf[NPF], pg[NPG];
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    // global loads of row r into registers (block-uniform r and flags): the
+    // staged quantities when wq, the filtered fields when wf
+    auto fetch = [&](int r, bool wq, bool wf) {
+        if (wq) {
 #pragma unroll
-        for (int u = 0; u < NPF; u++) {
-            const int p = threadIdx.x + u * DSM_WG;
-            pf[u] = 0.0;
-            if (p < LM_NQ * LM_NP) {
-                const int qa = p / LM_NP, pt = p - qa * LM_NP;
-                const int pz = pt / LM_HX, px = pt - pz * LM_HX;
-                const int I = ih + px, K = kh + pz;
-                if (I <= nxg && K <= nzg) pf[u] = Q[qa * nC + i3(nxg, nyg, I, r, K)];
+            for (int u = 0; u < NPF; u++) {
+                const int p = threadIdx.x + u * DSM_WG;
+                pf[u] = 0.0;
+                if (p < LM_NQ * LM_NP) {
+                    const int qa = p / LM_NP, pt = p - qa * LM_NP;
+                    const int pz = pt / LM_HX, px = pt - pz * LM_HX;
+                    const int I = ih + px, K = kh + pz;
+                    if (I <= nxg && K <= nzg) pf[u] = Q[qa * nC + i3(nxg, nyg, I, r, K)];
+                }
+            }
+        }
+        if (wf) {
+#pragma unroll
+            for (int u = 0; u < NPG; u++) {
+                const int p = threadIdx.x + u * DSM_WG;
+                pg[u] = 0.0;
+                if (p < 3 * LM_NP) {
+                    const int a = p / LM_NP, pt = p - a * LM_NP;
+                    const int pz = pt / LM_HX, px = pt - pz * LM_HX;
+                    const int I = ih + px, K = kh + pz;
+                    if (a == 0) {
+                        if (I <= nx && K <= nzg) pg[u] = Uf[i3(nx, nyg, I, r, K)];
+                    } else if (a == 1) {
+                        if (I <= nxg && r <= ny && K <= nzg) pg[u] = Vf[i3(nxg, ny, I, r, K)];
+                    } else {
+                        if (I <= nxg && K <= nz) pg[u] = Wf[i3(nxg, nyg, I, r, K)];
+                    }
+                }
             }
         }
     };
-    auto commit = [&]() {
+    auto commit = [&](int r, bool wq, bool wf) {
         __syncthreads();
+        if (wq) {
 #pragma unroll
-        for (int u = 0; u < NPF; u++) {
-            const int p = threadIdx.x + u * DSM_WG;
-            if (p < LM_NQ * LM_NP) {
-                const int qa = p / LM_NP, pt = p - qa * LM_NP;
-                sq[qa][pt] = pf[u];
+            for (int u = 0; u < NPF; u++) {
+                const int p = threadIdx.x + u * DSM_WG;
+                if (p < LM_NQ * LM_NP) {
+                    const int qa = p / LM_NP, pt = p - qa * LM_NP;
+                    sq[qa][pt] = pf[u];
+                }
+            }
+        }
+        if (wf) {
+            const int sl = r % 3;
+#pragma unroll
+            for (int u = 0; u < NPG; u++) {
+                const int p = threadIdx.x + u * DSM_WG;
+                if (p < 3 * LM_NP) {
+                    const int a = p / LM_NP, pt = p - a * LM_NP;
+                    fr[a][sl][pt] = pg[u];
+                }
             }
         }
         __syncthreads();
     };
+    // filtered fields at (I, J, K) from the ring (J in r-2..r)
+    auto fat = [&](int a, int I, int J, int K) { return fr[a][J % 3][(K - kh) * LM_HX + (I - ih)]; };
+    auto uf = [&](int I, int J, int K) { return fat(0, I, J, K); };
+    auto vf = [&](int I, int J, int K) { return fat(1, I, J, K); };
+    auto wf_ = [&](int I, int J, int K) { return fat(2, I, J, K); };
     // Streaming y combination: hat(j) = am(j) F(j-1) + 2/3 F(j) + ap(j) F(j+1)
     // with F the x-z filtered row sums.  P1 holds the partial hat of the row
     // below the staged one, P2 that of the staged row; when row r is staged,
@@ -339,19 +422,31 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
 #pragma unroll
     for (int c = 0; c < 15; c++) { P1[c] = 0.0; P2[c] = 0.0; }
     const int rs = j0 > 2 ? j0 - 1 : j0;
-    const int rlast = min(j1 + 1, nyg - 1);  // last row to stage
-    fetch(rs);
-    commit();
+    const int rlast = min(j1 + 1, nyg - 1);  // last row of Q to stage
+    const int rflast = j1 + 1;               // last filtered row to stage (<= nyg)
+    if (j0 == 2) {                           // filtered row 1 (the wall ghost row)
+        fetch(1, false, true);
+        commit(1, false, true);
+    }
+    fetch(rs, true, true);
+    commit(rs, true, true);
     for (int r = rs; r <= j1 + 1; r++) {
         const bool have = r <= nyg - 1;  // block-uniform
-        if (r + 1 <= rlast) fetch(r + 1);
+        const bool nq = r + 1 <= rlast, nf = r + 1 <= rflast;
+        if (nf) fetch(r + 1, nq, true);
         const int jd = r - 1;            // row completed by this one
         const bool use = act && jd >= j0 && jd <= j1;
         const double wap = use ? ap(jd) : 0.0, wam = am(r + 1);
+        // 4 |hat S| hat(S)ij at (i, jd, k) from the filtered rows jd-1..jd+1
         double T[6];
-        const int64_t qd = i3(nxg, nyg, i, jd, k);
+        if (use) {
+            const double sm = strain6(m, i, jd, k, uf, vf, wf_, T);
 #pragma unroll
-        for (int c = 0; c < 6; c++) T[c] = use ? Q[(10 + c) * nC + qd] : 0.0;
+            for (int c = 0; c < 6; c++) T[c] = 4.0 * sm * T[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; c++) T[c] = 0.0;
+        }
         double hv[3], lm = 0.0, mm = 0.0;
         const int pc = (wv + 1) * LM_HX + lane + 1;  // own point in the staged tile
         auto nb = [&](int e, int q) { return sq[e][pc + (q % 3 - 1) * LM_HX + q / 3 - 1]; };
@@ -385,7 +480,7 @@ k_dsm_lm(Dims d, const double *__restrict__ Q, double *__restrict__ part, int ti
             lm += wc * (L * M);
             mm += wc * (M * M);
         }
-        if (r + 1 <= rlast) commit();
+        if (nf) commit(r + 1, nq, true);
         if (jd >= j0 && jd <= j1) {  // block-uniform
             lm = use ? lm * wgt : 0.0;
             mm = use ? mm * wgt : 0.0;
@@ -467,23 +562,22 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
     const int txc = (d.nxg + 63) / 64;
     const int stx = (d.nxg - 3 + 63) / 64, sty = (d.nyg - 2 + 3) / 4;
     const int spl = dsm_chunk(d.nzg - 3, stx * sty), skc = (d.nzg - 3 + spl - 1) / spl, snt = stx * sty * skc;
-    hipLaunchKernelGGL((k_dsm_strain<S, true>), dim3(snt), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q, nC, stx,
+    hipLaunchKernelGGL((k_dsm_strain<S>), dim3(snt), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q, nC, stx,
                        skc, snt, spl);
     double *Uf = w.Fu, *Vf = Uf + (int64_t)d.nx * d.nyg * d.nzg, *Wf = Vf + (int64_t)d.nxg * d.ny * d.nzg;
     auto fv = [&](auto kern, const S *F, double *Fo, int n1, int n2) {
-        const int tx = (n1 + 63) / 64, tiles = tx * ((n2 + 3) / 4);
+        const int tx = (d.nxg - 3 + 63) / 64, tiles = tx * ((n2 + 3) / 4);  // nxg - 3 unique columns
         const int kch = dsm_chunk(d.nzp, tiles), nkc = (d.nzp + kch - 1) / kch;
         hipLaunchKernelGGL(kern, dim3(tiles * nkc), dim3(DSM_WG), 0, st, d, F, ubar, Fo, tx, nkc, tiles * nkc, kch);
     };
     fv(k_dsm_filter_vel<S, 0>, U, Uf, d.nx, d.nyg);
     fv(k_dsm_filter_vel<S, 1>, V, Vf, d.nxg, d.ny);
     fv(k_dsm_filter_vel<S, 2>, W, Wf, d.nxg, d.nyg);
-    hipLaunchKernelGGL((k_dsm_strain<double, false>), dim3(snt), dim3(DSM_WG), 0, st, d, m, (const double *)Uf,
-                       (const double *)Vf, (const double *)Wf, (const double *)nullptr, w.Q, nC, stx, skc, snt, spl);
     const int tx = dsm_tiles_x(d), tz = dsm_tiles_z(d);
     const int nyp = d.nyg - 2;
     const int rows = dsm_chunk(nyp, tx * tz), chunks = (nyp + rows - 1) / rows;
-    hipLaunchKernelGGL(k_dsm_lm, dim3(tx * tz * chunks), dim3(DSM_WG), 0, st, d, w.Q, w.part, tx, tz, chunks, rows);
+    hipLaunchKernelGGL(k_dsm_lm, dim3(tx * tz * chunks), dim3(DSM_WG), 0, st, d, m, w.Q, (const double *)Uf,
+                       (const double *)Vf, (const double *)Wf, w.part, tx, tz, chunks, rows);
     hipLaunchKernelGGL(k_dsm_avg, dim3(nyp), dim3(DSM_WG), 0, st, tx * tz * 4, nyp, w.part, w.avg);
     {
         const int nt = txc * ((d.nyg + 3) / 4) * d.nzg;

@@ -16,8 +16,7 @@ constexpr bool FFT_X_RADIX8 = true;
 
 // ---- dynamic Smagorinsky (cf_dsm.hip) ----
 struct DsmWork {
-    double *Q;     // 16 centre arrays (nxg*nyg*nzg each): centre velocities, |S|Sij, |S|,
-                   // 4 |hat S| hat(S)ij
+    double *Q;     // 10 centre arrays (nxg*nyg*nzg each): centre velocities, |S|Sij, |S|
     double *Fu;    // filtered U, V, W on their own grids, back to back
     double *part;  // dsm_part_elems(d): per-wave plane partials of LM, MM
     double *avg;   // nyg: <LM>/<MM> per row (rows 1 and nyg are ghosts)

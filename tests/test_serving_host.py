@@ -49,14 +49,14 @@ def test_ensemble_is_seeded_and_mixed():
 
 
 def test_footprint_of_the_dynamic_model():
-    """les_model 2 adds the model's work arrays: 16 centre fields, the
+    """les_model 2 adds the model's work arrays: 10 centre fields, the
     filtered velocities and the plane partials."""
     c = cases.case_example()
     s = cases.case_example()
     s.les_model = 1
     NX, NY, NZ = c.cells
     ncent = (NX + 3) * (NY + 2) * (NZ + 3)
-    assert footprint(c) - footprint(s) >= 8 * (16 * ncent) + _field_bytes(c, 8)
+    assert footprint(c) - footprint(s) >= 8 * (10 * ncent) + _field_bytes(c, 8)
 
 
 def test_invalid_parameters_are_rejected():
