@@ -279,22 +279,26 @@ k_dsm_strain(Dims d, Metrics m, const T *__restrict__ U, const T *__restrict__ V
             const double cu = 0.5 * (u(i - 1, j, k) + u(i, j, k));
             const double cv = 0.5 * (v(i, j - 1, k) + v(i, j, k));
             const double cw = 0.5 * (w(i, j, k - 1) + w(i, j, k));
-            // the centre and its periodic ghost copies
-            const int ia[2] = {i, i == 2 ? nxg - 1 : (i == 3 ? nxg : (i == nxg - 2 ? 1 : 0))};
-            const int ka[2] = {k, k == 2 ? nzg - 1 : (k == 3 ? nzg : (k == nzg - 2 ? 1 : 0))};
+            double val[10] = {cu, cv, cw, sm * s[0], sm * s[1], sm * s[2], sm * s[3], sm * s[4], sm * s[5], sm};
+            {
+                const int64_t q = i3(nxg, nyg, i, j, k);
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+                for (int e = 0; e < 10; e++) Q[e * nC + q] = val[e];
+            }
+            // periodic ghost copies of the centre (rare: three columns and
+            // three planes)
+            const int ig = i == 2 ? nxg - 1 : (i == 3 ? nxg : (i == nxg - 2 ? 1 : 0));
+            const int kg = k == 2 ? nzg - 1 : (k == 3 ? nzg : (k == nzg - 2 ? 1 : 0));
+            if (ig | kg) {
+#pragma unroll 1
+                for (int g = 0; g < 3; g++) {
+                    const int I = g == 1 ? i : ig, K = g == 0 ? k : kg;
+                    if (I == 0 || K == 0) continue;
+                    double *q = Q + i3(nxg, nyg, I, j, K);
 #pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    if (ia[a] == 0 || ka[c] == 0) continue;
-                    const int64_t q = i3(nxg, nyg, ia[a], j, ka[c]);
-                    Q[q] = cu;
-                    Q[nC + q] = cv;
-                    Q[2 * nC + q] = cw;
-#pragma unroll
-                    for (int e = 0; e < 6; e++) Q[(3 + e) * nC + q] = sm * s[e];
-                    Q[9 * nC + q] = sm;
+                    for (int e = 0; e < 10; e++) q[e * nC] = val[e];
                 }
+            }
         }
     }
 }
