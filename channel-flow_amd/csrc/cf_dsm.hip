@@ -39,10 +39,11 @@ namespace cf {
 namespace {
 
 constexpr int DSM_WG = 256;
-// The marching kernels walk chunks of up to DSM_MAX_CHUNK planes or rows;
-// on small grids the chunks shorten until there are DSM_MIN_BLOCKS
-// workgroups (down to 2 rows or planes per chunk).
-constexpr int DSM_MAX_CHUNK = 16, DSM_MIN_BLOCKS = 512;
+// The marching kernels walk chunks of up to *_CHUNK planes or rows (strain
+// and filter marches in z, the LM march in y); on small grids the chunks
+// shorten until there are DSM_MIN_BLOCKS workgroups (down to 2 rows or
+// planes per chunk).
+constexpr int SK_CHUNK = 32, FV_CHUNK = 16, LM_CHUNK = 32, DSM_MIN_BLOCKS = 512;
 
 // periodic image of a centre index (ghosts 1, n-1, n of a centred axis of n
 // points: 1 <- n-2, n-1 <- 2, n <- 3) and of a face index (1 <- n-1, n <- 2)
@@ -547,8 +548,8 @@ k_dsm_nut(Dims d, const double *__restrict__ Q, double *__restrict__ avg, S *__r
 
 inline int dsm_tiles_x(const Dims &d) { return (d.nxg - 3 + 63) / 64; }
 // chunk length of a march over n planes or rows with `tiles` tiles per plane
-inline int dsm_chunk(int n, int tiles) {
-    const int64_t len = std::min<int64_t>(DSM_MAX_CHUNK, (int64_t)n * tiles / DSM_MIN_BLOCKS);
+inline int dsm_chunk(int n, int tiles, int maxc) {
+    const int64_t len = std::min<int64_t>(maxc, (int64_t)n * tiles / DSM_MIN_BLOCKS);
     return (int)std::max<int64_t>(2, len);
 }
 inline int dsm_tiles_z(const Dims &d) { return (d.nzg - 3 + 3) / 4; }
@@ -565,13 +566,13 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
     const int64_t nC = (int64_t)d.nxg * d.nyg * d.nzg;
     const int txc = (d.nxg + 63) / 64;
     const int stx = (d.nxg - 3 + 63) / 64, sty = (d.nyg - 2 + 3) / 4;
-    const int spl = dsm_chunk(d.nzg - 3, stx * sty), skc = (d.nzg - 3 + spl - 1) / spl, snt = stx * sty * skc;
+    const int spl = dsm_chunk(d.nzg - 3, stx * sty, SK_CHUNK), skc = (d.nzg - 3 + spl - 1) / spl, snt = stx * sty * skc;
     hipLaunchKernelGGL((k_dsm_strain<S>), dim3(snt), dim3(DSM_WG), 0, st, d, m, U, V, W, ubar, w.Q, nC, stx,
                        skc, snt, spl);
     double *Uf = w.Fu, *Vf = Uf + (int64_t)d.nx * d.nyg * d.nzg, *Wf = Vf + (int64_t)d.nxg * d.ny * d.nzg;
     auto fv = [&](auto kern, const S *F, double *Fo, int n1, int n2) {
         const int tx = (d.nxg - 3 + 63) / 64, tiles = tx * ((n2 + 3) / 4);  // nxg - 3 unique columns
-        const int kch = dsm_chunk(d.nzp, tiles), nkc = (d.nzp + kch - 1) / kch;
+        const int kch = dsm_chunk(d.nzp, tiles, FV_CHUNK), nkc = (d.nzp + kch - 1) / kch;
         hipLaunchKernelGGL(kern, dim3(tiles * nkc), dim3(DSM_WG), 0, st, d, F, ubar, Fo, tx, nkc, tiles * nkc, kch);
     };
     fv(k_dsm_filter_vel<S, 0>, U, Uf, d.nx, d.nyg);
@@ -579,7 +580,7 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
     fv(k_dsm_filter_vel<S, 2>, W, Wf, d.nxg, d.nyg);
     const int tx = dsm_tiles_x(d), tz = dsm_tiles_z(d);
     const int nyp = d.nyg - 2;
-    const int rows = dsm_chunk(nyp, tx * tz), chunks = (nyp + rows - 1) / rows;
+    const int rows = dsm_chunk(nyp, tx * tz, LM_CHUNK), chunks = (nyp + rows - 1) / rows;
     hipLaunchKernelGGL(k_dsm_lm, dim3(tx * tz * chunks), dim3(DSM_WG), 0, st, d, m, w.Q, (const double *)Uf,
                        (const double *)Vf, (const double *)Wf, w.part, tx, tz, chunks, rows);
     hipLaunchKernelGGL(k_dsm_avg, dim3(nyp), dim3(DSM_WG), 0, st, tx * tz * 4, nyp, w.part, w.avg);
