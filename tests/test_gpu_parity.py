@@ -180,11 +180,16 @@ def test_smagorinsky_matches():
     assert relerr(g.nu_t, o.nu_t) < 1e-12
 
 
-@pytest.mark.parametrize("cells,dirichlet", [((32, 24, 16), 1), ((36, 22, 18), 0)])
+@pytest.mark.parametrize("cells,dirichlet", [((32, 24, 16), 1), ((36, 22, 18), 0), ((144, 20, 72), 1),
+                                             ((64, 96, 384), 0)])
 def test_dynamic_smagorinsky_matches(cells, dirichlet):
     """sgs_dynamic_Smagorinsky (subgrid.f90:376-538) on a stretched grid.
     (36, 22, 18): x and z tiles and the y row chunks of the kernels are
-    ragged.  The plane sums are tree-ordered on the GPU: tolerance 1e-11."""
+    ragged.  (144, 20, 72): three 64-column x tiles, the last 16 columns
+    wide, so halos and ghost copies cross tile edges.  (64, 96, 384): long
+    marches, 22 chunks of 18 z planes in the strain kernel and 6 chunks of
+    18 rows in the LM kernel, the last of each ragged.  The plane sums are
+    tree-ordered on the GPU: tolerance 1e-11."""
     cfg = small_cfg(les_model=2, cells=cells, dirichlet_nu_t=dirichlet)
     g, o = pair(cfg)
     U, V, W = cases.initial_field(cfg)
