@@ -261,7 +261,7 @@ int Ctx<S>::init() {
             if (A(&st[b][c], n3[c])) return -1;
     for (int c = 0; c < 3; c++)
         if (A(&G[c], n3[c])) return -1;
-    if (thomas_uses_scratch((int)sizeof(Cx<S>), d.mh * d.nzp) && A(&cscr, thomas_scratch_elems(d))) return -1;
+    if (thomas_uses_scratch(d.mh * d.nzp) && A(&cscr, thomas_scratch_elems(d))) return -1;
     if (A(&NT, nC) || A(&spec, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
         A(&part, npart) || A(&dtscr, dt_scratch_doubles(d)) || A(&sc, 1) ||
         (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, stats_out_doubles(d)) || A(&stats_acc, 7 * d.ny + 8))

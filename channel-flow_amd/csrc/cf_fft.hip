@@ -726,17 +726,13 @@ static int env_int(const char *name, int dflt) {
 static bool fft_static_sizes() { return env_int("CF_FFT_STATIC", 1) != 0; }
 
 // threads per workgroup of the x transforms (env CF_FFT_XWG: 256 or 512)
-static int fft_x_wg(int esize) {
-    const int v = env_int("CF_FFT_XWG", 256);
-    (void)esize;
-    return v == 512 ? 512 : 256;
-}
+static int fft_x_wg() { return env_int("CF_FFT_XWG", 256) == 512 ? 512 : 256; }
 
 // host twin of fft_line for a complex element of esize bytes
 static int line_slots(int n, int esize) { return esize == 16 ? fft_line<double>(n) : fft_line<float>(n); }
 
 int fft_rows_per_wg(int M, int elems_per_row, int esize) {
-    int rb = 8 * fft_x_wg(esize) / M;
+    int rb = 8 * fft_x_wg() / M;
     if (rb > 16) rb = 16;
     // tuning knob: cap on rows per workgroup (more workgroups per CU)
     // (8 rows = runs of 8 consecutive kz in the spectrum, 128 bytes in FP64)
@@ -780,7 +776,7 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>);
     const dim3 grid((nrows + rb - 1) / rb);
-    if (fft_x_wg((int)sizeof(Cx<T>)) == 512) {
+    if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_div_fftx<S, T, 512, 0>), grid, dim3(512), shm, st, d, m, U, V, W, spec, pM, twM, twN,
                            rb, ysrc);
         return;
@@ -823,16 +819,13 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
 
 // lane form when there are enough modes to fill the GPU one lane each (both
 // precisions); the partitioned form below that
-bool thomas_uses_scratch(int esize, int nmodes) {
-    (void)esize;
-    return env_int("CF_TH_LANE", nmodes >= 65536 ? 1 : 0) != 0;
-}
+bool thomas_uses_scratch(int nmodes) { return env_int("CF_TH_LANE", nmodes >= 65536 ? 1 : 0) != 0; }
 
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st) {
     const int nmodes = d.mh * d.nzp;
-    if (cscr && thomas_uses_scratch((int)sizeof(Cx<T>), nmodes)) {
+    if (cscr && thomas_uses_scratch(nmodes)) {
         hipLaunchKernelGGL((k_thomas_lane<T>), dim3((2 * nmodes + 255) / 256), dim3(256), 3 * d.nyp * sizeof(double),
                            st, d, spec, cscr, kxx, kzz, dd, dl, du);
         return;
@@ -850,7 +843,7 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) ncu = v;
     }
-    const int grid = std::min(d.mh * nkzb, ncu * env_int("CF_TH_WGS_PER_CU", 2));
+    const int grid = std::min(d.mh * nkzb, ncu * std::max(1, env_int("CF_TH_WGS_PER_CU", 2)));
     if (ms == 4)
         hipLaunchKernelGGL((k_thomas<T, 4>), dim3(grid), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl, du, KZ,
                            P);
@@ -867,7 +860,7 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>);
     const dim3 grid((nrows + rb - 1) / rb);
-    if (fft_x_wg((int)sizeof(Cx<T>)) == 512) {
+    if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512, 0>), grid, dim3(512), shm, st, d, m, spec, phi, U, pM, twM,
                            twN, rb, norm);
         return;

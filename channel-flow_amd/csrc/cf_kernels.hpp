@@ -94,9 +94,9 @@ bool thomas_supported(const Dims &d);
 template <typename T>
 void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ, int sign,
                  hipStream_t st);
-// the y solve form for a spectrum element of esize bytes needs an FP64
-// scratch of one double per spectrum element
-bool thomas_uses_scratch(int esize, int nmodes);
+// whether the y solve of nmodes (kx, kz) systems takes the lane form, which
+// needs the FP64 checkpoint scratch of thomas_scratch_elems
+bool thomas_uses_scratch(int nmodes);
 size_t thomas_scratch_elems(const Dims &d);  // FP64 checkpoints of the lane-form y solve
 
 template <typename T>
