@@ -63,7 +63,7 @@ def test_fortran_driver_matches_python_host(tmp_path):
     U, V, W = cases.initial_field(cfg)
     g.initialise(U, V, W)
     g.run(6, graph=True)
-    # the driver builds its own tanh grid (flang's libm vs numpy: 1 ulp)
+    # the Fortran main program builds its own tanh grid (flang's libm vs numpy: 1 ulp)
     np.testing.assert_allclose(snap["y"], g.y, rtol=0, atol=4e-16)
     for name, a in zip("UVW", g.get_state()):
         assert np.max(np.abs(snap[name] - a)) <= 1e-12 * np.max(np.abs(a))
