@@ -573,37 +573,41 @@ struct OpAdd { __device__ double operator()(double a, double b) const { return a
 struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
 
+// workgroup tile of the dt reduction below: 64 x 8 (x, y) by 16 z planes
+constexpr int MF_TX = 64, MF_TY = 8, MF_KC = 16;
+
 // mass_flow.f90:116-123: partial sums of the U trapezoid over k = 2..nzg-1
 // (the unique planes plus the duplicate plane nzg-1, which the reference
-// includes), one deterministic partial per workgroup tile (64 x 8 x 16).
-constexpr int MF_TX = 64, MF_TY = 8, MF_KC = 16;
-// one thread per (i, j) column segment of MF_KC planes (i = 2..nx-1,
-// j = 3..nyg-1, k = 2..nzg-1), loads of the segment issued together; one
-// deterministic partial per 256-thread block
+// includes).
+// one thread per (i, k) column (i = 2..nx-1, k = 2..nzg-1) marching the rows
+// j = 2..nyg-1, each U(i, j, k) loaded once and weighted by its share of the
+// trapezoid: 0.5 (dyg(j) + dyg(j+1)), one-sided on the first and last row
+// (0.5 dyg(3), 0.5 dyg(nyg-1)); one deterministic partial per 256-thread
+// block.  Rows are unrolled by MF_UNROLL so that many loads are in flight.
+constexpr int MF_UNROLL = 8;
 template <typename S>
 __global__ void __launch_bounds__(256)
 k_massflow_u_partial(Dims d, Metrics m, const S *__restrict__ U, const double *__restrict__ ubar,
                      double *__restrict__ part) {
     __shared__ double sh[8];
-    const int ni = d.nx - 2, nj = d.nyg - 3, nk = d.nzg - 2;
-    const int nch = (nk + MF_KC - 1) / MF_KC;
-    const int64_t n = (int64_t)ni * nj * nch;
+    const int ni = d.nx - 2, nk = d.nzg - 2, nyg = d.nyg;
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double acc = 0.0;
-    if (g < n) {
-        const int i = 2 + (int)(g % ni);
-        const int j = 3 + (int)((g / ni) % nj);
-        const int kb = 2 + (int)(g / ((int64_t)ni * nj)) * MF_KC;
-        const double wj = 0.5 * m.dyg[j];
-        double v[MF_KC];
+    if (g < (int64_t)ni * nk) {
+        const int i = 2 + (int)(g % ni), k = 2 + (int)(g / ni);
+        const S *col = U + i3(d.nx, nyg, i, 1, k);  // row j at col[j - 1] * nx
+        auto wt = [&](int j) {
+            return 0.5 * ((j > 2 ? m.dyg[j] : 0.0) + (j < nyg - 1 ? m.dyg[j + 1] : 0.0));
+        };
+        int j = 2;
+        for (; j + MF_UNROLL - 1 <= nyg - 1; j += MF_UNROLL) {
+            double v[MF_UNROLL];
 #pragma unroll
-        for (int q = 0; q < MF_KC; q++) {
-            const int k = kb + q;
-            v[q] = k <= d.nzg - 1 ? (double)U[i3(d.nx, d.nyg, i, j, k)] + (double)U[i3(d.nx, d.nyg, i, j - 1, k)]
-                                  : 0.0;
+            for (int q = 0; q < MF_UNROLL; q++) v[q] = (double)col[(int64_t)(j + q - 1) * d.nx];
+#pragma unroll
+            for (int q = 0; q < MF_UNROLL; q++) acc += wt(j + q) * v[q];
         }
-#pragma unroll
-        for (int q = 0; q < MF_KC; q++) acc += v[q] * wj;
+        for (; j <= nyg - 1; j++) acc += wt(j) * (double)col[(int64_t)(j - 1) * d.nx];
     }
     acc = block_reduce(acc, OpAdd(), sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -1000,8 +1004,7 @@ void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, 
     hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, part, nb, sc, apply);
 }
 int massflow_parts(const Dims &d) {
-    const int nch = (d.nzg - 2 + MF_KC - 1) / MF_KC;
-    const int64_t n = (int64_t)(d.nx - 2) * (d.nyg - 3) * nch;
+    const int64_t n = (int64_t)(d.nx - 2) * (d.nzg - 2);
     return (int)((n + 255) / 256);
 }
 
