@@ -313,7 +313,7 @@ def main():
         copy_calibration(a.copy_calibration)
         return
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs a single case per GPU and has no "
+        print("bench.py measures one GPU: the solver runs a single case per GPU and has no "
               "multi-GPU path; use --gpus 1", file=sys.stderr)
         sys.exit(2)
     from channel_flow import cases
