@@ -63,6 +63,9 @@ SIGNATURES = {
     "cf_eqwm_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      _dp, _ip]),
+    "cf_eqwm_batch_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             _dp, _ip]),
     "cf_mlp_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(ctypes.c_void_p)]),
     "cf_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "cf_mlp_nparams": (ctypes.c_int, []),

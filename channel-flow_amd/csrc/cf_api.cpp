@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1309,6 +1310,92 @@ int cf_eqwm_batch(int n, const double *umag, const double *nu, const double *h, 
     hipFree(d);
     hipFree(di);
     if (e != hipSuccess) { set_error(std::string("cf_eqwm_batch: ") + hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// cf_eqwm_batch_noalloc: the staging of cf_eqwm_batch (the four input columns,
+// tau and iter of n points: 5 doubles + 1 int per point) in a file-scope
+// device array instead of per-call device allocations.  The array is part of
+// the code object: every GPU the library runs on holds one copy of
+// CF_EQWM_NOALLOC_MAX points (23 MiB) until the process ends.  Its device
+// address is looked up once per device (hipGetSymbolAddress, with that device
+// current) and only that address is handed to copies and kernels: the array's
+// name is a host-side stand-in in host code.  Calls on one device share the
+// array, so calls that overlap on one device race; nothing in it has to start
+// at zero (every input is copied in and every output written before it is
+// read back).
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t EQWM_NOALLOC_BYTES = (size_t)CF_EQWM_NOALLOC_MAX * (5 * sizeof(double) + sizeof(int));
+__device__ __align__(256) char scratch[EQWM_NOALLOC_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_ptr;  // device ordinal -> address of that device's scratch
+
+// device address of `scratch` on the current device (looked up on first use)
+char *scratch_on_current_device(int *dev_out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) { set_error("cf_eqwm_batch_noalloc: hipGetDevice failed"); return nullptr; }
+    *dev_out = dev;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    char *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error(std::string("cf_eqwm_batch_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+    return p;
+}
+}  // namespace
+
+extern "C" {
+
+int cf_eqwm_batch_noalloc(int n, const double *umag, const double *nu, const double *h, const double *tau_in,
+                          int ncv, double stretch, double tol, int max_iter, double *tau, int *iter) {
+    if (n < 0 || (n > 0 && (!umag || !nu || !h || !tau_in || !tau || !iter))) {
+        set_error("cf_eqwm_batch_noalloc: bad arguments");
+        return -1;
+    }
+    if (n > CF_EQWM_NOALLOC_MAX) {
+        set_error("cf_eqwm_batch_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                  std::to_string(CF_EQWM_NOALLOC_MAX) + " points (use cf_eqwm_batch)");
+        return -1;
+    }
+    if (ncv < 3 || ncv > cf::EQWM_MAXNCV) { set_error("cf_eqwm_batch_noalloc: ncv out of range [3,128]"); return -1; }
+    if (n == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("cf_eqwm_batch_noalloc: no HIP device");
+        return -1;
+    }
+    int dev = -1;
+    char *base = scratch_on_current_device(&dev);
+    if (!base) return -1;
+    double *d = (double *)base;  // umag | nu | h | tau_in | tau, n each
+    int *di = (int *)(d + 5 * (size_t)n);
+    const size_t b = (size_t)n * sizeof(double);
+    hipError_t e = hipMemcpy(d, umag, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, nu, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 2 * (size_t)n, h, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 3 * (size_t)n, tau_in, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        cf::EqwmParams p{ncv, max_iter, stretch, tol, 1.0};
+        cf::launch_eqwm_batch(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, d + 4 * (size_t)n, di, p, 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e == hipSuccess) e = hipMemcpy(tau, d + 4 * (size_t)n, b, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(iter, di, (size_t)n * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_error(std::string("cf_eqwm_batch_noalloc: ") + hipGetErrorString(e));
+        return -1;
+    }
     return 0;
 }
 

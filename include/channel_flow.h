@@ -119,6 +119,18 @@ int cf_eqwm_batch(int n, const double *umag, const double *nu, const double *h,
                   const double *tau_in, int ncv, double stretch, double tol, int max_iter,
                   double *tau, int *iter);
 
+/* cf_eqwm_batch without any device allocation or release.  No counterpart in
+ * the reference (which calls solve_u_eqwm point by point): it is for callers
+ * that cannot allocate device memory, such as serving loops and harnesses
+ * that forbid hipMalloc.  The staging of the n points lives in a fixed
+ * device array of CF_EQWM_NOALLOC_MAX points per GPU, held until the process
+ * ends; larger n is rejected with an error.  Runs on the current device.
+ * Calls that overlap on one GPU share that array and race. */
+#define CF_EQWM_NOALLOC_MAX 524288
+int cf_eqwm_batch_noalloc(int n, const double *umag, const double *nu, const double *h,
+                          const double *tau_in, int ncv, double stretch, double tol, int max_iter,
+                          double *tau, int *iter);
+
 /* ---- learned wall model ----
  * A 4 -> 64 -> 64 -> 64 -> 1 tanh MLP that maps the wall-model inputs
  * (u_par, h, nu, dP/dx) to the wall stress tau_w.  It stands in for the
