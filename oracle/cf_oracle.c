@@ -13,6 +13,11 @@
  * or drop them).
  *
  * Compile with -ffp-contract=off so that no multiply-add is fused.
+ *
+ * ORC_PLANT (0 in every normal build) selects one deliberate mistake at a
+ * PLANT(n, wrong, right) site.  tests/test_oracle_planted.py builds the
+ * oracle with -DORC_PLANT=n for each n and checks that the pin guarding that
+ * site fails, which shows the pins can tell a plausible mistake apart.
  */
 #include "cf_oracle.h"
 
@@ -25,6 +30,11 @@
 #endif
 
 typedef double complex cplx;
+
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(n, wrong, right) (ORC_PLANT == (n) ? (wrong) : (right))
 
 struct orc {
     int nx, ny, nz, nxg, nyg, nzg, nxp, nzp, nr;
@@ -374,8 +384,12 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     double c0 = 0.5 * (u(i - 1, j, k) + u(i, j, k));
                     double t1 = -(sq(c1) - sq(c0)) / (XG(i + 1) - XG(i));
                     /* 2) -duv/dy (:62-79) */
-                    double uv1 = (W0(j) * u(i, j, k) + W1(j) * u(i, j + 1, k)) * (0.5 * (v(i, j, k) + v(i + 1, j, k)));
-                    double uv0 = (W0(j - 1) * u(i, j - 1, k) + W1(j - 1) * u(i, j, k)) * (0.5 * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
+                    double uv1 = PLANT(4, W1(j) * u(i, j, k) + W0(j) * u(i, j + 1, k),
+                                       W0(j) * u(i, j, k) + W1(j) * u(i, j + 1, k)) *
+                                 (0.5 * (v(i, j, k) + v(i + 1, j, k)));
+                    double uv0 = PLANT(4, W1(j - 1) * u(i, j - 1, k) + W0(j - 1) * u(i, j, k),
+                                       W0(j - 1) * u(i, j - 1, k) + W1(j - 1) * u(i, j, k)) *
+                                 (0.5 * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
                     double t2 = -(uv1 - uv0) / (Yf(j) - Yf(j - 1));
                     /* 3) -duw/dz (:84-101) */
                     double uw1 = (0.5 * (u(i, j, k) + u(i, j, k + 1))) * (0.5 * (w(i, j, k) + w(i + 1, j, k)));
@@ -386,8 +400,9 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     double dy_3 = Yf(j) - Yf(j - 1);
                     double dx_1 = X(i) - X(i - 1), dx_2 = X(i + 1) - X(i), dx_3 = XG(i + 1) - XG(i);
                     double nu_x1 = nu + nt(i, j, k), nu_x2 = nu + nt(i + 1, j, k);
-                    double nu_y1 = nu + 0.5 * (W0(j - 1) * nt(i, j - 1, k) + W1(j - 1) * nt(i, j, k) +
-                                               W0(j - 1) * nt(i + 1, j - 1, k) + W1(j - 1) * nt(i + 1, j, k));
+                    const int jw = PLANT(1, j, j - 1);
+                    double nu_y1 = nu + 0.5 * (W0(jw) * nt(i, j - 1, k) + W1(jw) * nt(i, j, k) +
+                                               W0(jw) * nt(i + 1, j - 1, k) + W1(jw) * nt(i + 1, j, k));
                     double nu_y2 = nu + 0.5 * (W0(j) * nt(i, j, k) + W1(j) * nt(i, j + 1, k) +
                                                W0(j) * nt(i + 1, j, k) + W1(j) * nt(i + 1, j + 1, k));
                     double nu_z1 = nu + 0.25 * (nt(i, j, k) + nt(i, j, k - 1) + nt(i + 1, j, k) + nt(i + 1, j, k - 1));
@@ -405,8 +420,9 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     r = r + visc;
                     r = r + dPdx;
                     if (rot) { /* :169-177 */
-                        double a = 0.5 * (v(i, j - 1, k) + v(i, j, k));
-                        double b = 0.5 * (v(i + 1, j - 1, k) + v(i + 1, j, k));
+                        const int jc = PLANT(7, j + 1, j);
+                        double a = 0.5 * (v(i, jc - 1, k) + v(i, jc, k));
+                        double b = 0.5 * (v(i + 1, jc - 1, k) + v(i + 1, jc, k));
                         r = r + 2.0 * Om * (0.5 * (a + b));
                     }
                     rhs_u[IRU(i, j, k)] = r;
@@ -420,7 +436,7 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     /* 1) -dv^2/dy (:205-223) */
                     double c1 = 0.5 * (v(i, j, k) + v(i, j + 1, k));
                     double c0 = 0.5 * (v(i, j - 1, k) + v(i, j, k));
-                    double t1 = -(sq(c1) - sq(c0)) / (YG(j + 1) - YG(j));
+                    double t1 = -(sq(c1) - sq(c0)) / PLANT(8, Yf(j + 1) - Yf(j), YG(j + 1) - YG(j));
                     /* 2) -duv/dx (:228-245) */
                     double uv1 = (W0(j) * u(i, j, k) + W1(j) * u(i, j + 1, k)) * (0.5 * (v(i, j, k) + v(i + 1, j, k)));
                     double uv0 = (W0(j) * u(i - 1, j, k) + W1(j) * u(i - 1, j + 1, k)) * (0.5 * (v(i - 1, j, k) + v(i, j, k)));
@@ -431,13 +447,16 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     double t3 = -(vw1 - vw0) / (Z(k) - Z(k - 1));
                     /* 4) viscous (:276-325) */
                     double dz_1 = ZG(k) - ZG(k - 1), dz_2 = ZG(k + 1) - ZG(k), dz_3 = Z(k) - Z(k - 1);
-                    double dy_1 = Yf(j) - Yf(j - 1), dy_2 = Yf(j + 1) - Yf(j), dy_3 = YG(j + 1) - YG(j);
+                    double dy_1 = PLANT(3, Yf(j + 1) - Yf(j), Yf(j) - Yf(j - 1));
+                    double dy_2 = PLANT(3, Yf(j) - Yf(j - 1), Yf(j + 1) - Yf(j));
+                    double dy_3 = PLANT(2, Yf(j + 1) - Yf(j), YG(j + 1) - YG(j));
                     double dx_1 = XG(i) - XG(i - 1), dx_2 = XG(i + 1) - XG(i), dx_3 = X(i) - X(i - 1);
-                    double nu_x1 = nu + 0.5 * (W0(j) * nt(i - 1, j, k) + W1(j) * nt(i - 1, j + 1, k) +
-                                               W0(j) * nt(i, j, k) + W1(j) * nt(i, j + 1, k));
+                    const int jx = PLANT(14, j - 1, j);
+                    double nu_x1 = nu + 0.5 * (W0(jx) * nt(i - 1, j, k) + W1(jx) * nt(i - 1, j + 1, k) +
+                                               W0(jx) * nt(i, j, k) + W1(jx) * nt(i, j + 1, k));
                     double nu_x2 = nu + 0.5 * (W0(j) * nt(i, j, k) + W1(j) * nt(i, j + 1, k) +
                                                W0(j) * nt(i + 1, j, k) + W1(j) * nt(i + 1, j + 1, k));
-                    double nu_y1 = nu + nt(i, j, k), nu_y2 = nu + nt(i, j + 1, k);
+                    double nu_y1 = nu + PLANT(9, nt(i, j - 1, k), nt(i, j, k)), nu_y2 = nu + nt(i, j + 1, k);
                     double nu_z1 = nu + 0.5 * (W0(j) * nt(i, j, k - 1) + W1(j) * nt(i, j + 1, k - 1) +
                                                W0(j) * nt(i, j, k) + W1(j) * nt(i, j + 1, k));
                     double nu_z2 = nu + 0.5 * (W0(j) * nt(i, j, k) + W1(j) * nt(i, j + 1, k) +
@@ -454,7 +473,7 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     if (rot) { /* :334-342 */
                         double a = 0.5 * (u(i - 1, j, k) + u(i, j, k));
                         double b = 0.5 * (u(i - 1, j + 1, k) + u(i, j + 1, k));
-                        r = r - 2.0 * Om * (W0(j) * a + W1(j) * b);
+                        r = PLANT(6, r + 2.0 * Om * (W0(j) * a + W1(j) * b), r - 2.0 * Om * (W0(j) * a + W1(j) * b));
                     }
                     rhs_v[IRV(i, j, k)] = r;
                 }
@@ -474,7 +493,8 @@ void orc_compute_rhs(orc_t *o, const double *U, const double *V, const double *W
                     double t2 = -(uw1 - uw0) / (X(i) - X(i - 1));
                     /* 3) -dvw/dy (:411-428) */
                     double vw1 = (0.5 * (v(i, j, k) + v(i, j, k + 1))) * (W0(j) * w(i, j, k) + W1(j) * w(i, j + 1, k));
-                    double vw0 = (0.5 * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (W0(j - 1) * w(i, j - 1, k) + W1(j - 1) * w(i, j, k));
+                    const int jv = PLANT(5, j, j - 1);
+                    double vw0 = (0.5 * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (W0(jv) * w(i, j - 1, k) + W1(jv) * w(i, j, k));
                     double t3 = -(vw1 - vw0) / (Yf(j) - Yf(j - 1));
                     /* 4) viscous (:435-491) */
                     double dz_1 = Z(k) - Z(k - 1), dz_2 = Z(k + 1) - Z(k), dz_3 = ZG(k + 1) - ZG(k);
@@ -1406,7 +1426,7 @@ void orc_statistics(const orc_t *o, double *out) {
     for (int jj = 1; jj <= ny; jj++) {
         double su = 0, su2 = 0, sw = 0, sw2 = 0, sv = 0, sv2 = 0, suv = 0;
         /* term_1 = interpolate_y(U,2): U at y faces; sum over (1:nx-2, 1:nzg-2) */
-        for (int k = 1; k <= nzg - 2; k++)
+        for (int k = PLANT(10, 2, 1); k <= nzg - 2; k++)
             for (int i = 1; i <= nx - 2; i++) {
                 double t = W0(jj) * u(i, jj, k) + W1(jj) * u(i, jj + 1, k);
                 su += t; su2 += sq(t);
@@ -1414,14 +1434,14 @@ void orc_statistics(const orc_t *o, double *out) {
         for (int k = 1; k <= nz - 2; k++)
             for (int i = 1; i <= nxg - 2; i++) {
                 double t = W0(jj) * w(i, jj, k) + W1(jj) * w(i, jj + 1, k);
-                sw += t; sw2 += sq(t);
+                sw += t; sw2 += PLANT(11, t, sq(t));
             }
         for (int k = 2; k <= nzg - 1; k++)
-            for (int i = 2; i <= nxg - 1; i++) { sv += v(i, jj, k); sv2 += sq(v(i, jj, k)); }
+            for (int i = 2; i <= PLANT(15, nxg - 2, nxg - 1); i++) { sv += v(i, jj, k); sv2 += sq(v(i, jj, k)); }
         /* UVmean: V(2:nxg-1) * term(1:nxg-2), term = interpolate_x(term_1) */
         for (int k = 2; k <= nzg - 1; k++)
             for (int i = 2; i <= nxg - 1; i++) {
-                int ii = i - 1;
+                int ii = PLANT(13, i, i - 1);
                 double ta = W0(jj) * u(ii, jj, k) + W1(jj) * u(ii, jj + 1, k);
                 double tb = ii + 1 <= nx ? W0(jj) * u(ii + 1, jj, k) + W1(jj) * u(ii + 1, jj + 1, k) : 0.0;
                 suv += v(i, jj, k) * (0.5 * (ta + tb));
@@ -1454,7 +1474,7 @@ void orc_statistics(const orc_t *o, double *out) {
     nb = nb / (double)((nxg - 2) * (nzg - 2)) / 2.0;
     ntp = ntp / (double)((nxg - 2) * (nzg - 2)) / 2.0;
     double Ly = Yf(ny) - Yf(1);
-    double utau = sqrt((UV[ny - 1] - UV[0] - (o->p.nu + ntp) * dUt + (o->p.nu + nb) * dUb) / Ly);
+    double utau = sqrt((PLANT(12, UV[0] - UV[ny - 1], UV[ny - 1] - UV[0]) - (o->p.nu + ntp) * dUt + (o->p.nu + nb) * dUb) / Ly);
     double *tail = out + 7 * ny;
     tail[0] = utau;
     tail[1] = utau * (Yf(ny) - Yf(1)) / 2.0 / o->p.nu;

@@ -21,15 +21,17 @@ class OrcParams(ctypes.Structure):
                  "eqwm_ncv", "eqwm_max_iter", "nthreads"]]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
+def lib(path=None):
+    """The oracle library (oracle/liboracle.so), or a build of it at `path`
+    (the planted-mistake variants of tests/test_oracle_planted.py)."""
+    path = path or ORACLE_SO
+    if path not in _libs:
+        if not os.path.exists(path):
             raise RuntimeError("oracle not built: make -C oracle")
-        L = ctypes.CDLL(ORACLE_SO)
+        L = ctypes.CDLL(path)
         v = ctypes.c_void_p
         sig = {
             "orc_create": (v, [ctypes.c_int] * 3 + [_dp] * 3 + [ctypes.POINTER(OrcParams)]),
@@ -67,8 +69,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def P(a):
@@ -82,8 +84,8 @@ def _f(shape):
 class Oracle:
     """Same surface as channel_flow.solver.ChannelFlow, on the CPU oracle."""
 
-    def __init__(self, cfg, x=None, y=None, z=None, nthreads=0, jref=None):
-        self.L = lib()
+    def __init__(self, cfg, x=None, y=None, z=None, nthreads=0, jref=None, path=None):
+        self.L = lib(path)
         if x is None:
             x, y, z = cfg.grid()
         self.x, self.y, self.z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
