@@ -28,6 +28,7 @@ class CfParams(ctypes.Structure):
         ("eqwm_stretch", ctypes.c_double), ("eqwm_tol", ctypes.c_double),
         ("tau_mean_eqwm", ctypes.c_double),
         ("wall_surrogate", ctypes.c_int), ("device", ctypes.c_int),
+        ("u_centerline", ctypes.c_double),
     ]
 
 
@@ -58,6 +59,8 @@ SIGNATURES = {
     "cf_compute_eddy_viscosity": (ctypes.c_int, [ctypes.c_void_p]),
     "cf_compute_wall_model": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
     "cf_mean_mass_flow_u": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_mean_mass_flow_v": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "cf_mean_centerline_u": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_check_divergence": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "cf_statistics": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "cf_eqwm_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int,

@@ -23,8 +23,9 @@ class CaseConfig:
     dPdx: float = 0.0
     dPdz: float = 0.0
     omega_z: float = 0.0
-    x_mass_cte: int = 1
-    y_mass_cte: int = 0
+    x_mass_cte: int = 1                  # 1 constant bulk flow, 2 constant centreline velocity
+    y_mass_cte: int = 0                  # 1 constant bulk flow in y
+    u_centerline: float = 0.0            # x_mass_cte = 2 target (initialization.f90:453-455)
     les_model: int = 0
     iwall_model: int = 0
     dirichlet_nu_t: int = 0

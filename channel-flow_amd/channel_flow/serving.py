@@ -33,6 +33,7 @@ def case_params(cfg: CaseConfig, device=0):
     p.tau_mean_eqwm = cfg.dPdx
     p.wall_surrogate = cfg.wall_surrogate
     p.device = device
+    p.u_centerline = cfg.u_centerline
     return p, x, y, z
 
 

@@ -13,7 +13,7 @@ C-ABI (include/channel_flow.h):
     compute_wall_model            wallmodel.f90:24
     compute_mean_mass_flow_U      mass_flow.f90:25
     check_divergence              projection.f90:234
-    compute_statistics            statistics.f90:22 (plane means)
+    compute_statistics            statistics.f90:22
 
 Arrays are numpy Fortran-ordered with the reference extents (1-based index
 i in the reference is index i-1 here).
@@ -55,6 +55,7 @@ class ChannelFlow:
         p.tau_mean_eqwm = cfg.dPdx
         p.wall_surrogate = cfg.wall_surrogate
         p.device = device
+        p.u_centerline = cfg.u_centerline
         self.params = p
         self.jref_eqwm = p.jref_eqwm
         h = ctypes.c_void_p()
@@ -221,6 +222,16 @@ class ChannelFlow:
         _lib.check(self.lib.cf_mean_mass_flow_u(self.h, ctypes.byref(q)), "compute_mean_mass_flow_U")
         return q.value
 
+    def compute_mean_mass_flow_V(self):
+        q = ctypes.c_double()
+        _lib.check(self.lib.cf_mean_mass_flow_v(self.h, ctypes.byref(q)), "compute_mean_mass_flow_V")
+        return q.value
+
+    def compute_mean_centerline_U(self):
+        q = ctypes.c_double()
+        _lib.check(self.lib.cf_mean_centerline_u(self.h, ctypes.byref(q)), "compute_mean_centerline_U")
+        return q.value
+
     def init_mass_flow(self):
         _lib.check(self.lib.cf_init_mass_flow(self.h), "init_mass_flow")
 
@@ -230,10 +241,15 @@ class ChannelFlow:
         return a.value, b.value
 
     def compute_statistics(self):
-        out = np.zeros(7 * self.ny)
+        """Plane means, u_tau, Re_tau and the bulk flows; raises when the
+        state holds a NaN (statistics.f90:165-167)."""
+        out = np.zeros(7 * self.ny + 4)
         _lib.check(self.lib.cf_statistics(self.h, _lib.dptr(out)), "compute_statistics")
         names = ["Umean", "Vmean", "Wmean", "U2mean", "V2mean", "W2mean", "UVmean"]
-        return {n: out[i * self.ny:(i + 1) * self.ny].copy() for i, n in enumerate(names)}
+        d = {n: out[i * self.ny:(i + 1) * self.ny].copy() for i, n in enumerate(names)}
+        t = out[7 * self.ny:]
+        d.update(utau=float(t[0]), Retau=float(t[1]), Qflow_x=float(t[2]), Qflow_y=float(t[3]))
+        return d
 
     def stats_accumulate(self):
         """Add the current plane statistics to the device window (async)."""

@@ -198,6 +198,9 @@ std::string Host::validate() const {
         if (p.eqwm_ncv < 3 || p.eqwm_ncv > EQWM_MAXNCV) return "eqwm_ncv out of range";
     }
     if (p.nu <= 0) return "nu must be positive";
+    if (p.x_mass_cte < 0 || p.x_mass_cte > 2)
+        return "x_mass_cte must be 0, 1 (constant bulk flow) or 2 (constant centreline velocity)";
+    if (p.y_mass_cte != 0 && p.y_mass_cte != 1) return "y_mass_cte must be 0 or 1 (constant bulk flow in y)";
     return "";
 }
 
@@ -356,6 +359,7 @@ int Ctx<S>::init() {
     s0.dPdz = h.prm.dPdz;
     s0.norm_u = h.norm_u;
     s0.norm_v = h.norm_v;
+    s0.ucl = h.prm.u_centerline;
     HCK(hipMemcpy(sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice));
     HCK(hipStreamSynchronize(stream));
     return 0;
@@ -515,10 +519,12 @@ void Ctx<S>::step_launch() {
         ghosts(Fn, 7);
         cur = 1 - cur;
     }
+    // mass-flow corrections after the third substep (time_integration.f90:152-171)
     S *const *F = st[cur];
-    if (h.prm.x_mass_cte == 1) {
+    if (h.prm.x_mass_cte == 1 || h.prm.x_mass_cte == 2) {
         tic("massflow");
-        launch_massflow<S>(d, met, F[0], part, sc, 1, stream);
+        if (h.prm.x_mass_cte == 1) launch_massflow<S>(d, met, F[0], part, sc, 1, stream);
+        else launch_centerline_u<S>(d, F[0], ubar, part, sc, 1, stream);
         if (ubar) {
             // deviation storage: the uniform correction shifts the profile
             launch_ubar_shift(sc, ubar, d.nyg, h.prm.iwall_model == 3 ? 1 : 0, stream);
@@ -528,6 +534,13 @@ void Ctx<S>::step_launch() {
             toc();
             ghosts(F, 1);
         }
+    }
+    if (h.prm.y_mass_cte == 1) {
+        tic("massflow_v");
+        launch_massflow_v<S>(d, m_y, F[1], part, sc, 1, stream);
+        launch_add_corr_v<S>(d, F[1], sc, stream);
+        toc();
+        ghosts(F, 2);
     }
     tic("advance_time");
     launch_advance_time(sc, stream);
@@ -957,6 +970,28 @@ int Ctx<S>::mean_mass_flow_u(double *q) {
 }
 
 template <typename S>
+int Ctx<S>::mean_mass_flow_v(double *q) {
+    HCK(hipSetDevice(dev));
+    launch_massflow_v<S>(h.d, m_y, st[cur][1], part, sc, 0, stream);
+    HCK(hipGetLastError());
+    Scalars s;
+    if (get_raw_scalars(&s)) return -1;
+    *q = s.pad[1];
+    return 0;
+}
+
+template <typename S>
+int Ctx<S>::mean_centerline_u(double *q) {
+    HCK(hipSetDevice(dev));
+    launch_centerline_u<S>(h.d, st[cur][0], ubar, part, sc, 0, stream);
+    HCK(hipGetLastError());
+    Scalars s;
+    if (get_raw_scalars(&s)) return -1;
+    *q = s.pad[0];
+    return 0;
+}
+
+template <typename S>
 int Ctx<S>::init_mass_flow() {
     double q;
     if (mean_mass_flow_u(&q)) return -1;
@@ -983,20 +1018,31 @@ int Ctx<S>::check_divergence(double *ms, double *ma) {
     return 0;
 }
 
+// compute_statistics (statistics.f90:22-171): the plane means, u_tau, Re_tau
+// and the bulk flows; a NaN anywhere in U, V or W is an error, as the
+// reference's Stop 'Error: NaNs!' (:165-167)
 template <typename S>
 int Ctx<S>::statistics(double *out) {
     HCK(hipSetDevice(dev));
     const Dims &d = h.d;
-    launch_stats<S>(d, met, st[cur][0], st[cur][1], st[cur][2], ubar, stats_out, stream);
+    S *const *F = st[cur];
+    launch_stats<S>(d, met, F[0], F[1], F[2], ubar, stats_out, stream);
+    launch_stats_tail<S>(d, met, m_y, m_yg, F[0], F[1], F[2], NT, ubar, stats_out, part, sc, h.prm.nu, stream);
     HCK(hipGetLastError());
     HCK(hipStreamSynchronize(stream));
-    HCK(hipMemcpy(out, stats_out, 7 * d.ny * sizeof(double), hipMemcpyDeviceToHost));
+    HCK(hipMemcpy(out, stats_out, (7 * d.ny + 4) * sizeof(double), hipMemcpyDeviceToHost));
+    double nan_flag = 0.0;
+    HCK(hipMemcpy(&nan_flag, stats_out + 7 * d.ny + 4, sizeof(double), hipMemcpyDeviceToHost));
     // statistics.f90:107-115 normalisations
     const double cu = (double)((d.nx - 2) * (d.nzg - 2)), cv = (double)((d.nxg - 2) * (d.nzg - 2)),
                  cw = (double)((d.nxg - 2) * (d.nz - 2));
     const double den[7] = {cu, cv, cw, cu, cv, cw, cv};
     for (int c = 0; c < 7; c++)
         for (int j = 0; j < d.ny; j++) out[c * d.ny + j] /= den[c];
+    if (nan_flag != 0.0) {
+        set_error("compute_statistics: Error: NaNs!");
+        return -1;
+    }
     return 0;
 }
 
@@ -1277,6 +1323,8 @@ int cf_compute_wall_model(cf_ctx *c, int *n_nan, int *n_iter) {
     return c->impl->compute_wall_model(n_nan, n_iter);
 }
 int cf_mean_mass_flow_u(cf_ctx *c, double *q) { CTX_CHECK(c); return c->impl->mean_mass_flow_u(q); }
+int cf_mean_mass_flow_v(cf_ctx *c, double *q) { CTX_CHECK(c); return c->impl->mean_mass_flow_v(q); }
+int cf_mean_centerline_u(cf_ctx *c, double *q) { CTX_CHECK(c); return c->impl->mean_centerline_u(q); }
 int cf_check_divergence(cf_ctx *c, double *ms, double *ma) { CTX_CHECK(c); return c->impl->check_divergence(ms, ma); }
 int cf_statistics(cf_ctx *c, double *out) { CTX_CHECK(c); return c->impl->statistics(out); }
 

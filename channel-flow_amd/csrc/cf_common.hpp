@@ -47,7 +47,9 @@ struct Scalars {
     double norm_v;
     double dt_conv[3];
     double qbar_u;    // bulk-flow contribution of the stored profile ubar (mixed path)
-    double pad[3];
+    double corr_v;    // last y mass-flow correction (V increment, mass_flow.f90:174)
+    double ucl;       // centreline target Ucenterline_x (x_mass_cte = 2)
+    double pad[3];    // [0] last bulk (or centreline) mean of U, [1] last bulk mean of V
     int n_nan_eqwm;
     int n_iter_eqwm;
     int n_fallback_mlp;

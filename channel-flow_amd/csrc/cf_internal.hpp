@@ -81,6 +81,8 @@ struct CtxBase {
     virtual int compute_eddy_viscosity() = 0;
     virtual int compute_wall_model(int *n_nan, int *n_iter) = 0;
     virtual int mean_mass_flow_u(double *q) = 0;
+    virtual int mean_mass_flow_v(double *q) = 0;
+    virtual int mean_centerline_u(double *q) = 0;
     virtual int check_divergence(double *ms, double *ma) = 0;
     virtual int statistics(double *out) = 0;
     virtual int stats_accumulate() = 0;
@@ -190,6 +192,8 @@ struct Ctx : CtxBase {
     int compute_eddy_viscosity() override;
     int compute_wall_model(int *n_nan, int *n_iter) override;
     int mean_mass_flow_u(double *q) override;
+    int mean_mass_flow_v(double *q) override;
+    int mean_centerline_u(double *q) override;
     int check_divergence(double *ms, double *ma) override;
     int statistics(double *out) override;
     int stats_accumulate() override;

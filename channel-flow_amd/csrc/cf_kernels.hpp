@@ -61,6 +61,17 @@ void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, 
                      int apply, hipStream_t st);
 template <typename S>
 void launch_add_corr_u(const Dims &d, S *U, const Scalars *sc, hipStream_t st);
+// the centreline mean of U (pad[0]); apply (x_mass_cte = 2): also the
+// correction (corr) and dPdx
+template <typename S>
+void launch_centerline_u(const Dims &d, const S *U, const double *ubar, double *part, Scalars *sc, int apply,
+                         hipStream_t st);
+// bulk V (pad[1]); apply: the y correction corr_v and dPdy
+template <typename S>
+void launch_massflow_v(const Dims &d, const double *y, const S *V, double *part, Scalars *sc, int apply,
+                       hipStream_t st);
+template <typename S>
+void launch_add_corr_v(const Dims &d, S *V, const Scalars *sc, hipStream_t st);
 template <typename S>
 void launch_dt(const Dims &d, const double *dxc, const double *dyc, const double *dzc,
                const S *U, const S *V, const S *W, const double *ubar, double *scratch,
@@ -80,6 +91,13 @@ void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const
 
 void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st);
 size_t stats_out_doubles(const Dims &d);  // the `out` buffer of launch_stats (results + block partials)
+// after launch_stats: u_tau, Re_tau, bulk U and V into out[7 ny .. 7 ny + 3] and
+// the NaN flag of U, V, W into out[7 ny + 4] (y, yg 1-based; `part` is the
+// mass-flow partial buffer)
+template <typename S>
+void launch_stats_tail(const Dims &d, const Metrics &m, const double *y, const double *yg, const S *U, const S *V,
+                       const S *W, const S *NT, const double *ubar, double *out, double *part, Scalars *sc, double nu,
+                       hipStream_t st);
 
 // ---- Poisson chain (cf_fft.hip) ----
 int fft_rows_per_wg(int M, int elems_per_row, int esize);

@@ -3,6 +3,7 @@
 // reductions, divergence check, constant-coefficient Smagorinsky and plane
 // statistics.  All are HBM-streaming kernels: one thread per unique cell,
 // x fastest, tiles remapped per XCD.
+#include <algorithm>
 #include <cstdlib>
 
 #include "cf_kernels.hpp"
@@ -646,6 +647,102 @@ k_add_corr_u(Dims d, S *__restrict__ U, const Scalars *__restrict__ sc) {
     U[q] = (S)((double)U[q] + sc->corr);
 }
 
+// compute_dPx_for_constant_Ucenterline (mass_flow.f90:192-209): the sum of
+// U(1:nx-2, jc, 1:nzg-2) with jc = ny/2 + 1 (integer division); the k range
+// holds the periodic image k = 1 of plane nzg-2 besides that plane, as the
+// reference's does.  One thread per (i, k), one partial per block.
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_centerline_partial(Dims d, const S *__restrict__ U, int jc, double *__restrict__ part) {
+    __shared__ double sh[8];
+    const int ni = d.nx - 2, nk = d.nzg - 2;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (g < (int64_t)ni * nk) {
+        const int i = 1 + (int)(g % ni), k = 1 + (int)(g / ni);
+        acc = (double)U[i3(d.nx, d.nyg, i, jc, k)];
+    }
+    acc = block_reduce(acc, OpAdd(), sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// compute_mean_mass_flow_V and compute_dPy_for_constant_mass_flow
+// (mass_flow.f90:61-81, 147-174): one thread per (i, k) column, i = 2..nxg-1,
+// k = 2..nzg-1 (the duplicate column and plane included, as in the
+// reference), marching the faces j = 1..ny; each V loaded once with its
+// trapezoid weight 0.5 ((y(j)-y(j-1)) + (y(j+1)-y(j))), one-sided at the
+// walls.  y is 1-based.
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_massflow_v_partial(Dims d, const double *__restrict__ y, const S *__restrict__ V, double *__restrict__ part) {
+    __shared__ double sh[8];
+    const int ni = d.nxg - 2, nk = d.nzg - 2, ny = d.ny;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (g < (int64_t)ni * nk) {
+        const int i = 2 + (int)(g % ni), k = 2 + (int)(g / ni);
+        const S *col = V + i3(d.nxg, ny, i, 1, k);
+#pragma unroll 4
+        for (int j = 1; j <= ny; j++) {
+            const double w = 0.5 * ((j > 1 ? y[j] - y[j - 1] : 0.0) + (j < ny ? y[j + 1] - y[j] : 0.0));
+            acc += w * (double)col[(int64_t)(j - 1) * d.nxg];
+        }
+    }
+    acc = block_reduce(acc, OpAdd(), sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// final step of the centreline and bulk-V reductions (partials in block
+// order).  MF_CENTERLINE_MEAN: mean of the centreline plane (count values;
+// the mixed path adds the stored profile ubar(jc) to each); MF_CENTERLINE
+// also the correction Ucenterline_x - mean, dPdx = corr/dt
+// (time_integration.f90:158-162).
+// MF_V / MF_V_APPLY: bulk V over norm_v, and for MF_V_APPLY the correction
+// Qflow_y_0 - bulk, dPdy = corr/dt (time_integration.f90:166-169).
+enum { MF_CENTERLINE_MEAN = 0, MF_CENTERLINE = 1, MF_V = 2, MF_V_APPLY = 3 };
+__global__ void k_flow_final(const double *__restrict__ part, int nparts, Scalars *sc,
+                             const double *__restrict__ ubar, int jc, double count, int mode) {
+    __shared__ double sh[8];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += part[q];
+    acc = block_reduce(acc, OpAdd(), sh);
+    if (threadIdx.x == 0) {
+        if (mode == MF_CENTERLINE || mode == MF_CENTERLINE_MEAN) {
+            const double mean = (acc + (ubar ? count * ubar[jc] : 0.0)) / count;
+            sc->pad[0] = mean;
+            if (mode == MF_CENTERLINE) {
+                const double corr = sc->ucl - mean;
+                sc->corr = corr;
+                sc->dPdx = corr / sc->dt;
+            }
+        } else {
+            const double mean = acc / sc->norm_v;
+            sc->pad[1] = mean;
+            if (mode == MF_V_APPLY) {
+                const double corr = sc->qflow_y0 - mean;
+                sc->corr_v = corr;
+                sc->dPdy = corr / sc->dt;
+            }
+        }
+    }
+}
+
+// V(2:nxg-1, 2:ny-1, 2:nzg-1) += corr_v (time_integration.f90:168); the wall
+// rows keep their values
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_add_corr_v(Dims d, S *__restrict__ V, const Scalars *__restrict__ sc) {
+    const int ni = d.nxg - 2, nj = d.ny - 2;
+    const int64_t n = (int64_t)ni * nj * (d.nzg - 2);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int i = 2 + (int)(g % ni);
+    const int j = 2 + (int)((g / ni) % nj);
+    const int k = 2 + (int)(g / ((int64_t)ni * nj));
+    const int64_t q = i3(d.nxg, d.ny, i, j, k);
+    V[q] = (S)((double)V[q] + sc->corr_v);
+}
+
 // time_integration.f90:188-199.  max over cells of spacing/|u| equals, per
 // line of constant spacing, spacing / min|u| (correctly rounded division is
 // monotone), so the kernel only takes minima: of |U| per i, |V| per j, |W|
@@ -896,6 +993,86 @@ k_stats_planes(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__
     }
 }
 
+// compute_statistics wall terms (statistics.f90:121-140): sums of U over
+// (2:nx-1, j, 2:nzg-1) for j = 1, 2, nyg-1, nyg (blockIdx.x = 0..3; the mixed
+// path adds the stored profile ubar(j) to each value) and of nu_t over
+// (2:nxg-1, 1:2, 2:nzg-1) and (2:nxg-1, nyg-1:nyg, 2:nzg-1) (blockIdx.x = 4,
+// 5).  blockIdx.y splits each sum; partials are stored in block order.  The
+// first block also clears the NaN flag that k_nan_scan sets afterwards.
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_stats_walls(Dims d, const S *__restrict__ U, const S *__restrict__ NT, const double *__restrict__ ubar,
+              double *__restrict__ part, double *__restrict__ flag) {
+    __shared__ double sh[8];
+    const int q = blockIdx.x, nb = gridDim.y;
+    const int t0 = threadIdx.x + blockIdx.y * 256, ts = 256 * nb;
+    double acc = 0.0;
+    if (q < 4) {
+        const int j = q == 0 ? 1 : (q == 1 ? 2 : (q == 2 ? d.nyg - 1 : d.nyg));
+        const int ni = d.nx - 2, n = ni * (d.nzg - 2);
+        const double ub = ubar ? ubar[j] : 0.0;
+        for (int e = t0; e < n; e += ts) {
+            const int i = 2 + e % ni, k = 2 + e / ni;
+            acc += (double)U[i3(d.nx, d.nyg, i, j, k)] + ub;
+        }
+    } else {
+        const int j0 = q == 4 ? 1 : d.nyg - 1;
+        const int ni = d.nxg - 2, n = ni * 2 * (d.nzg - 2);
+        for (int e = t0; e < n; e += ts) {
+            const int i = 2 + e % ni, r = e / ni;
+            acc += (double)NT[i3(d.nxg, d.nyg, i, j0 + (r & 1), 2 + (r >> 1))];
+        }
+    }
+    acc = block_reduce(acc, OpAdd(), sh);
+    if (threadIdx.x == 0) part[q * nb + blockIdx.y] = acc;
+    if (q == 0 && blockIdx.y == 0 && threadIdx.x == 0) *flag = 0.0;
+}
+
+// Any(Isnan(U)), Any(Isnan(V)), Any(Isnan(W)) over the whole arrays, ghosts
+// included (statistics.f90:165-167): blockIdx.y picks the field; a thread that
+// meets a NaN stores 1 into the flag (every such store writes the same value)
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_nan_scan(const S *__restrict__ U, const S *__restrict__ V, const S *__restrict__ W, int64_t nu, int64_t nv,
+           int64_t nw, double *__restrict__ flag) {
+    const S *a = blockIdx.y == 0 ? U : (blockIdx.y == 1 ? V : W);
+    const int64_t n = blockIdx.y == 0 ? nu : (blockIdx.y == 1 ? nv : nw);
+    bool bad = false;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const S v = a[e];
+        bad = bad || (v != v);
+    }
+    if (bad) *flag = 1.0;
+}
+
+// u_tau, Re_tau and the bulk flows (statistics.f90:121-158) from the plane
+// sums (raw UV sums at the walls in out[6 ny], out[7 ny - 1]), the wall sums
+// of k_stats_walls and the bulk means the mass-flow kernels left in
+// pad[0], pad[1].  One thread; writes out[7 ny .. 7 ny + 3].
+__global__ void k_stats_final(Dims d, const double *__restrict__ y, const double *__restrict__ yg,
+                              const double *__restrict__ wpart, int nb, double *__restrict__ out,
+                              const Scalars *__restrict__ sc, double nu) {
+    const int ny = d.ny, nyg = d.nyg;
+    double s[6];
+    for (int q = 0; q < 6; q++) {
+        double a = 0.0;
+        for (int b = 0; b < nb; b++) a += wpart[q * nb + b];
+        s[q] = a;
+    }
+    const double cu = (double)((d.nx - 2) * (d.nzg - 2)), cn = (double)((d.nxg - 2) * (d.nzg - 2));
+    const double U1 = s[0] / cu, U2 = s[1] / cu, Un1 = s[2] / cu, Un2 = s[3] / cu;
+    const double dUb = (U2 - U1) / (yg[2] - yg[1]);
+    const double dUt = (Un2 - Un1) / (yg[nyg] - yg[nyg - 1]);
+    const double ntb = s[4] / cn / 2.0, ntt = s[5] / cn / 2.0;
+    const double UVb = out[6 * ny] / cn, UVt = out[6 * ny + ny - 1] / cn;
+    const double Ly = y[ny] - y[1];
+    const double utau = sqrt((UVt - UVb - (nu + ntt) * dUt + (nu + ntb) * dUb) / Ly);
+    out[7 * ny + 0] = utau;
+    out[7 * ny + 1] = utau * (y[ny] - y[1]) / 2.0 / nu;
+    out[7 * ny + 2] = sc->pad[0];
+    out[7 * ny + 3] = sc->pad[1];
+}
+
 // out[e] = sum of the STATS_NB block partials of statistic e, in block order
 __global__ void k_stats_fold(const double *__restrict__ part, double *__restrict__ out, int n, int nb) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1004,9 +1181,36 @@ void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, 
     hipLaunchKernelGGL((k_massflow_u_partial<S>), dim3(nb), dim3(256), 0, st, d, m, U, nullptr, part);
     hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, part, nb, sc, apply);
 }
-int massflow_parts(const Dims &d) {
+static int massflow_u_parts(const Dims &d) {
     const int64_t n = (int64_t)(d.nx - 2) * (d.nzg - 2);
     return (int)((n + 255) / 256);
+}
+static int massflow_v_parts(const Dims &d) {
+    const int64_t n = (int64_t)(d.nxg - 2) * (d.nzg - 2);
+    return (int)((n + 255) / 256);
+}
+int massflow_parts(const Dims &d) { return std::max(massflow_u_parts(d), massflow_v_parts(d)); }
+
+template <typename S>
+void launch_centerline_u(const Dims &d, const S *U, const double *ubar, double *part, Scalars *sc, int apply,
+                         hipStream_t st) {
+    const int nb = massflow_u_parts(d), jc = d.ny / 2 + 1;
+    hipLaunchKernelGGL((k_centerline_partial<S>), dim3(nb), dim3(256), 0, st, d, U, jc, part);
+    hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(512), 0, st, part, nb, sc, ubar, jc,
+                       (double)((d.nx - 2) * (d.nzg - 2)), apply ? (int)MF_CENTERLINE : (int)MF_CENTERLINE_MEAN);
+}
+template <typename S>
+void launch_massflow_v(const Dims &d, const double *y, const S *V, double *part, Scalars *sc, int apply,
+                       hipStream_t st) {
+    const int nb = massflow_v_parts(d);
+    hipLaunchKernelGGL((k_massflow_v_partial<S>), dim3(nb), dim3(256), 0, st, d, y, V, part);
+    hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(512), 0, st, part, nb, sc, (const double *)nullptr, 0, 0.0,
+                       apply ? (int)MF_V_APPLY : (int)MF_V);
+}
+template <typename S>
+void launch_add_corr_v(const Dims &d, S *V, const Scalars *sc, hipStream_t st) {
+    const int64_t n = (int64_t)(d.nxg - 2) * (d.ny - 2) * (d.nzg - 2);
+    hipLaunchKernelGGL((k_add_corr_v<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, V, sc);
 }
 
 template <typename S>
@@ -1051,9 +1255,27 @@ void launch_smagorinsky(const Dims &d, const Metrics &m, const double *y, const 
                        ubar, cs, nu, damping, tx, ty, nt);
 }
 
-// each plane is split over STATS_NB workgroups, so the launch fills the GPU
+// each plane is split over STATS_NB workgroups, so the launch fills the GPU;
+// each wall sum over STATS_WNB
 constexpr int STATS_NB = 8;
-size_t stats_out_doubles(const Dims &d) { return (size_t)7 * d.ny * (STATS_NB + 1) + 8; }
+constexpr int STATS_WNB = 16;
+size_t stats_out_doubles(const Dims &d) { return (size_t)7 * d.ny * (STATS_NB + 1) + 8 + 6 * STATS_WNB; }
+
+template <typename S>
+void launch_stats_tail(const Dims &d, const Metrics &m, const double *y, const double *yg, const S *U, const S *V,
+                       const S *W, const S *NT, const double *ubar, double *out, double *part, Scalars *sc, double nu,
+                       hipStream_t st) {
+    double *tail = out + 7 * (size_t)d.ny;
+    double *wpart = out + 7 * (size_t)d.ny * (STATS_NB + 1) + 8;
+    hipLaunchKernelGGL((k_stats_walls<S>), dim3(6, STATS_WNB), dim3(256), 0, st, d, U, NT, ubar, wpart, tail + 4);
+    const int64_t nu_ = (int64_t)d.nx * d.nyg * d.nzg, nv = (int64_t)d.nxg * d.ny * d.nzg,
+                  nw = (int64_t)d.nxg * d.nyg * d.nz;
+    const int nbs = (int)std::min<int64_t>(2048, (std::max(std::max(nu_, nv), nw) + 255) / 256);
+    hipLaunchKernelGGL((k_nan_scan<S>), dim3(nbs, 3), dim3(256), 0, st, U, V, W, nu_, nv, nw, tail + 4);
+    launch_massflow<S>(d, m, U, part, sc, 0, st);
+    launch_massflow_v<S>(d, y, V, part, sc, 0, st);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(1), 0, st, d, y, yg, wpart, STATS_WNB, out, sc, nu);
+}
 
 template <typename S>
 void launch_stats(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
@@ -1149,6 +1371,14 @@ INST(float, float)
     template void launch_massflow<S>(const Dims &, const Metrics &, const S *, double *, Scalars *,  \
                                      int, hipStream_t);                                                \
     template void launch_add_corr_u<S>(const Dims &, S *, const Scalars *, hipStream_t);              \
+    template void launch_centerline_u<S>(const Dims &, const S *, const double *, double *, Scalars *, \
+                                         int, hipStream_t);                                            \
+    template void launch_massflow_v<S>(const Dims &, const double *, const S *, double *, Scalars *,  \
+                                       int, hipStream_t);                                              \
+    template void launch_add_corr_v<S>(const Dims &, S *, const Scalars *, hipStream_t);              \
+    template void launch_stats_tail<S>(const Dims &, const Metrics &, const double *, const double *,  \
+                                       const S *, const S *, const S *, const S *, const double *,     \
+                                       double *, double *, Scalars *, double, hipStream_t);            \
     template void launch_dt<S>(const Dims &, const double *, const double *, const double *,          \
                                const S *, const S *, const S *, const double *, double *,             \
                                Scalars *, double, double, hipStream_t);                                \

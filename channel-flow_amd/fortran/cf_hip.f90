@@ -18,6 +18,7 @@ Module cf_hip
      Integer(c_int) :: eqwm_ncv, eqwm_max_iter
      Real(c_double) :: eqwm_stretch, eqwm_tol, tau_mean_eqwm
      Integer(c_int) :: wall_surrogate, device
+     Real(c_double) :: u_centerline
   End Type cf_params_t
 
   Interface
@@ -73,6 +74,18 @@ Module cf_hip
        Real(c_double), Intent(Out) :: a(*)
        Integer(c_int) :: rc
      End Function cf_get_avg_nu_t
+     Function cf_get_wall(ctx, ax, az, ti) Bind(C, name='cf_get_wall') Result(rc)
+       Import :: c_int, c_double, c_ptr
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: ax(*), az(*), ti(*)
+       Integer(c_int) :: rc
+     End Function cf_get_wall
+     Function cf_set_tau_init(ctx, ti) Bind(C, name='cf_set_tau_init') Result(rc)
+       Import :: c_int, c_double, c_ptr
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(In) :: ti(*)
+       Integer(c_int) :: rc
+     End Function cf_set_tau_init
      Function cf_set_nu_t(ctx, a) Bind(C, name='cf_set_nu_t') Result(rc)
        Import :: c_int, c_double, c_ptr
        Type(c_ptr), Value :: ctx
@@ -142,6 +155,18 @@ Module cf_hip
        Integer(c_int), Intent(Out) :: nn, ni
        Integer(c_int) :: rc
      End Function cf_compute_wall_model
+     Function cf_mean_mass_flow_v(ctx, q) Bind(C, name='cf_mean_mass_flow_v') Result(rc)
+       Import :: c_int, c_double, c_ptr
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: q
+       Integer(c_int) :: rc
+     End Function cf_mean_mass_flow_v
+     Function cf_mean_centerline_u(ctx, q) Bind(C, name='cf_mean_centerline_u') Result(rc)
+       Import :: c_int, c_double, c_ptr
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: q
+       Integer(c_int) :: rc
+     End Function cf_mean_centerline_u
      Function cf_mean_mass_flow_u(ctx, q) Bind(C, name='cf_mean_mass_flow_u') Result(rc)
        Import :: c_int, c_double, c_ptr
        Type(c_ptr), Value :: ctx
@@ -195,7 +220,7 @@ Contains
        End Do
     End If
     Write(*,*) 'Error in ', Trim(what), ': ', Trim(msg)
-    Stop 'Error: HIP library call failed'
+    Error Stop 'Error: HIP library call failed'
   End Subroutine cf_check
 
 End Module cf_hip

@@ -26,8 +26,15 @@ Module global
   Integer(Int32) :: LES_model, iwall_model, Dirichlet_nu_t, jref_eqwm, est_Tij
   Real   (Int64) :: alpha_mean_x, alpha_mean_y, alpha_mean_z, alpha_std, freq_mult, fil_size_2
   Integer(Int32) :: gwm_cu, gwm_c_test, gwm_diag, gwm_ai(6)
-  Real   (Int64) :: Retau, utau
-  Real   (Int64), Allocatable, Dimension(:) :: Umean, Vmean, Wmean, U2mean, V2mean, W2mean, UVmean
+  Real   (Int64) :: Retau, utau, Qflow_x, Qflow_y, Ucenterline_x
+  Real   (Int64), Allocatable, Dimension(:) :: Umean, Vmean, Wmean, U2mean, V2mean, W2mean, UVmean, P2mean
+  ! wall-model state saved for restarts: tau_eqwm_initial(nx,2,nzg)
+  Real   (Int64), Allocatable, Dimension(:,:,:) :: tau_eqwm_initial
+  ! set by compute_statistics when it ran this step; output_data then writes
+  ! the pressure saved by the projection instead of computing it
+  Logical :: pressure_computed = .False.
+  ! monitor clock (system_clock counts)
+  Integer(Int64) :: time1 = 0, time2 = 0
   ! settings the reference hard-codes, read from an optional trailing block
   Real   (Int64) :: stretch_y = 2.0d0, amp_ic = 0.1d0, Cs = 0.1d0
   Real   (Int64) :: eqwm_stretch = 1.10d0, eqwm_tol = 1d-7

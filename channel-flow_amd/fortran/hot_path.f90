@@ -2,7 +2,7 @@
 ! The reference's hot-path modules with their subroutine names and  !
 ! dummy arguments, each body a call into the HIP library.           !
 !   equations.f90:24,186,351   boundary_conditions.f90:20           !
-!   projection.f90:20,234      mass_flow.f90:25,103                 !
+!   projection.f90:20,234      mass_flow.f90:25,61,103,147,192      !
 !   pressure.f90:21                                                 !
 !   subgrid.f90:23             wallmodel.f90:24   eqwm.f90:90       !
 !   time_integration.f90:89,179                                     !
@@ -104,6 +104,14 @@ Contains
     Qflow = q
   End Subroutine compute_mean_mass_flow_U
 
+  Subroutine compute_mean_mass_flow_V(V_, Qflow)
+    Real(Int64), Dimension(nxg,ny,nzg), Intent(In) :: V_
+    Real(Int64), Intent(Out) :: Qflow
+    Real(c_double) :: q
+    Call cf_check(cf_mean_mass_flow_v(ctx, q), 'compute_mean_mass_flow_V')
+    Qflow = q
+  End Subroutine compute_mean_mass_flow_V
+
   Subroutine compute_dPx_for_constant_mass_flow(U_, dPdx_)
     Real(Int64), Dimension(nx,nyg,nzg), Intent(In) :: U_
     Real(Int64), Intent(Out) :: dPdx_
@@ -111,6 +119,22 @@ Contains
     Call compute_mean_mass_flow_U(U_, q)
     dPdx_ = Qflow_x_0 - q
   End Subroutine compute_dPx_for_constant_mass_flow
+
+  Subroutine compute_dPy_for_constant_mass_flow(V_, dPdy_)
+    Real(Int64), Dimension(nxg,ny,nzg), Intent(In) :: V_
+    Real(Int64), Intent(Out) :: dPdy_
+    Real(Int64) :: q
+    Call compute_mean_mass_flow_V(V_, q)
+    dPdy_ = Qflow_y_0 - q
+  End Subroutine compute_dPy_for_constant_mass_flow
+
+  Subroutine compute_dPx_for_constant_Ucenterline(U_, dPdx_)
+    Real(Int64), Dimension(nx,nyg,nzg), Intent(In) :: U_
+    Real(Int64), Intent(Out) :: dPdx_
+    Real(c_double) :: q
+    Call cf_check(cf_mean_centerline_u(ctx, q), 'compute_dPx_for_constant_Ucenterline')
+    dPdx_ = Ucenterline_x - q
+  End Subroutine compute_dPx_for_constant_Ucenterline
 End Module mass_flow
 
 Module subgrid

@@ -1,8 +1,9 @@
 !--------------------------------------------------------------!
-! Input file, initial field, snapshots, statistics and the      !
-! driver loop.  The input file is the reference's positional    !
-! format (input_output.f90:21-81) with an optional trailing     !
-! block for the settings the reference hard-codes.              !
+! Input file, initial field, snapshots and the statistics file  !
+! (modules input_output and initialization of the reference).   !
+! The input file is the reference's positional format           !
+! (input_output.f90:21-81) with an optional trailing block for  !
+! the settings the reference hard-codes.                        !
 !--------------------------------------------------------------!
 Module input_output
   Use global
@@ -82,12 +83,17 @@ Contains
     End Do
   End Subroutine make_grid
 
-  ! Snapshot reader: the reference's stream layout (input_output.f90:231-384)
-  Subroutine read_input_data(have_trailer)
-    Logical, Intent(Out) :: have_trailer
+  ! Snapshot reader: the reference's stream layout (input_output.f90:231-384):
+  ! grids, then U, V, W.  The reference writer follows them with P and nu_t
+  ! (input_output.f90:487-544); both are read when present (P becomes the
+  ! module array, nu_t is recomputed by the first substep anyway).  Restart
+  ! state that the reference format has no room for (t, dt, the carried
+  ! forcing, the target bulk flows, tau_eqwm_initial) lives in the side file
+  ! <filein>.restart, which a reference reader never opens.
+  Subroutine read_input_data(restart)
+    Logical, Intent(Out) :: restart
     Integer(Int32) :: n1, nn(3)
     Integer :: ios
-    Real(Int64) :: tr(3)
     Open(1, file=filein, access='stream', form='unformatted', action='read')
     Read(1) n1; If ( n1 /= nx ) Stop 'nx_f/=nx'
     Read(1) x
@@ -95,39 +101,66 @@ Contains
     Read(1) y
     Read(1) n1; If ( n1 /= nz ) Stop 'nz_f/=nz'
     Read(1) z
-    Read(1) n1; Read(1) xm
-    Read(1) n1; Read(1) ym
-    Read(1) n1; Read(1) zm
+    Read(1) n1; If ( n1 /= nx-1 ) Stop 'nxm_f/=nxm'
+    Read(1) xm
+    Read(1) n1; If ( n1 /= ny-1 ) Stop 'nym_f/=nym'
+    Read(1) ym
+    Read(1) n1; If ( n1 /= nz-1 ) Stop 'nzm_f/=nzm'
+    Read(1) zm
     Read(1) nn; If ( nn(1)/=nx .Or. nn(2)/=nyg .Or. nn(3)/=nzg ) Stop 'Error! wrong size in input file (U)'
     Read(1) U
     Read(1) nn; If ( nn(1)/=nxg .Or. nn(2)/=ny .Or. nn(3)/=nzg ) Stop 'Error! wrong size in input file (V)'
     Read(1) V
     Read(1) nn; If ( nn(1)/=nxg .Or. nn(2)/=nyg .Or. nn(3)/=nz ) Stop 'Error! wrong size in input file (W)'
     Read(1) W
-    ! optional P block and restart trailer written by output_data
-    have_trailer = .False.
+    ! optional P and nu_t records, recognised by their extents
     Read(1, iostat=ios) nn
-    If ( ios == 0 ) Read(1, iostat=ios) P
-    If ( ios == 0 ) Read(1, iostat=ios) tr
-    If ( ios == 0 ) Then
-       t = tr(1); dPdx = tr(2); Qflow_x_0 = tr(3)
-       have_trailer = .True.
+    If ( ios == 0 .And. nn(1) == nxg .And. nn(2) == nyg .And. nn(3) == nzg ) Then
+       Read(1, iostat=ios) P
+       If ( ios == 0 ) Read(1, iostat=ios) nn
+       If ( ios == 0 .And. nn(1) == nxg .And. nn(2) == nyg .And. nn(3) == nzg ) Read(1, iostat=ios) nu_t
     End If
     Close(1)
+    Inquire(file=Trim(filein)//'.restart', exist=restart)
   End Subroutine read_input_data
 
-  ! Snapshot writer: same layout, all z planes of every field (the
-  ! reference drops the last plane on one rank), then P, and a trailer
-  ! with t, dPdx and Qflow_x_0 for restart
-  Subroutine output_data(final)
-    Logical, Intent(In) :: final
+  ! restart side file of a snapshot: the 7 device scalars
+  ! (t, dt, dPdx, dPdz, Qflow_x_0, dPdy, Qflow_y_0), then the extents and
+  ! values of tau_eqwm_initial (0 0 0 without the wall model)
+  Subroutine read_restart_file(fname)
+    Character(*), Intent(In) :: fname
+    Real(Int64) :: s(7)
+    Integer(Int32) :: nn(3)
+    Integer :: ios
+    Open(2, file=fname, access='stream', form='unformatted', action='read')
+    Read(2) s
+    t = s(1); dt = s(2); dPdx = s(3); dPdz = s(4); Qflow_x_0 = s(5); dPdy = s(6); Qflow_y_0 = s(7)
+    Read(2, iostat=ios) nn
+    If ( ios == 0 .And. nn(1) == nx .And. nn(2) == 2 .And. nn(3) == nzg ) Read(2) tau_eqwm_initial
+    Close(2)
+  End Subroutine read_restart_file
+
+  ! Snapshot writer (input_output.f90:393-544) every nsave steps: grids, U,
+  ! V, W, P, nu_t in the reference's record sequence, all z planes of each
+  ! field (the reference's one-rank writer drops the last plane).  P is the
+  ! pressure the projection saved when compute_statistics ran this step,
+  ! the actual pressure (compute_pressure) otherwise, as the reference
+  ! chooses it.  The restart side file goes next to it.
+  Subroutine output_data
     Character(200) :: fname
     Character(8)   :: ext
-    If ( .Not. final .And. Mod(istep,nsave) /= 0 ) Return
+    Real(Int64), Allocatable :: ax(:,:,:), az(:,:,:)
+    If ( Mod(istep,nsave) /= 0 ) Return
     Call sync_to_host
-    Call cf_check(cf_get_pressure(ctx, P), 'cf_get_pressure')
+    If ( pressure_computed ) Then
+       Call cf_check(cf_get_pressure(ctx, P), 'cf_get_pressure')
+    Else
+       Call cf_check(cf_compute_pressure(ctx, P), 'compute_pressure')
+    End If
+    Call cf_check(cf_get_nu_t(ctx, nu_t), 'cf_get_nu_t')
     Write(ext,'(I8)') istep + nstep_init
     fname = Trim(Adjustl(fileout))//'.'//Trim(Adjustl(ext))
+    Write(*,*) 'writting ', Trim(Adjustl(fname))
     Open(1, file=fname, access='stream', form='unformatted', action='write', status='replace')
     Write(1) Shape(x), x
     Write(1) Shape(y), y
@@ -143,32 +176,39 @@ Contains
     Write(1) W
     Write(1) nxg, nyg, nzg
     Write(1) P
-    Write(1) t, dPdx, Qflow_x_0
+    Write(1) nxg, nyg, nzg
+    Write(1) nu_t
     Close(1)
+    Call sync_scalars_from_device
+    Open(2, file=Trim(fname)//'.restart', access='stream', form='unformatted', action='write', status='replace')
+    Write(2) t, dt, dPdx, dPdz, Qflow_x_0, dPdy, Qflow_y_0
+    If ( iwall_model == 3 ) Then
+       Allocate(ax(nx,2,nzg), az(nxg,2,nz))
+       Call cf_check(cf_get_wall(ctx, ax, az, tau_eqwm_initial), 'cf_get_wall')
+       Write(2) nx, 2, nzg
+       Write(2) tau_eqwm_initial
+    Else
+       Write(2) 0, 0, 0
+    End If
+    Close(2)
   End Subroutine output_data
 
-  ! Plane statistics (statistics.f90:43-155) and the .stats.txt file
-  ! (input_output.f90:549-570)
-  Subroutine compute_statistics
-    Real(c_double), Allocatable :: out(:)
+  ! .stats.txt of the plane statistics (input_output.f90:549-570); the last
+  ! column is the reference's P2mean, which statistics.f90:117-118 assigns
+  ! from the normalised UVmean divided once more by (nxg-2)(nzg-2)
+  Subroutine output_statistics
     Character(200) :: fname
     Character(8)   :: ext
     Integer :: jj
-    If ( Mod(istep,nstats) /= 0 .And. istep /= 1 ) Return
-    Allocate(out(7*ny))
-    Call cf_check(cf_statistics(ctx, out), 'cf_statistics')
-    Umean  = out(1:ny);        Vmean  = out(ny+1:2*ny);   Wmean  = out(2*ny+1:3*ny)
-    U2mean = out(3*ny+1:4*ny); V2mean = out(4*ny+1:5*ny); W2mean = out(5*ny+1:6*ny)
-    UVmean = out(6*ny+1:7*ny)
     Write(ext,'(I8)') istep + nstep_init
     fname = Trim(Adjustl(fileout))//'.'//Trim(Adjustl(ext))//'.stats.txt'
     Open(3, file=fname, form='formatted', action='write')
     Write(3,'(A,4F15.8,4I8)') '%', t, Retau, utau, nu, nx_global, ny_global, nz_global, istep
     Do jj = 1, ny
-       Write(3,'(7F15.8)') y(jj), Umean(jj), U2mean(jj), V2mean(jj), W2mean(jj), UVmean(jj), 0d0
+       Write(3,'(8F15.8)') y(jj), Umean(jj), U2mean(jj), V2mean(jj), W2mean(jj), UVmean(jj), P2mean(jj)
     End Do
     Close(3)
-  End Subroutine compute_statistics
+  End Subroutine output_statistics
 
 End Module input_output
 
@@ -236,7 +276,8 @@ Contains
   Subroutine initialize
     Type(cf_params_t) :: prm
     Integer :: j
-    Real(Int64) :: best
+    Real(Int64) :: best, q, qref
+    Real(c_double) :: qd
     Logical :: restart
     Call read_input_parameters
     t = 0d0
@@ -244,7 +285,9 @@ Contains
     nxg = nx + 1; nyg = ny + 1; nzg = nz + 1
     Allocate(x(nx), y(ny), z(nz), xm(nx-1), ym(ny-1), zm(nz-1), xg(nxg), yg(nyg), zg(nzg))
     Allocate(U(nx,nyg,nzg), V(nxg,ny,nzg), W(nxg,nyg,nz), P(nxg,nyg,nzg), nu_t(nxg,nyg,nzg))
-    Allocate(Umean(ny), Vmean(ny), Wmean(ny), U2mean(ny), V2mean(ny), W2mean(ny), UVmean(ny))
+    Allocate(Umean(ny), Vmean(ny), Wmean(ny), U2mean(ny), V2mean(ny), W2mean(ny), UVmean(ny), P2mean(ny))
+    Allocate(tau_eqwm_initial(nx,2,nzg))
+    P = 0d0; nu_t = 0d0; tau_eqwm_initial = dPdx_ref
     restart = .False.
     If ( random_init == 1 ) Then
        Call make_grid
@@ -264,6 +307,9 @@ Contains
        End Do
     End If
     If ( jref_eqwm < 2 ) jref_eqwm = 2
+    ! initialization.f90:453-455: the centreline target comes from freq_mult
+    Ucenterline_x = 0d0
+    If ( x_mass_cte == 2 .And. Retau_int /= 0 ) Ucenterline_x = freq_mult
     prm%nx = nx; prm%ny = ny; prm%nz = nz; prm%precision = precision
     prm%cfl = CFL; prm%nu = nu; prm%dPdx = dPdx; prm%dPdz = dPdz; prm%omega_z = Omega_z
     prm%x_mass_cte = x_mass_cte; prm%y_mass_cte = y_mass_cte
@@ -271,7 +317,7 @@ Contains
     prm%dirichlet_nu_t = Dirichlet_nu_t; prm%jref_eqwm = jref_eqwm
     prm%cs = Cs; prm%eqwm_ncv = eqwm_ncv; prm%eqwm_max_iter = eqwm_max_iter
     prm%eqwm_stretch = eqwm_stretch; prm%eqwm_tol = eqwm_tol; prm%tau_mean_eqwm = dPdx_ref
-    prm%wall_surrogate = 0; prm%device = 0
+    prm%wall_surrogate = 0; prm%device = 0; prm%u_centerline = Ucenterline_x
     Call cf_check(cf_create(prm, x, y, z, ctx), 'cf_create')
     If ( random_init == 1 ) Then
        Call init_flow
@@ -283,18 +329,40 @@ Contains
        Call sync_to_device
     End If
     If ( restart ) Then
-       ! continue exactly: time, carried forcing and target bulk flow
+       ! continue exactly: time, carried forcing, target bulk flows and the
+       ! wall model's previous stress
        Call sync_scalars_from_device
-       Call read_restart_scalars
+       Call read_restart_file(Trim(filein)//'.restart')
        Call sync_scalars_to_device
+       If ( iwall_model == 3 ) Call cf_check(cf_set_tau_init(ctx, tau_eqwm_initial), 'cf_set_tau_init')
     Else
+       ! initialization.f90:431-463: the bulk flow of the initial field,
+       ! rescaled to a tabulated bulk flow when Retau_int asks for it
+       If ( Abs(Retau_int) > 0 .And. x_mass_cte == 1 ) Then
+          Call cf_check(cf_mean_mass_flow_u(ctx, qd), 'compute_mean_mass_flow_U')
+          q = qd
+          Write(*,*) 'warning: initial mass flow corrected'
+          Write(*,*) '   flag:', Retau_int
+          Select Case ( Retau_int )
+          Case ( 4200 ); qref = 0.873791265477842d0
+          Case ( 2000 ); qref = 0.898245304195718d0
+          Case ( 950 );  qref = 0.899952805947840d0
+          Case ( 550 );  qref = 0.899885379198784d0
+          Case ( 395 );  qref = 0.871974131836648d0
+          Case Default
+             If ( Retau_int < 0 ) Then
+                qref = -Retau_int*q
+             Else
+                Stop 'Error: Unknown mass flow correction'
+             End If
+          End Select
+          Call sync_to_host
+          U = qref/q*U
+          Call sync_to_device
+       End If
        Call cf_check(cf_init_mass_flow(ctx), 'cf_init_mass_flow')
        Call sync_scalars_from_device
     End If
   End Subroutine initialize
-  Subroutine read_restart_scalars
-    Logical :: h
-    Call read_input_data(h)
-  End Subroutine read_restart_scalars
 End Module initialization
 

@@ -1,22 +1,25 @@
 !--------------------------------------------------------------!
-! Driver with the call sequence of main.f90:61-87; steps that   !
-! need no host work run back to back on the device.            !
+! Driver with the call sequence of main.f90:61-87; runs of      !
+! steps that need no host work (statistics, monitor, snapshot)  !
+! go to the device as one batch of graph-replayed steps.  The   !
+! reference's own main.f90 also builds against these modules.  !
 !--------------------------------------------------------------!
 Program channel_FD
   Use global
   Use input_output
   Use initialization
   Use time_integration
-  Use projection
+  Use monitor
+  Use statistics
+  Use finalization
   Implicit None
-  Real(Int64) :: divmax
   Integer(Int32) :: chunk
 
   Call initialize
-  Write(*,'(A,3I6,A,I2)') ' channel_FD (HIP): grid ', nx, ny, nz, '  precision ', precision
+  Call summary
   istep = 0
   Do While ( istep < nsteps )
-     ! advance to the next step that needs the host (stats, monitor, save)
+     ! advance to the next step that needs the host
      chunk = 1
      Do While ( istep + chunk < nsteps .And. Mod(istep+chunk,nstats) /= 0 .And. &
           Mod(istep+chunk,nmonitor) /= 0 .And. Mod(istep+chunk,nsave) /= 0 .And. istep+chunk /= 1 )
@@ -25,13 +28,8 @@ Program channel_FD
      Call advance_steps(chunk)
      istep = istep + chunk
      Call compute_statistics
-     If ( Mod(istep,nmonitor) == 0 ) Then
-        Call check_divergence(divmax)
-        Write(*,'(A,I8,A,ES14.6,A,ES12.4,A,ES14.6,A,ES10.3)') ' step ', istep, ' t ', t, ' dt ', dt, &
-             ' dPdx ', dPdx, ' div ', divmax
-     End If
-     Call output_data(.False.)
+     Call output_monitor
+     Call output_data
   End Do
-  Call output_data(.True.)
-  Call cf_check(cf_destroy(ctx), 'cf_destroy')
+  Call finalize
 End Program channel_FD

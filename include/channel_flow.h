@@ -38,7 +38,10 @@ typedef struct cf_params {
     double nu;
     double dPdx, dPdz;       /* initial forcing                                   */
     double omega_z;          /* Rossby_plus*utau/2 (input_output.f90:84)         */
-    int x_mass_cte, y_mass_cte;
+    int x_mass_cte;          /* 0 constant dPdx, 1 constant bulk flow
+                                (mass_flow.f90:103-132), 2 constant centreline
+                                velocity (mass_flow.f90:192-211)                 */
+    int y_mass_cte;          /* 1 constant bulk flow in y (mass_flow.f90:147-176) */
     int les_model;           /* 0 none, 1 constant-coefficient Smagorinsky,
                                 2 dynamic Smagorinsky (subgrid.f90:376-538)     */
     int iwall_model;         /* 0 no-slip, 3 equilibrium wall model               */
@@ -52,6 +55,8 @@ typedef struct cf_params {
     double tau_mean_eqwm;    /* fallback stress = dPdx_ref                        */
     int wall_surrogate;      /* 0 ODE only, 1 learned MLP with ODE fallback       */
     int device;              /* HIP device ordinal                                */
+    double u_centerline;     /* x_mass_cte = 2: target centreline velocity
+                                Ucenterline_x (initialization.f90:453-455)       */
 } cf_params;
 
 const char *cf_last_error(void);
@@ -107,10 +112,17 @@ int cf_compute_eddy_viscosity(cf_ctx *c);
 int cf_compute_wall_model(cf_ctx *c, int *n_nan, int *n_iter);
 /* compute_mean_mass_flow_U (mass_flow.f90:25-52) of the current state */
 int cf_mean_mass_flow_u(cf_ctx *c, double *qflow);
+/* compute_mean_mass_flow_V (mass_flow.f90:61-88) of the current state */
+int cf_mean_mass_flow_v(cf_ctx *c, double *qflow);
+/* the centreline mean of compute_dPx_for_constant_Ucenterline
+ * (mass_flow.f90:200-207): U(1:nx-2, ny/2+1, 1:nzg-2) over (nx-2)(nzg-2) */
+int cf_mean_centerline_u(cf_ctx *c, double *umean);
 /* check_divergence (projection.f90:234-256) plus max |div| */
 int cf_check_divergence(cf_ctx *c, double *max_signed, double *max_abs);
-/* compute_statistics plane means (statistics.f90:43-118), FP64:
- * out = Umean,Vmean,Wmean,U2mean,V2mean,W2mean,UVmean (ny each) */
+/* compute_statistics (statistics.f90:22-171), FP64, 7*ny + 4 values:
+ * Umean,Vmean,Wmean,U2mean,V2mean,W2mean,UVmean (ny each), then utau, Retau,
+ * Qflow_x, Qflow_y.  Returns nonzero ("Error: NaNs!") when U, V or W holds a
+ * NaN (statistics.f90:165-167); the values are written either way. */
 int cf_statistics(cf_ctx *c, double *out);
 
 /* solve_u_eqwm (eqwm.f90:90-97) for a batch of n wall points on the device.

@@ -913,6 +913,17 @@ double orc_mean_mass_flow_U(const orc_t *o, const double *U) {
             }
     return q / nrm;
 }
+/* mass_flow.f90:192-209: the plane mean of U(1:nx-2, jc, 1:nzg-2) at
+ * jc = Floor(Real(ny/2))+1 (integer division first), over the count
+ * (nx_global-2)(nzg_global-2) -- the k range holds the periodic image k = 1
+ * of plane nzg-2 as well as that plane */
+double orc_mean_centerline_U(const orc_t *o, const double *U) {
+    const int jc = PLANT(16, o->ny / 2, o->ny / 2 + 1);
+    double s = 0.0;
+    for (int k = PLANT(17, 2, 1); k <= o->nzg - 2; k++)
+        for (int i = 1; i <= o->nx - 2; i++) s = s + u(i, jc, k);
+    return s / (double)((o->nx - 2) * (o->nzg - 2));
+}
 double orc_mean_mass_flow_V(const orc_t *o, const double *V) {
     const double *y = o->y;
     double q = 0.0, nrm = 0.0;
@@ -1404,9 +1415,16 @@ void orc_rk3_step(orc_t *o) {
                 for (int i = 2; i <= nx - 1; i++) U[IU(i, j, k)] = U[IU(i, j, k)] + d;
         o->dPdx = d / o->dt;
         orc_apply_boundary_conditions(o);
+    } else if (o->p.x_mass_cte == 2) { /* :158-163 */
+        double d = o->p.u_centerline - orc_mean_centerline_U(o, U);
+        for (int k = 2; k <= nzg - 1; k++)
+            for (int j = 2; j <= nyg - 1; j++)
+                for (int i = 2; i <= nx - 1; i++) U[IU(i, j, k)] = U[IU(i, j, k)] + d;
+        o->dPdx = d / o->dt;
+        orc_apply_boundary_conditions(o);
     }
     if (o->p.y_mass_cte == 1) { /* :166-171 */
-        double d = o->Qflow_y_0 - orc_mean_mass_flow_V(o, V);
+        double d = PLANT(18, orc_mean_mass_flow_V(o, V) - o->Qflow_y_0, o->Qflow_y_0 - orc_mean_mass_flow_V(o, V));
         for (int k = 2; k <= nzg - 1; k++)
             for (int j = 2; j <= ny - 1; j++)
                 for (int i = 2; i <= nxg - 1; i++) V[IV(i, j, k)] = V[IV(i, j, k)] + d;

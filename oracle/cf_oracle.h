@@ -39,6 +39,8 @@ typedef struct orc_params {
     int eqwm_ncv;          /* wallmodel.f90:621 passes 100                   */
     int eqwm_max_iter;     /* eqwm.f90:110 hard-codes 100                    */
     int nthreads;          /* OpenMP threads for the restatement (1 = serial)*/
+    double u_centerline;   /* x_mass_cte = 2 target Ucenterline_x
+                              (initialization.f90:453-455)                  */
 } orc_params;
 
 typedef struct orc orc_t;
@@ -75,6 +77,7 @@ double orc_compute_dt(orc_t *o);
 void orc_rk3_step(orc_t *o);
 double orc_mean_mass_flow_U(const orc_t *o, const double *U);
 double orc_mean_mass_flow_V(const orc_t *o, const double *V);
+double orc_mean_centerline_U(const orc_t *o, const double *U);
 void orc_eddy_viscosity(orc_t *o);
 void orc_wall_model(orc_t *o, int *n_nan, int *n_iter);
 void orc_check_divergence(const orc_t *o, double *max_signed, double *max_abs);

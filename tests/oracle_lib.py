@@ -18,7 +18,7 @@ class OrcParams(ctypes.Structure):
                 ["cfl", "nu", "dPdx", "dPdz", "omega_z", "cs", "eqwm_stretch", "eqwm_tol", "tau_mean_eqwm"]] + \
                [(n, ctypes.c_int) for n in
                 ["x_mass_cte", "y_mass_cte", "les_model", "iwall_model", "dirichlet_nu_t", "jref_eqwm",
-                 "eqwm_ncv", "eqwm_max_iter", "nthreads"]]
+                 "eqwm_ncv", "eqwm_max_iter", "nthreads"]] + [("u_centerline", ctypes.c_double)]
 
 
 _libs = {}
@@ -53,6 +53,7 @@ def lib(path=None):
             "orc_rk3_step": (None, [v]),
             "orc_mean_mass_flow_U": (ctypes.c_double, [v, _dp]),
             "orc_mean_mass_flow_V": (ctypes.c_double, [v, _dp]),
+            "orc_mean_centerline_U": (ctypes.c_double, [v, _dp]),
             "orc_eddy_viscosity": (None, [v]),
             "orc_wall_model": (None, [v, _ip, _ip]),
             "orc_check_divergence": (None, [v, _dp, _dp]),
@@ -99,6 +100,7 @@ class Oracle:
         p.dirichlet_nu_t = cfg.dirichlet_nu_t
         p.jref_eqwm = jref if jref is not None else (cfg.jref(self.y) if cfg.iwall_model == 3 else 2)
         p.eqwm_ncv, p.eqwm_max_iter, p.nthreads = cfg.eqwm_ncv, cfg.eqwm_max_iter, nthreads
+        p.u_centerline = cfg.u_centerline
         self.p = p
         self.h = self.L.orc_create(nx, ny, nz, P(self.x), P(self.y), P(self.z), ctypes.byref(p))
 
@@ -228,6 +230,14 @@ class Oracle:
     def compute_mean_mass_flow_U(self):
         U, _, _ = self.get_state()
         return self.L.orc_mean_mass_flow_U(self.h, P(U))
+
+    def compute_mean_centerline_U(self):
+        U, _, _ = self.get_state()
+        return self.L.orc_mean_centerline_U(self.h, P(U))
+
+    def compute_mean_mass_flow_V(self):
+        _, V, _ = self.get_state()
+        return self.L.orc_mean_mass_flow_V(self.h, P(V))
 
     def init_mass_flow(self):
         self.set_scalars(Qflow_x_0=self.compute_mean_mass_flow_U())

@@ -268,3 +268,75 @@ def check_statistics_utau_and_bulk(path=None):
 ALL = [check_rhs_linear_profiles, check_rhs_v_stretched_viscous, check_rhs_v_nu_t_on_edges,
        check_rhs_wall_mirror, check_statistics_ranges, check_statistics_uv_pairing,
        check_statistics_utau_and_bulk]
+
+
+# -------------------------------------------------------------- mass flow
+def check_centerline_mean(path=None):
+    """compute_dPx_for_constant_Ucenterline (mass_flow.f90:192-209) takes the
+    plane mean of U(1:nx-2, jc, 1:nzg-2) with jc = Floor(Real(ny/2))+1, the
+    integer division first.  U = f(yg) + e g_k (g_k the periodic pattern of
+    check_statistics_ranges, k = 1 counted besides its image nzg-2):
+    mean = f(yg(jc)) + e g_1/(nzg-2)."""
+    o = Oracle(cfg(cells=(12, 14, 8)), path=path)
+    yg = o.grid()["yg"]
+    nzg, ny = o.nzg, o.ny
+    Nz = nzg - 3
+    g = np.cos(2 * math.pi * (np.arange(1, nzg + 1) - 2) / Nz)
+    f = lambda t: 1.5 * t * (2 - t)
+    U = np.asfortranarray(np.broadcast_to(f(yg)[None, :, None] + 0.2 * g[None, None, :], o.shape_u))
+    _, V, W = _zeros(o)
+    o.set_state(U, V, W)
+    jc = ny // 2 + 1
+    close(o.compute_mean_centerline_U(), f(yg[jc - 1]) + 0.2 * math.cos(2 * math.pi / Nz) / (nzg - 2), 1e-14,
+          "centreline mean")
+
+
+def _stepped_pair(kw0, kw1, path):
+    """The same projected random field stepped once by two oracles whose
+    configurations differ only in the mass-flow options."""
+    outs = []
+    for kw in (kw0, kw1):
+        c = cfg(cells=(12, 14, 8), nu=0.01, cfl=-1e-3, dPdx=0.002, **kw)
+        o = Oracle(c, path=path)
+        U, V, W = cases.initial_field(c)
+        o.initialise(U, V, W)
+        o.set_scalars(Qflow_y_0=0.0)
+        before = o.get_state()
+        o.run(1)
+        outs.append((o, before, o.get_state()))
+    return outs
+
+
+def check_mass_flow_corrections(path=None):
+    """After the third substep (time_integration.f90:152-171):
+    x_mass_cte = 2 adds d = Ucenterline_x - <U(1:nx-2, jc, 1:nzg-2)> to
+    every interior U and sets dPdx = d/dt; y_mass_cte = 1 adds
+    d = Qflow_y_0 - trapz(V)/norm to the interior rows 2..ny-1 of V (the wall
+    rows stay 0) and sets dPdy = d/dt.  Stepping the same field with and
+    without the option isolates the correction, whose value the test
+    computes from the uncorrected field with the reference's ranges."""
+    (o0, _, (U0, V0, W0)), (o2, _, (U2, V2, W2)) = _stepped_pair(dict(x_mass_cte=0),
+                                                               dict(x_mass_cte=2, u_centerline=1.2), path)
+    nx, ny, nzg = o0.nx, o0.ny, o0.nzg
+    jc = ny // 2 + 1
+    d = 1.2 - U0[0:nx - 2, jc - 1, 0:nzg - 2].sum() / ((nx - 2) * (nzg - 2))
+    close(U2[1:nx - 1, 1:-1, 1:nzg - 1] - U0[1:nx - 1, 1:-1, 1:nzg - 1], d, 1e-13, "centreline correction of U")
+    close(V2, V0, 1e-15, "V untouched by the x correction")
+    dt = o2.scalars["dt"]
+    close(o2.scalars["dPdx"] * dt, d, 1e-13, "dPdx = d/dt")
+    (o0, _, (U0, V0, W0)), (o1, _, (U1, V1, W1)) = _stepped_pair(dict(x_mass_cte=0), dict(x_mass_cte=0, y_mass_cte=1),
+                                                               path)
+    y = o0.y
+    nxg = o0.nxg
+    col = V0[1:nxg - 1, :, 1:nzg - 1]
+    q = ((col[:, 1:, :] + col[:, :-1, :]) * 0.5 * np.diff(y)[None, :, None]).sum()
+    nrm = (2 * 0.5 * np.diff(y)).sum() * (nxg - 2) * (nzg - 2)
+    d = 0.0 - q / nrm
+    assert abs(d) > 1e-6, d  # the random field carries a bulk V to remove
+    close(V1[1:nxg - 1, 1:ny - 1, 1:nzg - 1] - V0[1:nxg - 1, 1:ny - 1, 1:nzg - 1], d, 1e-13, "y correction of V")
+    close(V1[:, [0, ny - 1], :], 0.0, 1e-15, "wall rows of V")
+    close(U1, U0, 1e-15, "U untouched by the y correction")
+    close(o1.scalars["dPdy"] * o1.scalars["dt"], d, 1e-13, "dPdy = d/dt")
+
+
+ALL += [check_centerline_mean, check_mass_flow_corrections]

@@ -164,9 +164,57 @@ def test_statistics_match():
     U, V, W = cases.initial_field(cfg)
     g.initialise(U, V, W)
     o.initialise(U, V, W)
+    g.run(2, graph=True)
+    o.run(2)
     gs, os_ = g.compute_statistics(), o.compute_statistics()
-    for k in gs:
-        np.testing.assert_allclose(gs[k], os_[k], rtol=1e-12, atol=1e-14)
+    assert set(gs) == set(os_)
+    for k in os_:
+        np.testing.assert_allclose(gs[k], os_[k], rtol=1e-12, atol=1e-14, err_msg=k)
+    assert os_["utau"] > 0 and os_["Retau"] > 0
+
+
+@pytest.mark.parametrize("les", [0, 1])
+def test_statistics_of_random_fields_match(les):
+    """Plane statistics, u_tau, Re_tau and bulk flows of a field that varies
+    in x and z (the reference's sum ranges with their duplicated periodic
+    images matter) with an eddy viscosity at the walls."""
+    cfg = small_cfg(les_model=les, cs=0.1)
+    g, o = pair(cfg)
+    rng = np.random.default_rng(11)
+    U = np.asfortranarray(1.0 + 0.3 * rng.standard_normal(g.shape_u))
+    V = np.asfortranarray(0.2 * rng.standard_normal(g.shape_v))
+    W = np.asfortranarray(0.2 * rng.standard_normal(g.shape_w))
+    # a wall-normal gradient that makes utau^2 positive
+    U[:, 0, :] -= 3.0
+    U[:, -1, :] -= 3.0
+    g.set_state(U, V, W)
+    o.set_state(U, V, W)
+    if les:
+        nt = np.asfortranarray(np.abs(rng.standard_normal(g.shape_c)) * 1e-3)
+        g.set_nu_t(nt)
+        o.set_nu_t(nt)
+    gs, os_ = g.compute_statistics(), o.compute_statistics()
+    for k in os_:
+        np.testing.assert_allclose(gs[k], os_[k], rtol=1e-12, atol=1e-14, err_msg=k)
+    assert np.isfinite(os_["utau"]) and os_["utau"] > 0
+
+
+def test_statistics_nan_is_an_error():
+    """statistics.f90:165-167: a NaN anywhere in U, V or W (ghosts included)
+    makes compute_statistics fail with 'Error: NaNs!'; the state itself is
+    only read."""
+    cfg = small_cfg()
+    CF = _solver()
+    g = CF(cfg)
+    U, V, W = cases.initial_field(cfg)
+    for comp, idx in ((0, (3, 4, 5)), (1, (0, 0, 0)), (2, (-1, -1, -1))):
+        F = [U.copy(order="F"), V.copy(order="F"), W.copy(order="F")]
+        F[comp][idx] = np.nan
+        g.set_state(*F)
+        with pytest.raises(RuntimeError, match="NaNs"):
+            g.compute_statistics()
+    g.set_state(U, V, W)
+    g.compute_statistics()  # a finite state passes
 
 
 def test_smagorinsky_matches():

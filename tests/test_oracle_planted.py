@@ -32,6 +32,9 @@ PLANTED = {
     13: ("statistics: UV pairs V(i) with the U average one column right", [pins.check_statistics_uv_pairing]),
     14: ("rhs_v: nu_t at the x edges with the weights of row j-1", [pins.check_rhs_v_nu_t_on_edges]),
     15: ("statistics: V x range without the duplicated column", [pins.check_statistics_ranges]),
+    16: ("centreline row ny/2 instead of ny/2+1", [pins.check_centerline_mean, pins.check_mass_flow_corrections]),
+    17: ("centreline mean without the periodic image plane k = 1", [pins.check_centerline_mean]),
+    18: ("y mass-flow correction with the wrong sign", [pins.check_mass_flow_corrections]),
 }
 
 
