@@ -90,21 +90,36 @@ def run_precision(cfg, steps, warmup, cases):
     return g, ms, ms_ev, rep, div
 
 
-def cpu_baseline(cells, steps_cap_s=30.0):
-    """The oracle (FP64 C restatement of the reference step) on the host,
-    OpenMP over all cores, on a bounded sample: one RK3 step of the same
-    workload (plus one more if time allows)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The cores this process may run on (its affinity mask, which is the
+    host's CPU share on a GPU box), unless CF_CPU_THREADS says otherwise."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return int(os.environ.get("CF_CPU_THREADS", n))
+
+
+def _oracle_step_time(cells, nth, max_steps, cap_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
     from channel_flow import cases
     cfg = cases.case_dns384()
     cfg.cells = cells
-    nth = int(os.environ.get("CF_CPU_THREADS", min(os.cpu_count() or 1, 16)))
     o = Oracle(cfg, nthreads=nth)
     U, V, W = cases.initial_field(cfg)
     o.initialise(U, V, W)
     del U, V, W
-    log("cpu baseline: oracle initialised with %d threads" % nth)
     n = 0
     t0 = time.perf_counter()
     while True:
@@ -112,14 +127,32 @@ def cpu_baseline(cells, steps_cap_s=30.0):
         o.compute_time_step_RK3()
         n += 1
         el = time.perf_counter() - t0
-        log("cpu baseline: step %d done, %.1f s" % (n, el))
-        if el > steps_cap_s / 2 or n >= 3:
+        log("cpu baseline %s, %d threads: step %d done, %.1f s" % (cells, nth, n, el))
+        if el > cap_s / 2 or n >= max_steps:
             break
-    sps = el / n
-    ncell = cells[0] * cells[1] * cells[2]
     o.close()
-    return {"value": ncell / sps / 1e6, "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
-            "sample": "%d RK3 step(s) of the %dx%dx%d workload, %.2f s/step" % (n, *cells, sps)}
+    return el / n, n
+
+
+def cpu_baseline(cells, steps_cap_s=30.0):
+    """The oracle (FP64 C restatement of the reference step) on the host on a
+    bounded sample of the workload: RK3 steps of the full grid with OpenMP
+    over every core this process may use, and one step single-threaded on
+    the same x-y planes with 1/8 of the z extent (the per-cell cost of the
+    step does not depend on that extent, and a full-size single-threaded
+    step would take minutes)."""
+    nth = cpu_threads()
+    sps, n = _oracle_step_time(cells, nth, 2, steps_cap_s)
+    ncell = cells[0] * cells[1] * cells[2]
+    small = (cells[0], cells[1], max(cells[2] // 8, 8))
+    sps1, _ = _oracle_step_time(small, 1, 1, steps_cap_s)
+    ncell1 = small[0] * small[1] * small[2]
+    return {"value": round(ncell / sps / 1e6, 3), "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": "%d RK3 step(s) of the %dx%dx%d workload on %d threads, %.2f s/step" % (n, *cells, nth, sps),
+            "single_thread": {"value": round(ncell1 / sps1 / 1e6, 3), "unit": "Mcell-updates/s", "cores": 1,
+                              "sample": "1 RK3 step of %dx%dx%d (the workload's x-y planes, 1/8 of its z "
+                                        "extent), %.2f s/step" % (*small, sps1)}}
 
 
 def other_configs(steps=50, warmup=5):
@@ -296,7 +329,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the 64^3 and LES step timings")
-    ap.add_argument("--serve-scale", type=float, default=0.1,
+    ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="steps of the serving ensemble relative to 200-2000 per case")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)

@@ -266,6 +266,10 @@ int Ctx<S>::init() {
     for (int c = 0; c < 3; c++)
         if (A(&G[c], n3[c])) return -1;
     if (thomas_uses_scratch(d.mh * d.nzp) && A(&cscr, thomas_scratch_elems(d))) return -1;
+    if (A(&mfpart, rhs_rk_tiles(d, true))) return -1;
+    if (A(&dtfp, (size_t)ifftx_blocks(d, sizeof(S)) + 2 * (size_t)project_vw_blocks(d, sizeof(S)) +
+                     dt_fused_mid_doubles()))
+        return -1;
     if (A(&NT, nC) || A(&spec, nspec) || A(&phi1, nphi) || A(&phi23, nphi) ||
         A(&part, npart) || A(&dtscr, dt_scratch_doubles(d)) || A(&sc, 1) ||
         (prec_mixed() && A(&ubar, d.nyg + 2)) || A(&stats_out, stats_out_doubles(d)) || A(&stats_acc, 7 * d.ny + 8))
@@ -309,6 +313,7 @@ int Ctx<S>::init() {
         return -1;
     met.rdxf = m_rdxf; met.rdxc = m_rdxc; met.rdyf = m_rdyf; met.rdyc = m_rdyc;
     met.rdzf = m_rdzf; met.rdzc = m_rdzc; met.w0 = m_w0; met.w1 = m_w1; met.dyg = m_dyg;
+    met.dxc = m_dxc; met.dyc = m_dyc; met.dzc = m_dzc;
     wt.jjz = m_jjz; wt.wz = m_wz; wt.jjx = m_jjx; wt.wx = m_wx;
     // FFT plans and twiddles: x uses a length-M complex FFT plus the N-point split
     std::string err;
@@ -426,7 +431,7 @@ void Ctx<S>::ghosts(S *const *F, int mask) {
 }
 
 template <typename S>
-void Ctx<S>::poisson(S *const *F, S *phi) {
+void Ctx<S>::poisson(S *const *F, S *phi, const Scalars *corr, const DtParts *dtp) {
     const Dims &d = h.d;
     tic("div_fftx");
     launch_div_fftx<S, S>(d, met, F[0], F[1], F[2], spec, pM, twM, twN, stream);
@@ -441,10 +446,10 @@ void Ctx<S>::poisson(S *const *F, S *phi) {
     launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
     toc();
     tic("ifftx_proju");
-    launch_ifftx_proju<S, S, S>(d, met, spec, phi, F[0], pM, twM, twN, (double)(d.nxp * d.nzp), stream);
+    launch_ifftx_proju<S, S, S>(d, met, spec, phi, F[0], pM, twM, twN, (double)(d.nxp * d.nzp), stream, corr, dtp);
     toc();
     tic("project_vw");
-    launch_project_vw<S, S>(d, met, phi, F[1], F[2], stream);
+    launch_project_vw<S, S>(d, met, phi, F[1], F[2], stream, dtp ? dtp->pv : nullptr, dtp ? dtp->pw : nullptr);
     toc();
 }
 
@@ -502,38 +507,45 @@ void Ctx<S>::step_launch() {
     const Dims &d = h.d;
     const double rk[3][3] = {{8.0 / 15.0, 0, 0}, {1.0 / 4.0, 5.0 / 12.0, 0}, {1.0 / 4.0, 0.0, 3.0 / 4.0}};
     const bool nut = h.prm.les_model != 0 || nut_user;
+    // the x mass-flow correction (time_integration.f90:152-163) is taken from
+    // the third substep's U* as k_rhs_rk writes it (the projection leaves the
+    // bulk and centreline means unchanged), and added by that substep's U
+    // projection; the mixed path shifts its profile instead
+    const int mfm = (h.prm.x_mass_cte == 1 || h.prm.x_mass_cte == 2) ? h.prm.x_mass_cte : 0;
+    // compute_dt of the next step from the final velocities as the last
+    // projection writes them (the mixed path's U with its shifted profile)
+    const bool fz = fuse_dt();
+    const int ndu = ifftx_blocks(d, sizeof(S)), ndvw = project_vw_blocks(d, sizeof(S));
+    const DtParts dp{dtfp, dtfp + ndu, dtfp + ndu + ndvw, ubar, sc, (ubar && mfm) ? 1 : 0};
     for (int s = 1; s <= 3; s++) {
         S *const *Fc = st[cur];
         S *const *Fn = st[1 - cur];
         eddy_viscosity(Fc);
         if (s == 1) wall_model(Fc);
+        double *mfp = (s == 3 && mfm) ? mfpart : nullptr;
         tic("rhs_rk");
         launch_rhs_rk<S, S>(d, met, Fc[0], Fc[1], Fc[2], nut ? NT : nullptr, Fn[0], Fn[1], Fn[2], G[0], G[1],
                             G[2], sc, ubar, h.prm.nu, h.prm.omega_z, rk[s - 1][s - 1], rk[1][0],
-                            s == 1 ? 1 : 2, stream);
+                            s == 1 ? 1 : 2, stream, mfp, mfm);
         toc();
+        if (mfp) {
+            tic("massflow");
+            launch_massflow_fused_final(d, mfpart, rhs_rk_tiles(d, nut), mfm, ubar, sc, stream);
+            toc();
+        }
         // the divergence reads the x and z ghost faces at their periodic
         // images, so the boundary conditions are applied once, after the
         // projection (time_integration.f90 applies them before and after)
-        poisson(Fn, s == 1 ? phi1 : phi23);
+        poisson(Fn, s == 1 ? phi1 : phi23, (mfp && !ubar) ? sc : nullptr, (s == 3 && fz) ? &dp : nullptr);
         ghosts(Fn, 7);
         cur = 1 - cur;
     }
-    // mass-flow corrections after the third substep (time_integration.f90:152-171)
     S *const *F = st[cur];
-    if (h.prm.x_mass_cte == 1 || h.prm.x_mass_cte == 2) {
+    if (mfm && ubar) {
+        // deviation storage: the uniform correction shifts the profile
         tic("massflow");
-        if (h.prm.x_mass_cte == 1) launch_massflow<S>(d, met, F[0], part, sc, 1, stream);
-        else launch_centerline_u<S>(d, F[0], ubar, part, sc, 1, stream);
-        if (ubar) {
-            // deviation storage: the uniform correction shifts the profile
-            launch_ubar_shift(sc, ubar, d.nyg, h.prm.iwall_model == 3 ? 1 : 0, stream);
-            toc();
-        } else {
-            launch_add_corr_u<S>(d, F[0], sc, stream);
-            toc();
-            ghosts(F, 1);
-        }
+        launch_ubar_shift(sc, ubar, d.nyg, h.prm.iwall_model == 3 ? 1 : 0, stream);
+        toc();
     }
     if (h.prm.y_mass_cte == 1) {
         tic("massflow_v");
@@ -545,19 +557,35 @@ void Ctx<S>::step_launch() {
     tic("advance_time");
     launch_advance_time(sc, stream);
     toc();
+    if (fz) {
+        tic("dt_fused");
+        launch_dt_fused_final<S>(d, dp, ndu, ndvw, m_dyc, st[cur][1], sc, h.prm.cfl, h.dt_vis, stream);
+        toc();
+    }
+}
+
+// main.f90:70-73: compute_dt, then the step.  With the fused dt the step
+// ends by computing the next step's dt into dt_next, and a step starts by
+// taking it; a standalone compute_dt runs first whenever the state changed
+// since the last step (dt_fresh false).
+template <typename S>
+void Ctx<S>::step_head() {
+    if (fuse_dt()) launch_dt_take_next(sc, stream);
+    else dt_launch();
 }
 
 template <typename S>
 int Ctx<S>::run_steps(int nsteps, bool use_graph) {
     HCK(hipSetDevice(dev));
     for (int n = 0; n < nsteps; n++) {
+        if (fuse_dt() && !dt_fresh) dt_launch();
         if (use_graph && !timing) {
             hipGraphExec_t &ge = graphs[cur];
             const int start = cur;
             if (!ge) {
                 hipGraph_t g;
                 HCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                dt_launch();
+                step_head();
                 step_launch();
                 HCK(hipStreamEndCapture(stream, &g));
                 HCK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -567,9 +595,10 @@ int Ctx<S>::run_steps(int nsteps, bool use_graph) {
             HCK(hipGraphLaunch(ge, stream));
             cur = 1 - start;  // three substeps flip the ping-pong parity
         } else {
-            dt_launch();
+            step_head();
             step_launch();
         }
+        dt_fresh = fuse_dt();
     }
     HCK(hipGetLastError());
     return 0;
@@ -585,7 +614,7 @@ int Ctx<S>::prepare_graphs() {
         cur = p;
         hipGraph_t g;
         HCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        dt_launch();
+        step_head();
         step_launch();
         HCK(hipStreamEndCapture(stream, &g));
         HCK(hipGraphInstantiate(&graphs[p], g, nullptr, nullptr, 0));
@@ -595,11 +624,14 @@ int Ctx<S>::prepare_graphs() {
     return 0;
 }
 
+// compute_time_step_RK3 with the dt the last compute_dt set, as the
+// reference's step uses the module's dt
 template <typename S>
 int Ctx<S>::rk3_step() {
     HCK(hipSetDevice(dev));
     step_launch();
     HCK(hipGetLastError());
+    dt_fresh = fuse_dt();
     return 0;
 }
 
@@ -607,6 +639,7 @@ template <typename S>
 int Ctx<S>::compute_dt(double *dt) {
     HCK(hipSetDevice(dev));
     dt_launch();
+    dt_fresh = true;
     HCK(hipGetLastError());
     Scalars s;
     HCK(hipMemcpyAsync(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
@@ -618,6 +651,7 @@ int Ctx<S>::compute_dt(double *dt) {
 template <typename S>
 int Ctx<S>::projection_step(int rk_step) {
     HCK(hipSetDevice(dev));
+    dt_fresh = false;
     poisson(st[cur], rk_step == 1 ? phi1 : phi23);
     HCK(hipGetLastError());
     HCK(hipStreamSynchronize(stream));
@@ -627,6 +661,7 @@ int Ctx<S>::projection_step(int rk_step) {
 template <typename S>
 int Ctx<S>::apply_bc() {
     HCK(hipSetDevice(dev));
+    dt_fresh = false;
     ghosts(st[cur], 7);
     HCK(hipGetLastError());
     HCK(hipStreamSynchronize(stream));
@@ -698,6 +733,7 @@ int Ctx<S>::sync_profile() {
 template <typename S>
 int Ctx<S>::set_state(const double *U, const double *V, const double *W) {
     HCK(hipSetDevice(dev));
+    dt_fresh = false;
     HCK(hipStreamSynchronize(stream));
     if (ubar) {
         if (set_profile(U)) return -1;
@@ -934,6 +970,7 @@ int Ctx<S>::set_tau_init(const double *ti) {
 template <typename S>
 int Ctx<S>::set_scalars(const double *v) {
     HCK(hipSetDevice(dev));
+    dt_fresh = false;
     HCK(hipStreamSynchronize(stream));
     Scalars s;
     HCK(hipMemcpy(&s, sc, sizeof(Scalars), hipMemcpyDeviceToHost));

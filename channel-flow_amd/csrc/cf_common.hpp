@@ -31,6 +31,7 @@ struct Metrics {
     const double *rdzc;   // 1/(zg(k+1)-zg(k))    k = 1..nzg-1
     const double *w0, *w1;  // y-face weights (initialization.f90:387-388)
     const double *dyg;    // yg(j)-yg(j-1)        j = 2..nyg
+    const double *dxc, *dyc, *dzc;  // xg(i+1)-xg(i) etc. i = 1..nxg-1 (compute_dt's spacings)
 };
 
 // Device-resident scalars shared by all kernels of a step (no host sync).
@@ -48,12 +49,25 @@ struct Scalars {
     double dt_conv[3];
     double qbar_u;    // bulk-flow contribution of the stored profile ubar (mixed path)
     double corr_v;    // last y mass-flow correction (V increment, mass_flow.f90:174)
+    double dt_next;   // compute_dt of the state at the end of the last step
     double ucl;       // centreline target Ucenterline_x (x_mass_cte = 2)
     double pad[3];    // [0] last bulk (or centreline) mean of U, [1] last bulk mean of V
     int n_nan_eqwm;
     int n_iter_eqwm;
     int n_fallback_mlp;
     int pad_i;
+};
+
+// Partial maxima of compute_dt's spacing/|u| (time_integration.f90:191-196)
+// taken by the kernels that write the final velocities of a step: one value
+// per workgroup (null: not taken).  U's values are those of the stored U plus
+// ubar(j) (mixed path) plus corr when add_corr (the profile shift that
+// follows the step).
+struct DtParts {
+    double *pu, *pv, *pw;
+    const double *ubar;
+    const Scalars *sc;
+    int add_corr;
 };
 
 __host__ __device__ inline int64_t i3(int n1, int n2, int i, int j, int k) {

@@ -336,19 +336,21 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 // imaginary part of the right-hand side share the matrix, so each lane runs
 // the forward sweep and back substitution of projection.f90:90-113 for one
 // of them (FP64 arithmetic), recomputing the same pivots.  Rows go in blocks
-// of LB: the next block's values are loaded while the current block is
-// eliminated, d' is stored in the spectrum, and c' is kept only at the last
-// row of each block (a checkpoint, one per lane).  The back substitution
-// recomputes a block's c' from the previous block's checkpoint -- the same
-// operations on the same values, so the same c' -- and walks the block
-// backwards.  Two passes over the spectrum and 2/LB of a pass of FP64
-// scratch, no serial phase: the faster form whenever the modes fill the GPU
-// (the sweep is latency bound, and two lanes per mode double the waves in
-// flight).
+// of LB, the next block's values loaded while the current block is
+// eliminated.  The forward sweep writes nothing per row: at the last row of
+// each block it stores c' and d' (FP64 checkpoints, one pair per lane).  The
+// back substitution walks the blocks from the top; for each it reads the
+// block's right-hand side again, recomputes c' and d' from the previous
+// block's checkpoint -- the same operations on the same values, so the same
+// c' and d' -- and walks the block backwards writing the solution in place.
+// Three passes over the spectrum (read, read, write) and 4/LB of a pass of
+// FP64 scratch, no serial phase: the faster form whenever the modes fill the
+// GPU (the sweep is latency bound, and two lanes per mode double the waves
+// in flight).
 // ---------------------------------------------------------------------------
 constexpr int TH_LB = 16;
 size_t thomas_scratch_elems(const Dims &d) {
-    return (size_t)2 * ((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp;
+    return (size_t)4 * ((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp;
 }
 
 // pivot reciprocal 1 / (D(j) - dl(j-1) c'(j-1)); c'(j) = du(j) times it
@@ -375,6 +377,7 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
     T *col = reinterpret_cast<T *>(spec) + t;        // this lane's part of mode q, row 0
     const int nb = (nyp + TH_LB - 1) / TH_LB;
     const int64_t cs = 2 * (int64_t)nmodes;          // checkpoint stride per block
+    double *ck_c = ckpt, *ck_d = ckpt + (int64_t)nb * cs;
     auto Dj = [&](int j) {
         const double D = (s_dd[j] + kx2) + kz2;
         return (j == 0 && zm) ? 1.5 * D : D;
@@ -387,7 +390,7 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
             dst[u] = j < nyp ? col[j * js] : (T)0;
         }
     };
-    // forward sweep
+    // forward sweep: checkpoints only
     double cp = 0.0, dp = 0.0;
     loadblk(0, cur);
     for (int b = 0; b < nb; b++) {
@@ -399,33 +402,37 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
                 const double lo = j > 0 ? s_dl[j - 1] : 0.0;
                 const double r = thomas_pivot(Dj(j), lo, cp);
                 dp = ((double)cur[u] - lo * dp) * r;
-                col[j * js] = (T)dp;
                 cp = (j < nyp - 1) ? s_du[j] * r : 0.0;
             }
         }
-        ckpt[(int64_t)b * cs + t] = cp;
+        ck_c[(int64_t)b * cs + t] = cp;
+        ck_d[(int64_t)b * cs + t] = dp;
 #pragma unroll
         for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
     }
     // back substitution, block by block from the top
-    double x = dp;  // row nyp-1
+    double x = dp;  // row nyp-1: x = d'
     loadblk(nb - 1, cur);
-    double cnext = nb >= 2 ? ckpt[(int64_t)(nb - 2) * cs + t] : 0.0;
+    double c0n = nb >= 2 ? ck_c[(int64_t)(nb - 2) * cs + t] : 0.0;
+    double d0n = nb >= 2 ? ck_d[(int64_t)(nb - 2) * cs + t] : 0.0;
     for (int b = nb - 1; b >= 0; b--) {
-        const double cstart = cnext;  // c' at the row before the block
+        double cprev = c0n, dprev = d0n;  // c', d' at the row before the block
         if (b > 0) {
             loadblk(b - 1, nxt);
-            cnext = b >= 2 ? ckpt[(int64_t)(b - 2) * cs + t] : 0.0;
+            c0n = b >= 2 ? ck_c[(int64_t)(b - 2) * cs + t] : 0.0;
+            d0n = b >= 2 ? ck_d[(int64_t)(b - 2) * cs + t] : 0.0;
         }
-        double c[TH_LB];
-        double cprev = cstart;
+        double c[TH_LB], dv[TH_LB];
 #pragma unroll
         for (int u = 0; u < TH_LB; u++) {
             const int j = b * TH_LB + u;
             c[u] = 0.0;
+            dv[u] = 0.0;
             if (j < nyp) {
                 const double lo = j > 0 ? s_dl[j - 1] : 0.0;
                 const double r = thomas_pivot(Dj(j), lo, cprev);
+                dprev = ((double)cur[u] - lo * dprev) * r;
+                dv[u] = dprev;
                 c[u] = (j < nyp - 1) ? s_du[j] * r : 0.0;
                 cprev = c[u];
             }
@@ -433,10 +440,8 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
 #pragma unroll
         for (int u = TH_LB - 1; u >= 0; u--) {
             const int j = b * TH_LB + u;
-            if (j < nyp - 1) {
-                x = (double)cur[u] - c[u] * x;
-                col[j * js] = (T)x;
-            }
+            if (j < nyp - 1) x = dv[u] - c[u] * x;
+            if (j < nyp) col[j * js] = (T)x;
         }
 #pragma unroll
         for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
@@ -616,7 +621,7 @@ template <typename S, typename T, typename P, int WG, int MC>
 __global__ void __launch_bounds__(WG)
 k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__ phi,
               S *__restrict__ U, FFTPlan pM, const Cx<T> *__restrict__ twM,
-              const Cx<T> *__restrict__ twN, int rb, double norm) {
+              const Cx<T> *__restrict__ twN, int rb, double norm, const Scalars *__restrict__ csc, DtParts dtp) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
@@ -688,11 +693,24 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     const T *zr = reinterpret_cast<const T *>(lds);
     const int nx = d.nx, nyg = d.nyg;
     const double rnorm = 1.0 / norm;
+    const double corr = csc ? csc->corr : 0.0;
+    // compute_dt partial: element e = tid + c WG of the loop below sits at
+    // x index (tid + c WG) mod N, so a thread meets NS = N / gcd(N, WG)
+    // distinct x indices, cycling with c (NS = 3 for N = 2^a 3, 1 for N =
+    // 2^a): it keeps the smallest |u| per index and divides once per index at
+    // the end (correctly rounded division is monotone); NS > 3 divides per value
+    int NS = N, g2 = WG;
+    while (g2) { const int t2 = NS % g2; NS = g2; g2 = t2; }
+    NS = N / NS;
+    double umin0 = INFINITY, umin1 = INFINITY, umin2 = INFINITY, lu = 0.0;
+    int slot = 0;
     StepDiv el(threadIdx.x, WG, N);  // (row lr = q, ip = r)
 #pragma unroll 4
     for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
         const int lr = el.q, ip = el.r;
         const int r = row0 + lr;
+        const int sl = slot;
+        slot = slot + 1 == NS ? 0 : slot + 1;
         if (r >= nrows) continue;
         int jp, kp;
         small_divmod(r, d.nzp, inz, jp, kp);
@@ -703,7 +721,35 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         if (U) {
             const int i = 2 + ip, j = 2 + jp, k = 2 + kp;
             const int64_t qu = i3(nx, nyg, i, j, k);
-            U[qu] = (S)((double)U[qu] - (phn - ph) * m.rdxc[i]);
+            const double up = (double)U[qu] - (phn - ph) * m.rdxc[i];
+            const S us = (S)(csc ? up + corr : up);
+            U[qu] = us;
+            if (dtp.pu) {
+                double uf = (double)us;
+                if (dtp.ubar) uf = uf + (dtp.add_corr ? dtp.ubar[j] + dtp.sc->corr : dtp.ubar[j]);
+                const double au = fabs(uf);
+                if (NS > 3) lu = fmax(lu, m.dxc[i] / au);
+                else if (sl == 0) umin0 = fmin(umin0, au);
+                else if (sl == 1) umin1 = fmin(umin1, au);
+                else umin2 = fmin(umin2, au);
+            }
+        }
+    }
+    if (dtp.pu) {  // one partial maximum per workgroup
+        if (NS <= 3) {
+            const int t = threadIdx.x;
+            if (umin0 < INFINITY) lu = fmax(lu, m.dxc[2 + (int)(t % N)] / umin0);
+            if (NS > 1 && umin1 < INFINITY) lu = fmax(lu, m.dxc[2 + (int)((t + WG) % N)] / umin1);
+            if (NS > 2 && umin2 < INFINITY) lu = fmax(lu, m.dxc[2 + (int)((t + 2 * WG) % N)] / umin2);
+        }
+        __shared__ double sh[WG / 64];
+        for (int o = 32; o > 0; o >>= 1) lu = fmax(lu, __shfl_down(lu, o, 64));
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = lu;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int q = 0; q < WG / 64; q++) r = fmax(r, sh[q]);
+            dtp.pu[blockIdx.x] = r;
         }
     }
 }
@@ -860,29 +906,36 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
                            P);
 }
 
+int ifftx_blocks(const Dims &d, int esize) {
+    const int M = d.nxp / 2;
+    const int rb = fft_rows_per_wg(M, M + 1, 2 * esize);
+    return (d.nyp * d.nzp + rb - 1) / rb;
+}
+
 template <typename S, typename T, typename P>
 void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
                         const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,
-                        hipStream_t st) {
+                        hipStream_t st, const Scalars *corr, const DtParts *dtpp) {
+    const DtParts dtp = dtpp ? *dtpp : DtParts{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>);
     const dim3 grid((nrows + rb - 1) / rb);
     if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512, 0>), grid, dim3(512), shm, st, d, m, spec, phi, U, pM, twM,
-                           twN, rb, norm);
+                           twN, rb, norm, corr, dtp);
         return;
     }
 #define CF_IFFTX(MC)                                                                                          \
     case MC:                                                                                                  \
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, MC>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM, \
-                           twM, twN, rb, norm);                                                               \
+                           twM, twN, rb, norm, corr, dtp);                                                    \
         return;
     switch (fft_static_sizes() ? pM.n : 0) {
         CF_X_LENGTHS(CF_IFFTX)
         default:
             hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, 0>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM,
-                               twM, twN, rb, norm);
+                               twM, twN, rb, norm, corr, dtp);
     }
 #undef CF_IFFTX
 }
@@ -908,14 +961,15 @@ template void launch_thomas<float>(const Dims &, Cx<float> *, double *, const do
 template void launch_ifftx_proju<double, double, double>(const Dims &, const Metrics &,
                                                          const Cx<double> *, double *, double *,
                                                          const FFTPlan &, const Cx<double> *,
-                                                         const Cx<double> *, double, hipStream_t);
+                                                         const Cx<double> *, double, hipStream_t,
+                                                         const Scalars *, const DtParts *);
 template void launch_ifftx_proju<float, float, float>(const Dims &, const Metrics &, const Cx<float> *,
                                                       float *, float *, const FFTPlan &,
                                                       const Cx<float> *, const Cx<float> *, double,
-                                                      hipStream_t);
+                                                      hipStream_t, const Scalars *, const DtParts *);
 template void launch_ifftx_proju<float, float, double>(const Dims &, const Metrics &, const Cx<float> *,
                                                        double *, float *, const FFTPlan &,
                                                        const Cx<float> *, const Cx<float> *, double,
-                                                       hipStream_t);
+                                                       hipStream_t, const Scalars *, const DtParts *);
 
 }  // namespace cf

@@ -124,6 +124,14 @@ struct Ctx : CtxBase {
     DsmWork dsm{};           // dynamic Smagorinsky work arrays (les_model 2 only)
     S *phi1 = nullptr, *phi23 = nullptr;
     double *part = nullptr, *stats_out = nullptr;
+    double *mfpart = nullptr;  // per-workgroup mass-flow partials of the last substep
+    double *dtfp = nullptr;    // per-workgroup compute_dt maxima of the last projection
+    bool dt_fresh = false;     // dt_next holds compute_dt of the current state
+    // dt taken from the last projection: adaptive dt (a fixed dt needs no
+    // reduction) and V untouched after that projection (no y mass-flow
+    // correction)
+    bool fuse_dt() const { return h.prm.cfl >= 0 && h.prm.y_mass_cte == 0; }
+    void step_head();
     double *dtscr = nullptr;
     double *ubar = nullptr;            // mixed path: FP64 profile U = ubar(j) + U'
     std::vector<double> ubar_h;
@@ -160,7 +168,7 @@ struct Ctx : CtxBase {
             if (g) { hipGraphExecDestroy(g); g = nullptr; }
     }
     void ghosts(S *const *F, int mask);
-    void poisson(S *const *F, S *phi);
+    void poisson(S *const *F, S *phi, const Scalars *corr = nullptr, const DtParts *dtp = nullptr);
     void eddy_viscosity(S *const *F);
     void wall_model(S *const *F);
     void dt_launch();

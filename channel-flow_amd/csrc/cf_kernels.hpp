@@ -27,11 +27,17 @@ void launch_dsm(const Dims &d, const Metrics &m, const S *U, const S *V, const S
                 const DsmWork &w, S *NT, int dirichlet, hipStream_t st);
 
 // ---- stencils (cf_stencil.hip) ----
+// mfpart (substeps 2 and 3 only; null: none): one partial per workgroup of
+// the bulk (mf_mode 1) or centreline (mf_mode 2) sum of the new U, for
+// launch_massflow_fused_final; rhs_rk_tiles(d, nut) workgroups
 template <typename S, typename R>
 void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                    const S *NT, S *Uo, S *Vo, S *Wo, S *Gu, S *Gv, S *Gw,
                    const Scalars *sc, const double *ubar, double nu, double om, double a_new,
-                   double a_g, int sub, hipStream_t st);
+                   double a_g, int sub, hipStream_t st, double *mfpart = nullptr, int mf_mode = 0);
+int rhs_rk_tiles(const Dims &d, bool nut);
+void launch_massflow_fused_final(const Dims &d, const double *mfpart, int nparts, int mf_mode, const double *ubar,
+                                 Scalars *sc, hipStream_t st);
 template <typename S, typename R>
 void launch_rhs_only(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                      const S *NT, double *Ru, double *Rv, double *Rw, const Scalars *sc,
@@ -54,8 +60,19 @@ void launch_ghosts(const Dims &d, S *F, int field, int wall_mode, const double *
 template <typename S>
 void launch_ghosts3(const Dims &d, S *const *F, const int *field, const int *wall_mode,
                     const double *const *alpha, int n, double dyg_bot, double dyg_top, hipStream_t st);
+// dtv, dtw (optional): per-workgroup maxima of compute_dt's spacing/|v| and
+// spacing/|w| over the projected values, project_vw_blocks(d, sizeof(S))
 template <typename S, typename P>
-void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st);
+void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st,
+                       double *dtv = nullptr, double *dtw = nullptr);
+int project_vw_blocks(const Dims &d, int esize);
+int ifftx_blocks(const Dims &d, int esize);
+// dt_next from the partial maxima of a step's last projection (DtParts)
+template <typename S>
+void launch_dt_fused_final(const Dims &d, const DtParts &p, int nu_, int nvw, const double *dyc, const S *V,
+                           Scalars *sc, double cfl, double dt_vis, hipStream_t st);
+void launch_dt_take_next(Scalars *sc, hipStream_t st);
+size_t dt_fused_mid_doubles();  // room launch_dt_fused_final needs after the partial arrays
 template <typename S>
 void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, Scalars *sc,
                      int apply, hipStream_t st);
@@ -120,11 +137,14 @@ size_t thomas_scratch_elems(const Dims &d);  // FP64 checkpoints of the lane-for
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st);
-// U null: phi only, no projection of U
+// U null: phi only, no projection of U; corr (optional): the U projection
+// also adds corr->corr (the mass-flow correction, time_integration.f90:155)
+// dtp (optional, dtp.pu non-null): per-workgroup maxima of compute_dt's
+// spacing/|u| over the final U, ifftx_blocks(d, sizeof(T)) of them
 template <typename S, typename T, typename P>
 void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
                         const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,
-                        hipStream_t st);
+                        hipStream_t st, const Scalars *corr = nullptr, const DtParts *dtp = nullptr);
 
 // ---- wall model (cf_eqwm.hip) ----
 struct EqwmParams {

@@ -215,14 +215,18 @@ struct LAcc {
 // The ring has four slots so one barrier per plane suffices: during plane k
 // the workgroup reads slots (k-1, k, k+1) and writes plane k+2 (prefetched
 // into registers one iteration earlier) into the slot of plane k-2.
+// FP64 without nu_t and rotation: registers held to the four waves per
+// SIMD that the LDS ring allows anyway (two 512-thread workgroups per CU);
+// the mass-flow sum of the last substep would otherwise cost a wave
 template <typename S, typename R, bool NUT, bool ROT, int SUB, int TY>
 __global__ void __launch_bounds__(RK_TX *TY)
+    __attribute__((amdgpu_waves_per_eu((sizeof(R) == 8 && !NUT && !ROT) ? 4 : 1)))
 k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
          const S *__restrict__ W, const S *__restrict__ NT, S *__restrict__ Uo,
          S *__restrict__ Vo, S *__restrict__ Wo, S *__restrict__ Gu, S *__restrict__ Gv,
          S *__restrict__ Gw, const Scalars *__restrict__ sc, const double *__restrict__ ubar,
          double nu_d, double om_d, double a_new, double a_g, int tiles_x, int tiles_y,
-         int nchunks, int ntiles) {
+         int nchunks, int ntiles, double *__restrict__ mfpart, int mf_mode) {
     constexpr int LX = RK_TX + 2, PL = LX * (TY + 2), NT_ = RK_TX * TY;
     constexpr int NPF = (PL + NT_ - 1) / NT_;  // plane elements per thread
     extern __shared__ unsigned char smem_raw[];
@@ -316,6 +320,17 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         }
     };
     gload(kbeg, gu, gv, gw);
+    // SUB 3 (the last substep): U* feeds the mass-flow correction.  The
+    // projection that follows subtracts a periodic x difference of phi,
+    // whose sum over the unique x faces vanishes, so the bulk (mf_mode 1)
+    // and centreline (mf_mode 2) means of U* are those of the projected U.
+    // Each thread sums its column over the k march (plane 2 counted twice
+    // for its periodic image nzg-1 in the bulk, plane nzg-2 twice for its
+    // image k = 1 on the centreline) and weights it once by its row: the
+    // trapezoid weight over rows 2..nyg-1 (mass_flow.f90:116-118), or 1 on
+    // row ny/2+1 (mass_flow.f90:205).
+    double mf_acc = 0.0;
+    const int mf_kdup = mf_mode == 1 ? 2 : nzg - 2;
     for (int k = kbeg; k < kend; k++) {
         const int r0 = k - kbeg;
         R gu_n = 0, gv_n = 0, gw_n = 0;
@@ -340,9 +355,11 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
                 Wo[qw] = (S)(cw + cnew * rw);
                 Gw[qw] = (S)(cw + cg * rw);
             } else {
-                Uo[qu] = (S)(gu + cnew * ru);
+                const S un = (S)(gu + cnew * ru);
+                Uo[qu] = un;
                 if (j <= ny - 1) Vo[qv] = (S)(gv + cnew * rv);
                 Wo[qw] = (S)(gw + cnew * rw);
+                if (SUB == 3) mf_acc += (k == mf_kdup) ? 2.0 * (double)un : (double)un;
             }
         }
         gu = gu_n; gv = gv_n; gw = gw_n;
@@ -352,6 +369,24 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
             if (k + 3 <= kend) fetch(k + 3);
         }
         __syncthreads();
+    }
+    if (SUB == 3) {  // one partial per workgroup, wave sums in a fixed order
+        __shared__ double mf_sh[16];
+        double mf_w = 0.0;
+        if (active) {
+            if (mf_mode == 1) mf_w = 0.5 * ((j > 2 ? m.dyg[j] : 0.0) + (j < nyg - 1 ? m.dyg[j + 1] : 0.0));
+            else mf_w = j == ny / 2 + 1 ? 1.0 : 0.0;
+        }
+        double v = mf_w * mf_acc;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        const int wv = tid >> 6;
+        if ((tid & 63) == 0) mf_sh[wv] = v;
+        __syncthreads();
+        if (tid == 0) {
+            double r = 0.0;
+            for (int q = 0; q < (RK_TX * TY) / 64; q++) r += mf_sh[q];
+            mfpart[blockIdx.x] = r;
+        }
     }
 }
 
@@ -513,39 +548,6 @@ __global__ void k_ghosts(Dims d, GhostSet gs, double dyg_bot, double dyg_top) {
 // Each thread marches over a chunk of kc planes at fixed (i, j), carrying
 // phi(k) from one plane to the next, so phi streams through once; the
 // phi(j+1) neighbour is the row the next thread row loads (cache hit).
-template <typename S, typename P>
-__global__ void __launch_bounds__(256)
-k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
-             S *__restrict__ W, int tiles_x, int tiles_y, int ntiles, int kc) {
-    const int t = xcd_remap(blockIdx.x, ntiles);
-    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tc = t / (tiles_x * tiles_y);
-    const int i = 2 + tx * 64 + (threadIdx.x & 63);
-    const int j = 2 + ty * 4 + (threadIdx.x >> 6);
-    const int kb = 2 + tc * kc, ke = min(kb + kc, d.nzg - 1);  // planes kb..ke-1
-    if (i > d.nx - 1 || j > d.nyg - 1) return;
-    const int nxp = d.nxp, nyp = d.nyp;
-    const int64_t pps = (int64_t)nyp * nxp;  // phi plane stride
-    const P *pcol = phi + (int64_t)(j - 2) * nxp + (i - 2);
-    const bool dov = j <= d.ny - 1;
-    const double ry = m.rdyc[j];
-    const int64_t vs = (int64_t)d.nxg * d.ny, ws = (int64_t)d.nxg * d.nyg;
-    S *vp = V + i3(d.nxg, d.ny, i, j, kb);
-    S *wp = W + i3(d.nxg, d.nyg, i, j, kb);
-    double pc = (double)pcol[(int64_t)(kb - 2) * pps];
-    for (int k = kb; k < ke; k++) {
-        const int kn = (k + 1 == d.nzg - 1) ? 2 : k + 1;  // periodic image of phi(:,:,nz)
-        const double pn = (double)pcol[(int64_t)(kn - 2) * pps];
-        if (dov) {
-            const double pj = (double)pcol[(int64_t)(k - 2) * pps + nxp];
-            *vp = (S)((double)*vp - (pj - pc) * ry);
-        }
-        *wp = (S)((double)*wp - (pn - pc) * m.rdzc[k]);
-        pc = pn;
-        vp += vs;
-        wp += ws;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Block reductions (wavefront 64, LDS across wavefronts), fixed order so that
 // results are reproducible run to run.
@@ -573,6 +575,93 @@ __device__ T block_reduce(T v, Op op, T *sh) {
 struct OpAdd { __device__ double operator()(double a, double b) const { return a + b; } };
 struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
+
+template <typename S, typename P, int KM>
+__global__ void __launch_bounds__(256)
+k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
+             S *__restrict__ W, int tiles_x, int tiles_y, int ntiles, int kc, double *__restrict__ dtv,
+             double *__restrict__ dtw) {
+    const int t = xcd_remap(blockIdx.x, ntiles);
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tc = t / (tiles_x * tiles_y);
+    const int i = 2 + tx * 64 + (threadIdx.x & 63);
+    const int j = 2 + ty * 4 + (threadIdx.x >> 6);
+    const int kb = 2 + tc * kc, ke = min(kb + kc, d.nzg - 1);  // planes kb..ke-1
+    const bool act = i <= d.nx - 1 && j <= d.nyg - 1;
+    // compute_dt partials: min |v| of this thread's column (one row j), min
+    // |w| per plane for the first KM planes; planes beyond KM divide per value
+    double vmin = INFINITY, wmin[KM];
+#pragma unroll
+    for (int q = 0; q < KM; q++) wmin[q] = INFINITY;
+    double lw_direct = 0.0;
+    if (act) {
+        const int nxp = d.nxp, nyp = d.nyp;
+        const int64_t pps = (int64_t)nyp * nxp;  // phi plane stride
+        const P *pcol = phi + (int64_t)(j - 2) * nxp + (i - 2);
+        const bool dov = j <= d.ny - 1;
+        const double ry = m.rdyc[j];
+        const int64_t vs = (int64_t)d.nxg * d.ny, ws = (int64_t)d.nxg * d.nyg;
+        S *vp = V + i3(d.nxg, d.ny, i, j, kb);
+        S *wp = W + i3(d.nxg, d.nyg, i, j, kb);
+        double pc = (double)pcol[(int64_t)(kb - 2) * pps];
+        for (int k = kb; k < ke; k++) {
+            const int kn = (k + 1 == d.nzg - 1) ? 2 : k + 1;  // periodic image of phi(:,:,nz)
+            const double pn = (double)pcol[(int64_t)(kn - 2) * pps];
+            if (dov) {
+                const double pj = (double)pcol[(int64_t)(k - 2) * pps + nxp];
+                const S vn = (S)((double)*vp - (pj - pc) * ry);
+                *vp = vn;
+                vmin = fmin(vmin, fabs((double)vn));
+            }
+            const S wn = (S)((double)*wp - (pn - pc) * m.rdzc[k]);
+            *wp = wn;
+            if (dtw) {
+                const int q = k - kb;
+                if (q < KM) {
+#pragma unroll
+                    for (int u = 0; u < KM; u++)
+                        if (u == q) wmin[u] = fmin(wmin[u], fabs((double)wn));
+                } else {
+                    lw_direct = fmax(lw_direct, m.dzc[k] / fabs((double)wn));
+                }
+            }
+            pc = pn;
+            vp += vs;
+            wp += ws;
+        }
+    }
+    if (dtv) {
+        // spacing/|u| is largest at the smallest |u| of equal spacing
+        // (correctly rounded division is monotone): minima over each wave
+        // (one row j per wave) and over the four waves (the same planes),
+        // then one division per row and per plane, by thread 0
+        __shared__ double shv[4], shd[4], shw[KM][4];
+        for (int o = 32; o > 0; o >>= 1) {
+            vmin = fmin(vmin, __shfl_down(vmin, o, 64));
+            lw_direct = fmax(lw_direct, __shfl_down(lw_direct, o, 64));
+#pragma unroll
+            for (int u = 0; u < KM; u++) wmin[u] = fmin(wmin[u], __shfl_down(wmin[u], o, 64));
+        }
+        const int wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            shv[wv] = (j <= d.ny - 1 && vmin < INFINITY) ? m.dyc[j] / vmin : 0.0;
+            shd[wv] = lw_direct;
+#pragma unroll
+            for (int u = 0; u < KM; u++) shw[u][wv] = wmin[u];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double lv = fmax(fmax(shv[0], shv[1]), fmax(shv[2], shv[3]));
+            double lw = fmax(fmax(shd[0], shd[1]), fmax(shd[2], shd[3]));
+            for (int u = 0; u < KM && kb + u < ke; u++) {
+                const double mn = fmin(fmin(shw[u][0], shw[u][1]), fmin(shw[u][2], shw[u][3]));
+                if (mn < INFINITY) lw = fmax(lw, m.dzc[kb + u] / mn);
+            }
+            dtv[blockIdx.x] = lv;
+            dtw[blockIdx.x] = lw;
+        }
+    }
+}
+
 
 // workgroup tile of the dt reduction below: 64 x 8 (x, y) by 16 z planes
 constexpr int MF_TX = 64, MF_TY = 8, MF_KC = 16;
@@ -851,9 +940,86 @@ __global__ void k_dt_final(Dims d, const double *__restrict__ dxc, const double 
         dt = fmin(dt, 5e-3);
         if (cfl < 0) dt = -cfl;
         sc->dt = dt;
+        sc->dt_next = dt;
         sc->dt_conv[0] = lu; sc->dt_conv[1] = lv; sc->dt_conv[2] = lw;
     }
 }
+
+// compute_dt of the state a step leaves, from the partial maxima its last
+// projection kernels took (DtParts) and V's wall row j = ny, which those
+// kernels do not write (the reference's range j = 2..nyg-1 includes it).
+// Maxima of correctly rounded divisions: the same dt as k_dt_final.
+constexpr int DTF_NB = 128;  // workgroups of the first level
+__global__ void __launch_bounds__(256)
+k_dt_fused_mid(Dims d, const double *__restrict__ pu, int nu_, const double *__restrict__ pv,
+               const double *__restrict__ pw, int nvw, const double *__restrict__ dyc, const void *__restrict__ Vp,
+               int vfp32, double *__restrict__ mid) {
+    __shared__ double sh[8];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    double lu = 0.0, lv = 0.0, lw = 0.0, vmin = INFINITY;
+    for (int q = g; q < nu_; q += gs) lu = fmax(lu, pu[q]);
+    for (int q = g; q < nvw; q += gs) { lv = fmax(lv, pv[q]); lw = fmax(lw, pw[q]); }
+    const int ni = d.nxg - 2, nk = d.nzg - 2, ny = d.ny;
+    for (int q = g; q < ni * nk; q += gs) {  // V's wall row: one spacing, its smallest |v|
+        const int i = 2 + q % ni, k = 2 + q / ni;
+        const int64_t e = i3(d.nxg, ny, i, ny, k);
+        const double v = vfp32 ? (double)((const float *)Vp)[e] : ((const double *)Vp)[e];
+        vmin = fmin(vmin, fabs(v));
+    }
+    lu = block_reduce(lu, OpMax(), sh);
+    lv = block_reduce(lv, OpMax(), sh);
+    lw = block_reduce(lw, OpMax(), sh);
+    vmin = block_reduce(vmin, OpMin(), sh);
+    if (threadIdx.x == 0) {
+        mid[4 * blockIdx.x + 0] = lu;
+        mid[4 * blockIdx.x + 1] = lv;
+        mid[4 * blockIdx.x + 2] = lw;
+        mid[4 * blockIdx.x + 3] = vmin;
+    }
+}
+__global__ void __launch_bounds__(256)
+k_dt_fused_final(Dims d, const double *__restrict__ mid, const double *__restrict__ dyc, Scalars *sc, double cfl,
+                 double dt_vis) {
+    __shared__ double sh[8];
+    double lu = 0.0, lv = 0.0, lw = 0.0, vmin = INFINITY;
+    for (int q = threadIdx.x; q < DTF_NB; q += blockDim.x) {
+        lu = fmax(lu, mid[4 * q]);
+        lv = fmax(lv, mid[4 * q + 1]);
+        lw = fmax(lw, mid[4 * q + 2]);
+        vmin = fmin(vmin, mid[4 * q + 3]);
+    }
+    lu = block_reduce(lu, OpMax(), sh);
+    lv = block_reduce(lv, OpMax(), sh);
+    lw = block_reduce(lw, OpMax(), sh);
+    vmin = block_reduce(vmin, OpMin(), sh);
+    if (threadIdx.x == 0) {
+        lv = fmax(lv, dyc[d.ny] / vmin);
+        const double dc = fmin(fmin(cfl * lu, cfl * lv), cfl * lw);
+        double dt = fmin(dc, dt_vis);
+        dt = fmin(dt, 5e-3);
+        if (cfl < 0) dt = -cfl;
+        sc->dt_next = dt;
+        sc->dt_conv[0] = lu; sc->dt_conv[1] = lv; sc->dt_conv[2] = lw;
+    }
+}
+__global__ void k_dt_take_next(Scalars *sc) { sc->dt = sc->dt_next; }
+
+size_t dt_fused_mid_doubles() { return 4 * (size_t)DTF_NB; }
+template <typename S>
+void launch_dt_fused_final(const Dims &d, const DtParts &p, int nu_, int nvw, const double *dyc, const S *V,
+                           Scalars *sc, double cfl, double dt_vis, hipStream_t st) {
+    double *mid = p.pw + nvw;  // after the partial arrays (dt_fused_mid_doubles of room)
+    hipLaunchKernelGGL(k_dt_fused_mid, dim3(DTF_NB), dim3(256), 0, st, d, p.pu, nu_, p.pv, p.pw, nvw, dyc,
+                       (const void *)V, sizeof(S) == 4 ? 1 : 0, mid);
+    hipLaunchKernelGGL(k_dt_fused_final, dim3(1), dim3(256), 0, st, d, mid, dyc, sc, cfl, dt_vis);
+}
+void launch_dt_take_next(Scalars *sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_dt_take_next, dim3(1), dim3(1), 0, st, sc);
+}
+template void launch_dt_fused_final<double>(const Dims &, const DtParts &, int, int, const double *,
+                                            const double *, Scalars *, double, double, hipStream_t);
+template void launch_dt_fused_final<float>(const Dims &, const DtParts &, int, int, const double *, const float *,
+                                           Scalars *, double, double, hipStream_t);
 
 // projection.f90:234-256: signed maximum (from 0) and max |div| per plane
 template <typename S>
@@ -1099,11 +1265,16 @@ void launch_stats_accum(const double *out, double *acc, int n, hipStream_t st) {
 static inline int tiles_x(const Dims &d) { return (d.nxp + 63) / 64; }
 static inline int tiles_y(const Dims &d) { return (d.nyp + 3) / 4; }
 
+int rhs_rk_tiles(const Dims &d, bool nut) {
+    const int TYv = nut ? 4 : 8;
+    return ((d.nxp + RK_TX - 1) / RK_TX) * ((d.nyp + TYv - 1) / TYv) * ((d.nzp + RK_KC - 1) / RK_KC);
+}
+
 template <typename S, typename R>
 void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, const S *W,
                    const S *NT, S *Uo, S *Vo, S *Wo, S *Gu, S *Gv, S *Gw,
                    const Scalars *sc, const double *ubar, double nu, double om, double a_new,
-                   double a_g, int sub, hipStream_t st) {
+                   double a_g, int sub, hipStream_t st, double *mfpart, int mf_mode) {
     const bool nut = NT != nullptr, rot = fabs(om) > 1e-6;
     // 64x8 tiles without eddy viscosity, 64x4 with it (the ring of four
     // fields then stays below 64 KiB of LDS in FP64)
@@ -1113,13 +1284,17 @@ void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, cons
     const size_t shm = (size_t)(nut ? 4 : 3) * RK_NSLOT * (RK_TX + 2) * (TYv + 2) * sizeof(R);
 #define L(NUTB, ROTB, SUBV, TYB)                                                                           \
     hipLaunchKernelGGL((k_rhs_rk<S, R, NUTB, ROTB, SUBV, TYB>), dim3(nt), dim3(RK_TX, TYB), shm, st, d, m, U, V, \
-                       W, NT, Uo, Vo, Wo, Gu, Gv, Gw, sc, ubar, nu, om, a_new, a_g, tx, ty, nch, nt)
+                       W, NT, Uo, Vo, Wo, Gu, Gv, Gw, sc, ubar, nu, om, a_new, a_g, tx, ty, nch, nt,  \
+                       mfpart, mf_mode)
     if (sub == 1) {
         if (nut) { if (rot) L(true, true, 1, 4); else L(true, false, 1, 4); }
         else { if (rot) L(false, true, 1, 8); else L(false, false, 1, 8); }
-    } else {
+    } else if (!mfpart) {
         if (nut) { if (rot) L(true, true, 2, 4); else L(true, false, 2, 4); }
         else { if (rot) L(false, true, 2, 8); else L(false, false, 2, 8); }
+    } else {
+        if (nut) { if (rot) L(true, true, 3, 4); else L(true, false, 3, 4); }
+        else { if (rot) L(false, true, 3, 8); else L(false, false, 3, 8); }
     }
 #undef L
 }
@@ -1163,15 +1338,26 @@ void launch_ghosts3(const Dims &d, S *const *F, const int *field, const int *wal
                        dyg_top);
 }
 
+int project_vw_blocks(const Dims &d, int esize) {
+    static const int kc_env = getenv("CF_PV_KC") ? atoi(getenv("CF_PV_KC")) : 0;
+    const int kc = kc_env > 0 ? kc_env : (esize == 4 ? 8 : 1);
+    return tiles_x(d) * tiles_y(d) * ((d.nzp + kc - 1) / kc);
+}
+
 template <typename S, typename P>
-void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st) {
+void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st, double *dtv,
+                       double *dtw) {
     // planes per thread: a march hides the latency of the narrow FP32
     // accesses (8 planes measured best, 0.158 against 0.160 ms with 16);
     // FP64 rows move enough bytes per access with one plane
     static const int kc_env = getenv("CF_PV_KC") ? atoi(getenv("CF_PV_KC")) : 0;
     const int kc = kc_env > 0 ? kc_env : (sizeof(S) == 4 ? 8 : 1);
     const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * ((d.nzp + kc - 1) / kc);
-    hipLaunchKernelGGL((k_project_vw<S, P>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc);
+    // minima tracked for as many planes as the default march has (FP64: 1,
+    // FP32: 8)
+    constexpr int KM = sizeof(S) == 4 ? 8 : 1;
+    hipLaunchKernelGGL((k_project_vw<S, P, KM>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc, dtv,
+                       dtw);
 }
 
 template <typename S>
@@ -1190,6 +1376,17 @@ static int massflow_v_parts(const Dims &d) {
     return (int)((n + 255) / 256);
 }
 int massflow_parts(const Dims &d) { return std::max(massflow_u_parts(d), massflow_v_parts(d)); }
+
+// the correction from the partials k_rhs_rk left (mf_mode 1: bulk flow,
+// 2: centreline); writes corr and dPdx
+void launch_massflow_fused_final(const Dims &d, const double *mfpart, int nparts, int mf_mode, const double *ubar,
+                                 Scalars *sc, hipStream_t st) {
+    if (mf_mode == 1)
+        hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, mfpart, nparts, sc, 1);
+    else
+        hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(512), 0, st, mfpart, nparts, sc, ubar, d.ny / 2 + 1,
+                           (double)((d.nx - 2) * (d.nzg - 2)), (int)MF_CENTERLINE);
+}
 
 template <typename S>
 void launch_centerline_u(const Dims &d, const S *U, const double *ubar, double *part, Scalars *sc, int apply,
@@ -1355,7 +1552,7 @@ void launch_pressure_bc(const Dims &d, const S *V, const PressureBC &c, double *
     template void launch_rhs_rk<S, R>(const Dims &, const Metrics &, const S *, const S *, const S *, \
                                       const S *, S *, S *, S *, S *, S *, S *, const Scalars *,       \
                                       const double *, double, double, double, double, int,           \
-                                      hipStream_t);                                                   \
+                                      hipStream_t, double *, int);                                    \
     template void launch_rhs_only<S, R>(const Dims &, const Metrics &, const S *, const S *,          \
                                         const S *, const S *, double *, double *, double *,          \
                                         const Scalars *, const double *, double, double, hipStream_t, \
@@ -1397,9 +1594,9 @@ INST1(double)
 INST1(float)
 #undef INST1
 template void launch_project_vw<double, double>(const Dims &, const Metrics &, const double *, double *,
-                                                 double *, hipStream_t);
+                                                 double *, hipStream_t, double *, double *);
 template void launch_project_vw<float, float>(const Dims &, const Metrics &, const float *, float *,
-                                               float *, hipStream_t);
+                                               float *, hipStream_t, double *, double *);
 
 }  // namespace cf
 

@@ -46,8 +46,10 @@ def test_mixed_tracks_fp64(nsteps):
     assert div < 1e-5
     s32, s64 = g32.compute_statistics(), g64.compute_statistics()
     assert relerr(s32["Umean"], s64["Umean"]) < 1e-4
-    # bulk flow held at its initial value to FP64 accuracy by the FP64 reduction
-    assert abs(g32.compute_mean_mass_flow_U() - g32.scalars["Qflow_x_0"]) < 1e-12
+    # bulk flow held at its initial value by the FP64 correction; the bulk is
+    # taken from U* before the last projection, which changes it only by the
+    # FP32 rounding of the projected values (about 1e-12 per step)
+    assert abs(g32.compute_mean_mass_flow_U() - g32.scalars["Qflow_x_0"]) < 1e-10
 
 
 def test_mixed_lane_solve_and_fixed_plans_track_fp64(monkeypatch):
