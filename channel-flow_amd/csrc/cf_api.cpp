@@ -47,7 +47,9 @@ static FFTPlan make_plan(int n, std::string &err, bool radix8) {
     while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
     while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
-    if (m != 1) err = "FFT length " + std::to_string(n) + " has a prime factor other than 2 or 3";
+    while (m % 5 == 0 && p.nr < 16) { p.radix[p.nr++] = 5; m /= 5; }
+    while (m % 7 == 0 && p.nr < 16) { p.radix[p.nr++] = 7; m /= 7; }
+    if (m != 1) err = "FFT length " + std::to_string(n) + " has a prime factor other than 2, 3, 5 or 7";
     return p;
 }
 

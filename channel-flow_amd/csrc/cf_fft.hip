@@ -13,7 +13,7 @@
 // (one workgroup holds consecutive z rows of one y plane), and the fused
 // z-transform / y-solve kernels read whole contiguous kz rows.
 //
-// The FFT engine is an LDS Stockham autosort transform (radix 8, 4, 2, 3) over a
+// The FFT engine is an LDS Stockham autosort transform (radix 8, 4, 2, 3, 5, 7) over a
 // batch of lines per workgroup; each pass stages its butterflies through
 // registers so a single LDS buffer suffices.
 #include <cstdlib>
@@ -72,6 +72,57 @@ __device__ __forceinline__ void dft3(Cx<T> *a, int sign) {
     a[0] = cadd(a[0], t1);
     a[1] = cadd(m, r);
     a[2] = csub(m, r);
+}
+
+// cos and sin of 2 pi q / R for the odd radices 5 and 7, q = 1..(R-1)/2
+__host__ __device__ constexpr double root_cos(int R, int q) {
+    return R == 5 ? (q == 1 ? 0.309016994374947424102293417182819059 : -0.809016994374947424102293417182819059)
+                  : (q == 1 ? 0.623489801858733530525004884004239811
+                            : (q == 2 ? -0.222520933956314404288902564496794759
+                                      : -0.900968867902419126236102319507445051));
+}
+__host__ __device__ constexpr double root_sin(int R, int q) {
+    return R == 5 ? (q == 1 ? 0.951056516295153572116439333379382143 : 0.587785252292473129168705954639072769)
+                  : (q == 1 ? 0.78183148246802980870844452667405775
+                            : (q == 2 ? 0.974927912181823607018131682993931217
+                                      : 0.433883739117558120475768332848358755));
+}
+// DFT of odd prime length R (5, 7) from the input pairs (k, R-k):
+// t_k = a_k + a_{R-k}, u_k = a_k - a_{R-k},
+//   X(m)   = a_0 + sum_k t_k cos(2 pi k m/R) + sign i sum_k u_k sin(2 pi k m/R)
+//   X(R-m) = a_0 + sum_k t_k cos(2 pi k m/R) - sign i sum_k u_k sin(2 pi k m/R)
+// for m = 1..(R-1)/2; cos and sin of 2 pi q/R with q = k m mod R folded into
+// the first half (cos even, sin odd about R/2)
+template <int R, typename T>
+__device__ __forceinline__ void dft_odd(Cx<T> *a, int sign) {
+    constexpr int H = (R - 1) / 2;
+    Cx<T> t[H], u[H];
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        t[k - 1] = cadd(a[k], a[R - k]);
+        u[k - 1] = csub(a[k], a[R - k]);
+    }
+    const Cx<T> a0 = a[0];
+    Cx<T> s0 = a0;
+#pragma unroll
+    for (int k = 0; k < H; k++) s0 = cadd(s0, t[k]);
+    a[0] = s0;
+#pragma unroll
+    for (int m = 1; m <= H; m++) {
+        Cx<T> re = a0, im = {(T)0, (T)0};
+#pragma unroll
+        for (int k = 1; k <= H; k++) {
+            const int q = (k * m) % R;
+            const T c = (T)root_cos(R, q <= H ? q : R - q);
+            const T sn = (T)(q <= H ? root_sin(R, q) : -root_sin(R, R - q));
+            re = Cx<T>{re.x + c * t[k - 1].x, re.y + c * t[k - 1].y};
+            im = Cx<T>{im.x + sn * u[k - 1].x, im.y + sn * u[k - 1].y};
+        }
+        // sign i im
+        const Cx<T> r = sign < 0 ? Cx<T>{im.y, -im.x} : Cx<T>{-im.y, im.x};
+        a[m] = cadd(re, r);
+        a[R - m] = csub(re, r);
+    }
 }
 
 // LDS lines are padded with one element after every 8 (FP64 complex) or 16
@@ -164,10 +215,11 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n_,
                 }
                 v[c][r] = a;
             }
-            if (R == 2) dft2(v[c]);
-            else if (R == 4) dft4(v[c], sign);
-            else if (R == 8) dft8(v[c], sign);
-            else dft3(v[c], sign);
+            if constexpr (R == 2) dft2(v[c]);
+            else if constexpr (R == 4) dft4(v[c], sign);
+            else if constexpr (R == 8) dft8(v[c], sign);
+            else if constexpr (R == 3) dft3(v[c], sign);
+            else dft_odd<R>(v[c], sign);
         }
     }
     __syncthreads();
@@ -195,13 +247,15 @@ __device__ void lds_fft(Cx<T> *buf, int ls, int nlines, const FFTPlan &p, const 
         if (R == 8) fft_pass<8, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else if (R == 4) fft_pass<4, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else if (R == 2) fft_pass<2, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
-        else fft_pass<3, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
+        else if (R == 3) fft_pass<3, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
+        else if (R == 5) fft_pass<5, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
+        else fft_pass<7, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         Ns *= R;
     }
 }
 
 // The plan of a length-n transform (the host's make_plan): radix 8 first
-// when r8, then 4, 2 and 3.
+// when r8, then 4, 2, 3, 5 and 7.
 struct CPlan {
     int nr;
     int radix[16];
@@ -213,6 +267,8 @@ __host__ __device__ constexpr CPlan cplan(int n, bool r8) {
     while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
     while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
+    while (m % 5 == 0 && p.nr < 16) { p.radix[p.nr++] = 5; m /= 5; }
+    while (m % 7 == 0 && p.nr < 16) { p.radix[p.nr++] = 7; m /= 7; }
     return p;
 }
 

@@ -76,7 +76,12 @@ def test_invalid_parameters_are_rejected():
     assert "les_model" in err(les_model=3)
     assert "iwall_model" in err(iwall_model=2)
     assert "even" in err(cells=(35, 24, 16))
-    assert "FFT" in err(cells=(30, 24, 14 * 5))
+    assert "FFT" in err(cells=(30, 24, 2 * 11))   # z length 22: prime factor 11
+    assert "FFT" in err(cells=(2 * 13, 24, 16))  # half x length 13
+    for cells in ((40, 24, 28), (70, 16, 50), (30, 20, 42)):  # factors 5 and 7 are planned
+        c = cases.case_dns64()
+        c.cells = cells
+        assert footprint(c) > 0
     assert "nu" in err(nu=0.0)
     assert "eqwm_ncv" in err(iwall_model=3, eqwm_ncv=500)
     c = cases.case_dns64()
