@@ -128,6 +128,40 @@ def initial_field(cfg: CaseConfig):
     return np.asfortranarray(U), np.asfortranarray(V), np.asfortranarray(W)
 
 
+def large_scale_field(cfg: CaseConfig, amp=0.3, modes=24, seed=None):
+    """Poiseuille profile plus a seeded large-scale perturbation, for runs
+    that must become turbulent (the grid-scale noise of initial_field decays
+    at Re_b of a few thousand): each component is a sum of `modes` waves
+    sin(pi n y/Ly) cos(a x + b z + phase) with n in 1..3 and the x and z
+    wavenumbers among the four lowest of the box, amplitudes scaled so the
+    perturbation rms is `amp` (in units of the bulk velocity).  Vanishes at
+    the walls; not divergence-free until projected (initialise projects)."""
+    nx, ny, nz = cfg.faces
+    x, y, z = cfg.grid()
+    xg, yg, zg = centres(x), centres(y), centres(z)
+    Lx, Ly, Lz = cfg.lengths
+    rng = np.random.default_rng(cfg.seed if seed is None else seed)
+    prof = 1.5 * (1.0 - (yg - 1.0) ** 2)
+    out = []
+    # positions of each component: U on (x faces, y centres, z centres),
+    # V on (x centres, y faces, z centres), W on (x centres, y centres, z faces)
+    for c, (X, Y, Z) in enumerate(((x, yg, zg), (xg, y, zg), (xg, yg, z))):
+        f = np.zeros((len(X), len(Y), len(Z)))
+        for _ in range(modes):
+            n = rng.integers(1, 4)
+            a = 2.0 * np.pi * rng.integers(0, 4) / Lx
+            b = 2.0 * np.pi * rng.integers(0, 4) / Lz
+            ph = rng.uniform(0.0, 2.0 * np.pi)
+            w = rng.standard_normal()
+            f += w * (np.sin(np.pi * n * np.clip(Y, 0.0, Ly) / Ly)[None, :, None] *
+                      np.cos(a * X[:, None, None] + b * Z[None, None, :] + ph))
+        f *= amp / max(np.sqrt(np.mean(f ** 2)), 1e-300)
+        if c == 0:
+            f += prof[None, :, None]
+        out.append(np.asfortranarray(f))
+    return tuple(out)
+
+
 def retau_from_reb(reb):
     """Dean's correlation Re_tau = 0.09 (2 Re_b)^0.88."""
     return 0.09 * (2.0 * reb) ** 0.88
