@@ -283,6 +283,18 @@ __device__ __forceinline__ void lds_fft_c(Cx<T> *buf, int ls, int nlines, const 
     }
 }
 
+// Per-row offsets of the x transforms, written to LDS once per workgroup so
+// that the element loops add the x index instead of re-deriving (j, k) and
+// three-dimensional offsets per element.
+struct XRow {
+    int a, b, c, d;  // forward: U, V, W and W(k-1) at i = 2; inverse: phi, U at i = 2
+    int s;           // spectrum offset of (row, kx = 0): jp mh nzp + kp
+    int kr;          // z index kp of the row; -1 for rows beyond the last
+    int j;           // y index of the row (2..nyg-1)
+    int pad;
+    double r1, r2;   // forward: rdyf(j), rdzf(k); inverse: the bulk profile term added to u for dt
+};
+
 // ---------------------------------------------------------------------------
 // div(u*) fused with the forward real FFT along x.
 // compute_pseudo_pressure_rhs (projection.f90:45-53) for i = 2..nxp+1 then
@@ -306,28 +318,47 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
     T *ldr = reinterpret_cast<T *>(lds);
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
+    XRow *rows = reinterpret_cast<XRow *>(tw + M);
     for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
-    const float inz = 1.0f / (float)d.nzp;
+    const int mh = M + 1;
+    if (threadIdx.x < rb) {
+        const int r = row0 + threadIdx.x;
+        XRow t{};
+        t.kr = -1;
+        if (r < nrows) {
+            const int jr = r / d.nzp, kr = r - jr * d.nzp;
+            const int j = 2 + jr, k = 2 + kr;
+            t.a = (int)i3(nx, nyg, 2, j, k);
+            t.b = (int)i3(nxg, ny, 2, j, k);
+            t.c = (int)i3(nxg, nyg, 2, j, k);
+            // W(:,:,1) is read at its periodic image W(:,:,nz-1)
+            t.d = k == 2 ? t.c + (d.nz - 3) * nxg * nyg : t.c - nxg * nyg;
+            t.s = jr * mh * d.nzp + kr;
+            t.kr = kr;
+            t.j = j;
+            t.r1 = m.rdyf[j];
+            t.r2 = m.rdzf[k];
+        }
+        rows[threadIdx.x] = t;
+    }
+    __syncthreads();
     StepDiv el(threadIdx.x, WG, N);  // element (row lr = q, ip = r)
 #pragma unroll 4
     for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
         const int lr = el.q, ip = el.r;
-        const int r = row0 + lr;
+        const XRow &t = rows[lr];
         T dv = (T)0;
-        if (r < nrows) {
-            int jr, kr;
-            small_divmod(r, d.nzp, inz, jr, kr);
-            const int j = 2 + jr, k = 2 + kr, i = 2 + ip;
-            const int64_t qu = i3(nx, nyg, i, j, k), qv = i3(nxg, ny, i, j, k), qw = i3(nxg, nyg, i, j, k);
+        if (t.kr >= 0) {
+            const int i = 2 + ip;
             // the x and z ghost faces are read at their periodic images, so the
             // fields need no ghost fill between the RK update and this kernel
-            const int64_t qum = i == 2 ? qu + (nx - 3) : qu - 1;  // U(1) = U(nx-1)
-            const int64_t qwm = k == 2 ? qw + (int64_t)(d.nz - 3) * nxg * nyg : qw - (int64_t)nxg * nyg;  // W(:,:,1) = W(:,:,nz-1)
-            dv = ((T)U[qu] - (T)U[qum]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)m.rdyf[j] +
-                 ((T)W[qw] - (T)W[qwm]) * (T)m.rdzf[k];
+            const int qu = t.a + ip, qum = ip == 0 ? t.a + (nx - 3) : qu - 1;  // U(1) = U(nx-1)
+            const int qv = t.b + ip, qw = t.c + ip, qwm = t.d + ip;
+            dv = ((T)U[qu] - (T)U[qum]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)t.r1 +
+                 ((T)W[qw] - (T)W[qwm]) * (T)t.r2;
             if (ysrc) {
-                if (j == 2) dv += (T)ysrc[kr * d.nxp + ip];
-                if (j == nyg - 1) dv += (T)ysrc[(d.nzp + kr) * d.nxp + ip];
+                if (t.j == 2) dv += (T)ysrc[t.kr * d.nxp + ip];
+                if (t.j == nyg - 1) dv += (T)ysrc[(d.nzp + t.kr) * d.nxp + ip];
             }
         }
         ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
@@ -335,13 +366,12 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     __syncthreads();
     if constexpr (MC > 0) lds_fft_c<MC, FFT_X_RADIX8, WG, -1>(lds, ls, rb, tw);
     else lds_fft<WG>(lds, ls, rb, pM, tw, -1);
-    const int mh = M + 1;
     // consecutive threads take consecutive rows (kz) of one kx: runs of rb
     StepDiv ek(threadIdx.x, WG, rb);  // (kx = q, row lr = r)
     for (int e = threadIdx.x; e < rb * mh; e += WG, ek.next()) {
         const int kx = ek.q, lr = ek.r;
-        const int r = row0 + lr;
-        if (r >= nrows) continue;
+        const int sb = rows[lr].s;
+        if (rows[lr].kr < 0) continue;
         const Cx<T> *z = lds + lr * ls;
         const Cx<T> a = z[fpad<T>(kx == M ? 0 : kx)];
         const Cx<T> bc = z[fpad<T>(kx == 0 ? 0 : M - kx)];
@@ -350,9 +380,7 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         const Cx<T> dd = csub(a, b);
         const Cx<T> O = {(T)0.5 * dd.y, -(T)0.5 * dd.x};  // (a-b)/(2i)
         const Cx<T> X = cadd(E, cmul(twN[kx], O));
-        int jp, kp;
-        small_divmod(r, d.nzp, inz, jp, kp);
-        spec[((int64_t)jp * mh + kx) * d.nzp + kp] = X;
+        spec[sb + kx * d.nzp] = X;
     }
 }
 
@@ -721,6 +749,22 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
             lds[lr * ls + fpad<T>(kx)] = v;
         }
     }
+    // row table of the output loop
+    XRow *rows = reinterpret_cast<XRow *>(tw + M);
+    if (threadIdx.x < rb) {
+        const int r = row0 + threadIdx.x;
+        XRow t{};
+        t.kr = -1;
+        if (r < nrows) {
+            const int jp = r / d.nzp, kp = r - jp * d.nzp;
+            t.a = (kp * d.nyp + jp) * d.nxp;
+            t.b = (int)i3(d.nx, d.nyg, 2, 2 + jp, 2 + kp);
+            t.kr = kp;
+            t.j = 2 + jp;
+            if (dtp.pu && dtp.ubar) t.r1 = dtp.add_corr ? dtp.ubar[2 + jp] + dtp.sc->corr : dtp.ubar[2 + jp];
+        }
+        rows[threadIdx.x] = t;
+    }
     __syncthreads();
     // packed length-M input of the complex inverse, in place: each thread
     // owns the pair (k, M-k) (k = 0 pairs with M) and writes Z(k), Z(M-k)
@@ -747,7 +791,6 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     if constexpr (MC > 0) lds_fft_c<MC, FFT_X_RADIX8, WG, 1>(lds, ls, rb, tw);
     else lds_fft<WG>(lds, ls, rb, pM, tw, +1);
     const T *zr = reinterpret_cast<const T *>(lds);
-    const int nx = d.nx, nyg = d.nyg;
     const double rnorm = 1.0 / norm;
     const double corr = csc ? csc->corr : 0.0;
     // compute_dt partial: element e = tid + c WG of the loop below sits at
@@ -764,25 +807,23 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
 #pragma unroll 4
     for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
         const int lr = el.q, ip = el.r;
-        const int r = row0 + lr;
+        const XRow &t = rows[lr];
         const int sl = slot;
         slot = slot + 1 == NS ? 0 : slot + 1;
-        if (r >= nrows) continue;
-        int jp, kp;
-        small_divmod(r, d.nzp, inz, jp, kp);
+        if (t.kr < 0) continue;
         const double ph = (double)zr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] * rnorm;
-        phi[((int64_t)kp * d.nyp + jp) * d.nxp + ip] = (P)ph;
+        phi[t.a + ip] = (P)ph;
         const int ipn = ip + 1 == N ? 0 : ip + 1;
         const double phn = (double)zr[2 * (lr * ls + fpad<T>(ipn >> 1)) + (ipn & 1)] * rnorm;
         if (U) {
-            const int i = 2 + ip, j = 2 + jp, k = 2 + kp;
-            const int64_t qu = i3(nx, nyg, i, j, k);
+            const int i = 2 + ip;
+            const int qu = t.b + ip;
             const double up = (double)U[qu] - (phn - ph) * m.rdxc[i];
             const S us = (S)(csc ? up + corr : up);
             U[qu] = us;
             if (dtp.pu) {
                 double uf = (double)us;
-                if (dtp.ubar) uf = uf + (dtp.add_corr ? dtp.ubar[j] + dtp.sc->corr : dtp.ubar[j]);
+                if (dtp.ubar) uf = uf + t.r1;
                 const double au = fabs(uf);
                 if (NS > 3) lu = fmax(lu, m.dxc[i] / au);
                 else if (sl == 0) umin0 = fmin(umin0, au);
@@ -884,7 +925,7 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
                      hipStream_t st, const double *ysrc) {
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>);
+    const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>) + (size_t)rb * sizeof(XRow);
     const dim3 grid((nrows + rb - 1) / rb);
     if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_div_fftx<S, T, 512, 0>), grid, dim3(512), shm, st, d, m, U, V, W, spec, pM, twM, twN,
@@ -975,7 +1016,7 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
     const DtParts dtp = dtpp ? *dtpp : DtParts{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
-    const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>);
+    const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>) + (size_t)rb * sizeof(XRow);
     const dim3 grid((nrows + rb - 1) / rb);
     if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512, 0>), grid, dim3(512), shm, st, d, m, spec, phi, U, pM, twM,
