@@ -37,14 +37,16 @@ std::string last_error_copy() { return g_err; }
         }                                                                                      \
     } while (0)
 
-// radix 8 pays for the z transform (many lines per workgroup); the x
-// transforms keep more threads busy per pass with radix 4
+// radix 8 first when radix8 (both transforms use it: fewer LDS passes),
+// then 4, one 6 for a remaining 2 x 3, 2, 3, 5 and 7; the device's compiled
+// plans (cplan) factor the same way, so the pass twiddle tables match
 static FFTPlan make_plan(int n, std::string &err, bool radix8) {
     FFTPlan p{};
     p.n = n;
     int m = n;
     while (radix8 && m % 8 == 0 && p.nr < 16) { p.radix[p.nr++] = 8; m /= 8; }
     while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
+    if (m % 6 == 0 && p.nr < 16) { p.radix[p.nr++] = 6; m /= 6; }  // the last 2 with a 3: one pass
     while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
     while (m % 5 == 0 && p.nr < 16) { p.radix[p.nr++] = 5; m /= 5; }

@@ -13,7 +13,7 @@
 // (one workgroup holds consecutive z rows of one y plane), and the fused
 // z-transform / y-solve kernels read whole contiguous kz rows.
 //
-// The FFT engine is an LDS Stockham autosort transform (radix 8, 4, 2, 3, 5, 7) over a
+// The FFT engine is an LDS Stockham autosort transform (radix 8, 6, 4, 2, 3, 5, 7) over a
 // batch of lines per workgroup; each pass stages its butterflies through
 // registers so a single LDS buffer suffices.
 #include <cstdlib>
@@ -72,6 +72,25 @@ __device__ __forceinline__ void dft3(Cx<T> *a, int sign) {
     a[0] = cadd(a[0], t1);
     a[1] = cadd(m, r);
     a[2] = csub(m, r);
+}
+
+// radix 6 as two radix-3 transforms of the even and odd samples and one
+// radix-2 stage with the twiddles W6^k = exp(sign 2 pi i k / 6), k = 0, 1, 2
+template <typename T>
+__device__ __forceinline__ void dft6(Cx<T> *a, int sign) {
+    Cx<T> e[3] = {a[0], a[2], a[4]}, o[3] = {a[1], a[3], a[5]};
+    dft3(e, sign);
+    dft3(o, sign);
+    const T h = (T)0.5, s3 = (T)0.86602540378443864676372317075294;
+    const T sg = sign < 0 ? (T)-1 : (T)1;
+    const Cx<T> t1 = cmul(o[1], Cx<T>{h, sg * s3});
+    const Cx<T> t2 = cmul(o[2], Cx<T>{-h, sg * s3});
+    a[0] = cadd(e[0], o[0]);
+    a[3] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], t1);
+    a[4] = csub(e[1], t1);
+    a[2] = cadd(e[2], t2);
+    a[5] = csub(e[2], t2);
 }
 
 // cos and sin of 2 pi q / R for the odd radices 5 and 7, q = 1..(R-1)/2
@@ -219,6 +238,7 @@ __device__ __forceinline__ void fft_pass(Cx<T> *buf, int ls, int nlines, int n_,
             else if constexpr (R == 4) dft4(v[c], sign);
             else if constexpr (R == 8) dft8(v[c], sign);
             else if constexpr (R == 3) dft3(v[c], sign);
+            else if constexpr (R == 6) dft6(v[c], sign);
             else dft_odd<R>(v[c], sign);
         }
     }
@@ -248,6 +268,7 @@ __device__ void lds_fft(Cx<T> *buf, int ls, int nlines, const FFTPlan &p, const 
         else if (R == 4) fft_pass<4, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else if (R == 2) fft_pass<2, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else if (R == 3) fft_pass<3, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
+        else if (R == 6) fft_pass<6, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else if (R == 5) fft_pass<5, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         else fft_pass<7, WG>(buf, ls, nlines, p.n, Ns, tw, sign);
         Ns *= R;
@@ -255,7 +276,7 @@ __device__ void lds_fft(Cx<T> *buf, int ls, int nlines, const FFTPlan &p, const 
 }
 
 // The plan of a length-n transform (the host's make_plan): radix 8 first
-// when r8, then 4, 2, 3, 5 and 7.
+// when r8, then 4, one 6 for a remaining 2 x 3, then 2, 3, 5 and 7.
 struct CPlan {
     int nr;
     int radix[16];
@@ -265,6 +286,7 @@ __host__ __device__ constexpr CPlan cplan(int n, bool r8) {
     int m = n;
     while (r8 && m % 8 == 0 && p.nr < 16) { p.radix[p.nr++] = 8; m /= 8; }
     while (m % 4 == 0 && p.nr < 16) { p.radix[p.nr++] = 4; m /= 4; }
+    if (m % 6 == 0 && p.nr < 16) { p.radix[p.nr++] = 6; m /= 6; }  // the last 2 with a 3: one pass
     while (m % 2 == 0 && p.nr < 16) { p.radix[p.nr++] = 2; m /= 2; }
     while (m % 3 == 0 && p.nr < 16) { p.radix[p.nr++] = 3; m /= 3; }
     while (m % 5 == 0 && p.nr < 16) { p.radix[p.nr++] = 5; m /= 5; }
