@@ -17,8 +17,11 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("cells", [(40, 24, 28), (70, 16, 50), (30, 20, 42)])
+@pytest.mark.parametrize("cells", [(40, 24, 28), (70, 16, 50), (30, 20, 42),
+                                   (2048, 8, 12), (12, 8, 2048), (1400, 8, 10)])
 def test_ten_steps_with_radix_5_and_7(cells):
+    """Also the largest transform lengths cf_create accepts (half x length
+    1024, z length 2048, run-time plans) and 1400 = 2^3 5^2 7 in x."""
     from channel_flow.solver import ChannelFlow
     cfg = cases.case_dns64()
     cfg.cells = cells
