@@ -4,9 +4,12 @@
 // the ODE solver as fallback outside the training hull.
 //
 // Precision: the two 64x64 layers run on MFMA in BF16 with FP32
-// accumulation (v_mfma_f32_16x16x16_bf16); the 4->64 input layer and the
-// 64->1 output layer are FP32 FMAs (K = 4 and N = 1 do not fill a matrix
-// tile).  Master weights and Adam moments are FP32.
+// accumulation: gfx950's v_mfma_f32_16x16x32_bf16 (K = 32, two per 64-wide
+// reduction) for the forward and backward products, and
+// v_mfma_f32_16x16x16_bf16 for the weight gradients, whose reduction runs
+// over the 16 samples of a chunk; the 4->64 input layer and the 64->1
+// output layer are FP32 FMAs (K = 4 and N = 1 do not fill a matrix tile).
+// Master weights and Adam moments are FP32.
 //
 // Training: one single-wavefront workgroup owns 16 samples at a time, with
 // its activations in LDS (the backward pass needs them in both layouts).
@@ -27,7 +30,22 @@
 namespace cf {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short short8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// 16x16x32 BF16 product: lane (r16, q4) holds positions 8 q4 + j (j < 8) of
+// the 32-long reduction, for row r16 of A and column r16 of B; the
+// accumulator holds rows 4 q4 + i of column r16, as for K = 16.  A and B
+// share the position -> reduction-index map, so any one map gives the same
+// product: each kernel picks the map that matches where its data lies.
+// Operands are assembled as BF16 bit patterns in a short vector and cast as
+// a whole (element-wise __bf16 insertion from bit patterns was measured to
+// give wrong products).
+__device__ inline floatx4 mfma32(const short8 &a, const short8 &b, const floatx4 &c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
 
 __host__ __device__ inline unsigned short f2bf(float f) {
     unsigned u;
@@ -85,14 +103,14 @@ k_mlp(const float *__restrict__ X, const float *__restrict__ Y, int n, int chunk
     auto layer = [&](float (*A)[LDP], const unsigned short *Wb, float (*D)[LDP], const float *bias) {
         for (int t = 0; t < 4; t++) {
             floatx4 acc = {0, 0, 0, 0};
-            for (int kk = 0; kk < 4; kk++) {
-                bf16x4 af, bfr;
-                for (int i = 0; i < 4; i++) {
-                    const int k = 16 * kk + 4 * q4 + i;
+            for (int kk = 0; kk < 2; kk++) {
+                short8 af, bfr;
+                for (int i = 0; i < 8; i++) {
+                    const int k = 32 * kk + 8 * q4 + i;
                     af[i] = (short)f2bf(A[r16][k]);
                     bfr[i] = (short)Wb[k * MLP_H + 16 * t + r16];
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc, 0, 0, 0);
+                acc = mfma32(af, bfr, acc);
             }
             for (int i = 0; i < 4; i++) {
                 const int row = 4 * q4 + i, col = 16 * t + r16;
@@ -104,14 +122,14 @@ k_mlp(const float *__restrict__ X, const float *__restrict__ Y, int n, int chunk
     auto back = [&](float (*Dz)[LDP], const unsigned short *Wb, float (*Out)[LDP], float (*Act)[LDP]) {
         for (int t = 0; t < 4; t++) {  // output tile over the layer input index
             floatx4 acc = {0, 0, 0, 0};
-            for (int kk = 0; kk < 4; kk++) {  // reduction over the layer output index
-                bf16x4 af, bfr;
-                for (int i = 0; i < 4; i++) {
-                    const int k = 16 * kk + 4 * q4 + i;
+            for (int kk = 0; kk < 2; kk++) {  // reduction over the layer output index
+                short8 af, bfr;
+                for (int i = 0; i < 8; i++) {
+                    const int k = 32 * kk + 8 * q4 + i;
                     af[i] = (short)f2bf(Dz[r16][k]);
                     bfr[i] = (short)Wb[(16 * t + r16) * MLP_H + k];
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc, 0, 0, 0);
+                acc = mfma32(af, bfr, acc);
             }
             for (int i = 0; i < 4; i++) {
                 const int row = 4 * q4 + i, col = 16 * t + r16;
@@ -268,11 +286,13 @@ __host__ __device__ inline void mlp_features(double u, double h, double nu, doub
 // inference
 // ---------------------------------------------------------------------------
 // The 64x64 layers run as transposed products H^T = W^T A^T on
-// v_mfma_f32_16x16x16bf16_1k: the weights are the A operand and the 16
+// v_mfma_f32_16x16x32_bf16: the weights are the A operand and the 16
 // samples of a wavefront are the columns.  The accumulator of output tile t
-// holds, in lane (r16, q4), units 16t+4q4+i of sample r16 -- the B-operand
-// layout of reduction block t of the next layer -- so activations stay in
-// registers from the features to the output.  The BF16 A-operand fragments
+// holds, in lane (r16, q4), units 16t+4q4+i of sample r16.  The next layer's
+// reduction block kk (K = 32) takes, in lane (r16, q4), the accumulators of
+// tiles 2kk and 2kk+1: positions 8q4 + j hold units 16(2kk + j/4) + 4q4 +
+// j%4, and the weight fragments are loaded with the same map -- so
+// activations stay in registers from the features to the output.  The BF16 A-operand fragments
 // of both 64x64 layers are loaded into registers once per wavefront and
 // reused over its chunks of 16 samples; the FP32 input- and output-layer
 // weights and the biases are read from LDS as float4 per 4 units.
@@ -331,13 +351,13 @@ k_mlp_infer(int n, MlpInputs in, const float *__restrict__ P, const unsigned sho
     const int nwaves = gridDim.x * (MLP_INFER_WG / 64);
     // lane (r16, q4) owns units 16t + 4q4 + i: the float4 at sp[blk*16 + 4t + q4]
     auto par = [&](int blk, int t) { return sp[blk * (MLP_H / 4) + 4 * t + q4]; };
-    bf16x4 a2[4][4], a3[4][4];  // [output tile][reduction block]
+    short8 a2[4][2], a3[4][2];  // [output tile][reduction block of 32]
     for (int t = 0; t < 4; t++)
-        for (int kk = 0; kk < 4; kk++)
-            for (int i = 0; i < 4; i++) {
-                const int k = 16 * kk + 4 * q4 + i;
-                a2[t][kk][i] = (short)W2b[k * MLP_H + 16 * t + r16];
-                a3[t][kk][i] = (short)W3b[k * MLP_H + 16 * t + r16];
+        for (int kk = 0; kk < 2; kk++)
+            for (int j = 0; j < 8; j++) {
+                const int k = 16 * (2 * kk + j / 4) + 4 * q4 + j % 4;
+                a2[t][kk][j] = (short)W2b[k * MLP_H + 16 * t + r16];
+                a3[t][kk][j] = (short)W3b[k * MLP_H + 16 * t + r16];
             }
     const float b4 = P[L::B4];
     // lane q4 standardises feature q4 of sample r16; features that do not
@@ -380,20 +400,30 @@ k_mlp_infer(int n, MlpInputs in, const float *__restrict__ P, const unsigned sho
                 h1[t][i] = bf16_bits(tanh_fast(z));
             }
         }
+        // the B operands of a layer: tiles 2kk and 2kk+1 side by side
+        auto bpair = [](const bf16x4 *h, int kk) {
+            short8 b;
+            for (int j = 0; j < 8; j++) b[j] = h[2 * kk + j / 4][j % 4];
+            return b;
+        };
         // layer 2
         bf16x4 h2[4];
+        const short8 b10 = bpair(h1, 0), b11 = bpair(h1, 1);
         for (int t = 0; t < 4; t++) {
             floatx4 acc = {0, 0, 0, 0};
-            for (int kk = 0; kk < 4; kk++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][kk], h1[kk], acc, 0, 0, 0);
+            acc = mfma32(a2[t][0], b10, acc);
+            acc = mfma32(a2[t][1], b11, acc);
             const float4 bb = par(5, t);
             const float zb[4] = {bb.x, bb.y, bb.z, bb.w};
             for (int i = 0; i < 4; i++) h2[t][i] = bf16_bits(tanh_fast(acc[i] + zb[i]));
         }
         // layer 3 and the output layer (FP32)
         float y = 0.0f;
+        const short8 b20 = bpair(h2, 0), b21 = bpair(h2, 1);
         for (int t = 0; t < 4; t++) {
             floatx4 acc = {0, 0, 0, 0};
-            for (int kk = 0; kk < 4; kk++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[t][kk], h2[kk], acc, 0, 0, 0);
+            acc = mfma32(a3[t][0], b20, acc);
+            acc = mfma32(a3[t][1], b21, acc);
             const float4 bb = par(6, t), ww = par(7, t);
             const float zb[4] = {bb.x, bb.y, bb.z, bb.w}, zw[4] = {ww.x, ww.y, ww.z, ww.w};
             for (int i = 0; i < 4; i++) y += tanh_fast(acc[i] + zb[i]) * zw[i];
