@@ -435,7 +435,7 @@ void Ctx<S>::ghosts(S *const *F, int mask) {
 }
 
 template <typename S>
-void Ctx<S>::poisson(S *const *F, S *phi, const Scalars *corr, const DtParts *dtp) {
+void Ctx<S>::poisson(S *const *F, S *phi, const Scalars *corr, const DtParts *dtp, int gcopy) {
     const Dims &d = h.d;
     tic("div_fftx");
     launch_div_fftx<S, S>(d, met, F[0], F[1], F[2], spec, pM, twM, twN, stream);
@@ -450,10 +450,11 @@ void Ctx<S>::poisson(S *const *F, S *phi, const Scalars *corr, const DtParts *dt
     launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
     toc();
     tic("ifftx_proju");
-    launch_ifftx_proju<S, S, S>(d, met, spec, phi, F[0], pM, twM, twN, (double)(d.nxp * d.nzp), stream, corr, dtp);
+    launch_ifftx_proju<S, S, S>(d, met, spec, phi, F[0], pM, twM, twN, (double)(d.nxp * d.nzp), stream, corr, dtp,
+                                gcopy);
     toc();
     tic("project_vw");
-    launch_project_vw<S, S>(d, met, phi, F[1], F[2], stream, dtp ? dtp->pv : nullptr, dtp ? dtp->pw : nullptr);
+    launch_project_vw<S, S>(d, met, phi, F[1], F[2], stream, dtp ? dtp->pv : nullptr, dtp ? dtp->pw : nullptr, gcopy);
     toc();
 }
 
@@ -519,6 +520,12 @@ void Ctx<S>::step_launch() {
     // compute_dt of the next step from the final velocities as the last
     // projection writes them (the mixed path's U with its shifted profile)
     const bool fz = fuse_dt();
+    // ghost copies by the projection kernels (FP64, Dirichlet walls; in the
+    // mixed path the extra registers cost the FP32 V/W projection a wave per
+    // SIMD, more than the ghost pass); env CF_GHOST_COPY=0 selects the ghost
+    // pass (read at graph capture)
+    const char *gce = getenv("CF_GHOST_COPY");
+    const bool gcp = sizeof(S) == 8 && h.prm.iwall_model != 3 && (!gce || atoi(gce) != 0);
     const int ndu = ifftx_blocks(d, sizeof(S)), ndvw = project_vw_blocks(d, sizeof(S));
     const DtParts dp{dtfp, dtfp + ndu, dtfp + ndu + ndvw, ubar, sc, (ubar && mfm) ? 1 : 0};
     for (int s = 1; s <= 3; s++) {
@@ -539,9 +546,12 @@ void Ctx<S>::step_launch() {
         }
         // the divergence reads the x and z ghost faces at their periodic
         // images, so the boundary conditions are applied once, after the
-        // projection (time_integration.f90 applies them before and after)
-        poisson(Fn, s == 1 ? phi1 : phi23, (mfp && !ubar) ? sc : nullptr, (s == 3 && fz) ? &dp : nullptr);
-        ghosts(Fn, 7);
+        // projection (time_integration.f90 applies them before and after);
+        // with Dirichlet walls the projection kernels write the ghosts
+        // themselves, with the wall model's Neumann walls a ghost pass does
+        poisson(Fn, s == 1 ? phi1 : phi23, (mfp && !ubar) ? sc : nullptr, (s == 3 && fz) ? &dp : nullptr,
+                gcp ? 1 : 0);
+        if (!gcp) ghosts(Fn, 7);
         cur = 1 - cur;
     }
     S *const *F = st[cur];

@@ -98,4 +98,34 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
     return xcd * q + (xcd < r ? xcd : r) + slot;
 }
 
+// Ghost copies of one unique value v at (i, j, k) of an (n1, n2, *) field,
+// as apply_boundary_conditions (boundary_conditions.f90:20-75, one rank)
+// leaves them with Dirichlet walls: gi, gk are the ghost column and plane
+// whose periodic image (i, k) is (0: none), gj the wall ghost row mirroring
+// row j (0: none), which takes -v.  Every combination other than (i, j, k)
+// itself is written, so each ghost cell is written once, by its source.
+template <typename S>
+__device__ inline void put_ghost_copies(S *F, int n1, int n2, int i, int j, int k, S v, int gi, int gj, int gk) {
+    for (int a = 0; a < 2; a++) {
+        const int ii = a ? gi : i;
+        if (!ii) continue;
+        for (int c = 0; c < 2; c++) {
+            const int kk = c ? gk : k;
+            if (!kk) continue;
+            for (int b = 0; b < 2; b++) {
+                const int jj = b ? gj : j;
+                if (!jj || (a == 0 && b == 0 && c == 0)) continue;
+                F[i3(n1, n2, ii, jj, kk)] = b ? (S)(-v) : v;
+            }
+        }
+    }
+}
+// the ghost column or plane whose periodic image is unique index i: faces
+// (1..n, unique 2..n-1): 1 <- n-1, n <- 2; centres (1..n, unique 2..n-2):
+// 1 <- n-2, n-1 <- 2, n <- 3
+__host__ __device__ inline int ghost_of_face(int i, int n) { return i == n - 1 ? 1 : (i == 2 ? n : 0); }
+__host__ __device__ inline int ghost_of_ctr(int i, int n) {
+    return i == n - 2 ? 1 : (i == 2 ? n - 1 : (i == 3 ? n : 0));
+}
+
 }  // namespace cf

@@ -727,7 +727,8 @@ template <typename S, typename T, typename P, int WG, int MC>
 __global__ void __launch_bounds__(WG)
 k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__ phi,
               S *__restrict__ U, FFTPlan pM, const Cx<T> *__restrict__ twM,
-              const Cx<T> *__restrict__ twN, int rb, double norm, const Scalars *__restrict__ csc, DtParts dtp) {
+              const Cx<T> *__restrict__ twN, int rb, double norm, const Scalars *__restrict__ csc, DtParts dtp,
+              int gcopy) {
     extern __shared__ unsigned char smem_raw[];
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
@@ -843,6 +844,11 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
             const double up = (double)U[qu] - (phn - ph) * m.rdxc[i];
             const S us = (S)(csc ? up + corr : up);
             U[qu] = us;
+            if (gcopy) {  // periodic and Dirichlet-wall ghost copies of U
+                const int gi = ghost_of_face(i, d.nx), gk = ghost_of_ctr(2 + t.kr, d.nzg);
+                const int gj = t.j == 2 ? 1 : (t.j == d.nyg - 1 ? d.nyg : 0);
+                if (gi | gj | gk) put_ghost_copies(U, d.nx, d.nyg, i, t.j, 2 + t.kr, us, gi, gj, gk);
+            }
             if (dtp.pu) {
                 double uf = (double)us;
                 if (dtp.ubar) uf = uf + t.r1;
@@ -1034,7 +1040,7 @@ int ifftx_blocks(const Dims &d, int esize) {
 template <typename S, typename T, typename P>
 void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
                         const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,
-                        hipStream_t st, const Scalars *corr, const DtParts *dtpp) {
+                        hipStream_t st, const Scalars *corr, const DtParts *dtpp, int gcopy) {
     const DtParts dtp = dtpp ? *dtpp : DtParts{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
@@ -1042,19 +1048,19 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
     const dim3 grid((nrows + rb - 1) / rb);
     if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512, 0>), grid, dim3(512), shm, st, d, m, spec, phi, U, pM, twM,
-                           twN, rb, norm, corr, dtp);
+                           twN, rb, norm, corr, dtp, gcopy);
         return;
     }
 #define CF_IFFTX(MC)                                                                                          \
     case MC:                                                                                                  \
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, MC>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM, \
-                           twM, twN, rb, norm, corr, dtp);                                                    \
+                           twM, twN, rb, norm, corr, dtp, gcopy);                                                    \
         return;
     switch (fft_static_sizes() ? pM.n : 0) {
         CF_X_LENGTHS(CF_IFFTX)
         default:
             hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, 0>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM,
-                               twM, twN, rb, norm, corr, dtp);
+                               twM, twN, rb, norm, corr, dtp, gcopy);
     }
 #undef CF_IFFTX
 }
@@ -1081,14 +1087,14 @@ template void launch_ifftx_proju<double, double, double>(const Dims &, const Met
                                                          const Cx<double> *, double *, double *,
                                                          const FFTPlan &, const Cx<double> *,
                                                          const Cx<double> *, double, hipStream_t,
-                                                         const Scalars *, const DtParts *);
+                                                         const Scalars *, const DtParts *, int);
 template void launch_ifftx_proju<float, float, float>(const Dims &, const Metrics &, const Cx<float> *,
                                                       float *, float *, const FFTPlan &,
                                                       const Cx<float> *, const Cx<float> *, double,
-                                                      hipStream_t, const Scalars *, const DtParts *);
+                                                      hipStream_t, const Scalars *, const DtParts *, int);
 template void launch_ifftx_proju<float, float, double>(const Dims &, const Metrics &, const Cx<float> *,
                                                        double *, float *, const FFTPlan &,
                                                        const Cx<float> *, const Cx<float> *, double,
-                                                       hipStream_t, const Scalars *, const DtParts *);
+                                                       hipStream_t, const Scalars *, const DtParts *, int);
 
 }  // namespace cf

@@ -168,7 +168,8 @@ struct Ctx : CtxBase {
             if (g) { hipGraphExecDestroy(g); g = nullptr; }
     }
     void ghosts(S *const *F, int mask);
-    void poisson(S *const *F, S *phi, const Scalars *corr = nullptr, const DtParts *dtp = nullptr);
+    // gcopy: the projection kernels also write the periodic and wall ghosts
+    void poisson(S *const *F, S *phi, const Scalars *corr = nullptr, const DtParts *dtp = nullptr, int gcopy = 0);
     void eddy_viscosity(S *const *F);
     void wall_model(S *const *F);
     void dt_launch();

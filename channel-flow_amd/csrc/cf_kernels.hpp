@@ -64,7 +64,7 @@ void launch_ghosts3(const Dims &d, S *const *F, const int *field, const int *wal
 // spacing/|w| over the projected values, project_vw_blocks(d, sizeof(S))
 template <typename S, typename P>
 void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st,
-                       double *dtv = nullptr, double *dtw = nullptr);
+                       double *dtv = nullptr, double *dtw = nullptr, int gcopy = 0);
 int project_vw_blocks(const Dims &d, int esize);
 int ifftx_blocks(const Dims &d, int esize);
 // dt_next from the partial maxima of a step's last projection (DtParts)
@@ -140,11 +140,16 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
 // U null: phi only, no projection of U; corr (optional): the U projection
 // also adds corr->corr (the mass-flow correction, time_integration.f90:155)
 // dtp (optional, dtp.pu non-null): per-workgroup maxima of compute_dt's
-// spacing/|u| over the final U, ifftx_blocks(d, sizeof(T)) of them
+// spacing/|u| over the final U, ifftx_blocks(d, sizeof(T)) of them.
+// gcopy (here and in launch_project_vw): each projected value is also
+// written to the periodic and Dirichlet-wall ghost cells it is the source
+// of, so no ghost pass is needed after the projection (walls without a wall
+// model)
 template <typename S, typename T, typename P>
 void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *phi, S *U,
                         const FFTPlan &pM, const Cx<T> *twM, const Cx<T> *twN, double norm,
-                        hipStream_t st, const Scalars *corr = nullptr, const DtParts *dtp = nullptr);
+                        hipStream_t st, const Scalars *corr = nullptr, const DtParts *dtp = nullptr,
+                        int gcopy = 0);
 
 // ---- wall model (cf_eqwm.hip) ----
 struct EqwmParams {

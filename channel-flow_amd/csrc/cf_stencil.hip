@@ -576,7 +576,7 @@ struct OpAdd { __device__ double operator()(double a, double b) const { return a
 struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
 struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
 
-template <typename S, typename P, int KM>
+template <typename S, typename P, int KM, bool GC>
 __global__ void __launch_bounds__(256)
 k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
              S *__restrict__ W, int tiles_x, int tiles_y, int ntiles, int kc, double *__restrict__ dtv,
@@ -611,9 +611,18 @@ k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
                 const S vn = (S)((double)*vp - (pj - pc) * ry);
                 *vp = vn;
                 vmin = fmin(vmin, fabs((double)vn));
+                if (GC) {  // periodic ghost copies of V (its wall faces stay zero)
+                    const int gi = ghost_of_ctr(i, d.nxg), gk = ghost_of_ctr(k, d.nzg);
+                    if (gi | gk) put_ghost_copies(V, d.nxg, d.ny, i, j, k, vn, gi, 0, gk);
+                }
             }
             const S wn = (S)((double)*wp - (pn - pc) * m.rdzc[k]);
             *wp = wn;
+            if (GC) {  // periodic and Dirichlet-wall ghost copies of W
+                const int gi = ghost_of_ctr(i, d.nxg), gk = ghost_of_face(k, d.nz);
+                const int gj = j == 2 ? 1 : (j == d.nyg - 1 ? d.nyg : 0);
+                if (gi | gj | gk) put_ghost_copies(W, d.nxg, d.nyg, i, j, k, wn, gi, gj, gk);
+            }
             if (dtw) {
                 const int q = k - kb;
                 if (q < KM) {
@@ -1346,7 +1355,7 @@ int project_vw_blocks(const Dims &d, int esize) {
 
 template <typename S, typename P>
 void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W, hipStream_t st, double *dtv,
-                       double *dtw) {
+                       double *dtw, int gcopy) {
     // planes per thread: a march hides the latency of the narrow FP32
     // accesses (8 planes measured best, 0.158 against 0.160 ms with 16);
     // FP64 rows move enough bytes per access with one plane
@@ -1356,8 +1365,12 @@ void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W
     // minima tracked for as many planes as the default march has (FP64: 1,
     // FP32: 8)
     constexpr int KM = sizeof(S) == 4 ? 8 : 1;
-    hipLaunchKernelGGL((k_project_vw<S, P, KM>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc, dtv,
-                       dtw);
+    if (gcopy)
+        hipLaunchKernelGGL((k_project_vw<S, P, KM, true>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc,
+                           dtv, dtw);
+    else
+        hipLaunchKernelGGL((k_project_vw<S, P, KM, false>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt,
+                           kc, dtv, dtw);
 }
 
 template <typename S>
@@ -1594,9 +1607,9 @@ INST1(double)
 INST1(float)
 #undef INST1
 template void launch_project_vw<double, double>(const Dims &, const Metrics &, const double *, double *,
-                                                 double *, hipStream_t, double *, double *);
+                                                 double *, hipStream_t, double *, double *, int);
 template void launch_project_vw<float, float>(const Dims &, const Metrics &, const float *, float *,
-                                               float *, hipStream_t, double *, double *);
+                                               float *, hipStream_t, double *, double *, int);
 
 }  // namespace cf
 
