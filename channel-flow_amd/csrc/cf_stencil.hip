@@ -262,35 +262,42 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
                          : (unsigned char)0;
     }
     const int64_t psu = (int64_t)nx * nyg, psv = (int64_t)nxg * ny, psw = (int64_t)nxg * nyg;
+    // The loads of fetch and gload are issued with no branch around them
+    // (an element outside the field reads element 0), and the zeroing of
+    // such elements waits for store: the compiler then counts the loads
+    // exactly and the plane prefetched one iteration ahead stays in flight
+    // through the next plane's arithmetic.
+    auto inu = [&](int q, int kk) { return (ok[q] & 1) && kk <= nzg; };
+    auto inw = [&](int q, int kk) { return (ok[q] & 2) && kk <= nz; };
+    auto inn = [&](int q, int kk) { return (ok[q] & 2) && kk <= nzg; };
+    auto inv = [&](int q, int kk) { return (ok[q] & 4) && kk <= nzg; };
     auto fetch = [&](int kk) {  // global -> registers (tile + halo of plane kk)
 #pragma unroll
         for (int q = 0; q < NPF; q++) {
-            R vu = 0, vv = 0, vw = 0, vn = 0;
-            if ((ok[q] & 1) && kk <= nzg) vu = (R)U[(int64_t)(kk - 1) * psu + ou[q]];
-            if ((ok[q] & 2) && kk <= nz) vw = (R)W[(int64_t)(kk - 1) * psw + ow[q]];
-            if (NUT && (ok[q] & 2) && kk <= nzg) vn = (R)NT[(int64_t)(kk - 1) * psw + ow[q]];
-            if ((ok[q] & 4) && kk <= nzg) vv = (R)V[(int64_t)(kk - 1) * psv + ov[q]];
-            pu[q] = vu; pv[q] = vv; pw[q] = vw; pn[q] = vn;
+            pu[q] = (R)U[inu(q, kk) ? (int64_t)(kk - 1) * psu + ou[q] : 0];
+            pw[q] = (R)W[inw(q, kk) ? (int64_t)(kk - 1) * psw + ow[q] : 0];
+            pv[q] = (R)V[inv(q, kk) ? (int64_t)(kk - 1) * psv + ov[q] : 0];
+            if (NUT) pn[q] = (R)NT[inn(q, kk) ? (int64_t)(kk - 1) * psw + ow[q] : 0];
         }
     };
-    auto store = [&](int sl) {  // registers -> ring slot
+    auto store = [&](int sl, int kk) {  // registers of plane kk -> ring slot
 #pragma unroll
         for (int q = 0; q < NPF; q++) {
             const int e = tid + q * NT_;
             if (e < PL) {
-                lu[sl * PL + e] = pu[q];
-                lv[sl * PL + e] = pv[q];
-                lw[sl * PL + e] = pw[q];
-                if (NUT) ln[sl * PL + e] = pn[q];
+                lu[sl * PL + e] = inu(q, kk) ? pu[q] : (R)0;
+                lv[sl * PL + e] = inv(q, kk) ? pv[q] : (R)0;
+                lw[sl * PL + e] = inw(q, kk) ? pw[q] : (R)0;
+                if (NUT) ln[sl * PL + e] = inn(q, kk) ? pn[q] : (R)0;
             }
         }
     };
     fetch(kbeg - 1);
-    store(0);
+    store(0, kbeg - 1);
     fetch(kbeg);
-    store(1);
+    store(1, kbeg);
     fetch(kbeg + 1);
-    store(2);
+    store(2, kbeg + 1);
     if (kbeg + 2 <= kend) fetch(kbeg + 2);
     __syncthreads();
 
@@ -313,10 +320,11 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const bool vrow = j <= ny - 1;
     R gu = 0, gv = 0, gw = 0;
     auto gload = [&](int kk, R &xu, R &xv, R &xw) {
-        if (SUB != 1 && active && kk < kend) {
-            xu = (R)Gu[(int64_t)(kk - 1) * psu + ou0];
-            if (vrow) xv = (R)Gv[(int64_t)(kk - 1) * psv + ov0];
-            xw = (R)Gw[(int64_t)(kk - 1) * psw + ov0];
+        if (SUB != 1) {  // values of inactive threads and rows are never used
+            const bool g = active && kk < kend, gv_ = g && vrow;
+            xu = (R)Gu[g ? (int64_t)(kk - 1) * psu + ou0 : 0];
+            xv = (R)Gv[gv_ ? (int64_t)(kk - 1) * psv + ov0 : 0];
+            xw = (R)Gw[g ? (int64_t)(kk - 1) * psw + ov0 : 0];
         }
     };
     gload(kbeg, gu, gv, gw);
@@ -333,8 +341,6 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     const int mf_kdup = mf_mode == 1 ? 2 : nzg - 2;
     for (int k = kbeg; k < kend; k++) {
         const int r0 = k - kbeg;
-        R gu_n = 0, gv_n = 0, gw_n = 0;
-        gload(k + 1, gu_n, gv_n, gw_n);
         if (active) {
             a.slot[0] = r0 & 3; a.slot[1] = (r0 + 1) & 3; a.slot[2] = (r0 + 2) & 3;
             R ru = 0, rv = 0, rw = 0;
@@ -362,10 +368,12 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
                 if (SUB == 3) mf_acc += (k == mf_kdup) ? 2.0 * (double)un : (double)un;
             }
         }
-        gu = gu_n; gv = gv_n; gw = gw_n;
+        // G of plane k+1 straight into the registers plane k has finished
+        // with (no copy of a load in flight at the loop's back edge)
+        gload(k + 1, gu, gv, gw);
         // plane k+2 into the slot of plane k-2, then prefetch plane k+3
         if (k + 2 <= kend) {
-            store((r0 + 3) & 3);
+            store((r0 + 3) & 3, k + 2);
             if (k + 3 <= kend) fetch(k + 3);
         }
         __syncthreads();
