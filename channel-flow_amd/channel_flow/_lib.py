@@ -86,6 +86,11 @@ SIGNATURES = {
     "cf_stats_window": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "cf_stats_reset": (ctypes.c_int, [ctypes.c_void_p]),
     "cf_footprint": (ctypes.c_longlong, [ctypes.POINTER(CfParams), _dp, _dp, _dp]),
+    "cf_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CfParams), _dp, _dp, _dp,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "cf_batch_member": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "cf_batch_run_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "cf_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "cf_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "cf_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -30,8 +30,29 @@ def _f(shape):
     return np.zeros(shape, dtype=np.float64, order="F")
 
 
+def case_params(cfg: CaseConfig, x, y, z, device=0):
+    """cf_params of a case (include/channel_flow.h) on the face grids x, y, z."""
+    p = _lib.CfParams()
+    p.nx, p.ny, p.nz = len(x), len(y), len(z)
+    p.precision = _lib.precision_code(cfg.precision)
+    p.cfl, p.nu, p.dPdx, p.dPdz = cfg.cfl, cfg.nu, cfg.dPdx, cfg.dPdz
+    p.omega_z = cfg.omega_z
+    p.x_mass_cte, p.y_mass_cte = cfg.x_mass_cte, cfg.y_mass_cte
+    p.les_model, p.iwall_model = cfg.les_model, cfg.iwall_model
+    p.dirichlet_nu_t = cfg.dirichlet_nu_t
+    p.jref_eqwm = cfg.jref(y) if cfg.iwall_model == 3 else 2
+    p.cs = cfg.cs
+    p.eqwm_ncv, p.eqwm_max_iter = cfg.eqwm_ncv, cfg.eqwm_max_iter
+    p.eqwm_stretch, p.eqwm_tol = cfg.eqwm_stretch, cfg.eqwm_tol
+    p.tau_mean_eqwm = cfg.dPdx
+    p.wall_surrogate = cfg.wall_surrogate
+    p.device = device
+    p.u_centerline = cfg.u_centerline
+    return p
+
+
 class ChannelFlow:
-    def __init__(self, cfg: CaseConfig, x=None, y=None, z=None, device=0):
+    def __init__(self, cfg: CaseConfig, x=None, y=None, z=None, device=0, _handle=None):
         self.lib = _lib.load()
         self.cfg = cfg
         if x is None:
@@ -40,34 +61,25 @@ class ChannelFlow:
         nx, ny, nz = len(self.x), len(self.y), len(self.z)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.nxg, self.nyg, self.nzg = nx + 1, ny + 1, nz + 1
-        p = _lib.CfParams()
-        p.nx, p.ny, p.nz = nx, ny, nz
-        p.precision = _lib.precision_code(cfg.precision)
-        p.cfl, p.nu, p.dPdx, p.dPdz = cfg.cfl, cfg.nu, cfg.dPdx, cfg.dPdz
-        p.omega_z = cfg.omega_z
-        p.x_mass_cte, p.y_mass_cte = cfg.x_mass_cte, cfg.y_mass_cte
-        p.les_model, p.iwall_model = cfg.les_model, cfg.iwall_model
-        p.dirichlet_nu_t = cfg.dirichlet_nu_t
-        p.jref_eqwm = cfg.jref(self.y) if cfg.iwall_model == 3 else 2
-        p.cs = cfg.cs
-        p.eqwm_ncv, p.eqwm_max_iter = cfg.eqwm_ncv, cfg.eqwm_max_iter
-        p.eqwm_stretch, p.eqwm_tol = cfg.eqwm_stretch, cfg.eqwm_tol
-        p.tau_mean_eqwm = cfg.dPdx
-        p.wall_surrogate = cfg.wall_surrogate
-        p.device = device
-        p.u_centerline = cfg.u_centerline
+        p = case_params(cfg, self.x, self.y, self.z, device)
         self.params = p
         self.jref_eqwm = p.jref_eqwm
+        self.yg = centres(self.y)
+        # a member of a ChannelFlowBatch borrows its handle from the batch
+        self._owned = _handle is None
+        if _handle is not None:
+            self.h = _handle
+            return
         h = ctypes.c_void_p()
         _lib.check(self.lib.cf_create(ctypes.byref(p), _lib.dptr(self.x), _lib.dptr(self.y),
                                       _lib.dptr(self.z), ctypes.byref(h)), "cf_create")
         self.h = h
-        self.yg = centres(self.y)
 
     # ---- lifetime ----
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            self.lib.cf_destroy(self.h)
+            if self._owned:
+                self.lib.cf_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
@@ -318,3 +330,49 @@ def eqwm_batch(umag, nu, h, tau_in, ncv=100, stretch=1.10, tol=1e-7, max_iter=10
                                  int(ncv), float(stretch), float(tol), int(max_iter),
                                  _lib.dptr(tau), _lib.iptr(it)), "cf_eqwm_batch")
     return tau, it
+
+
+class ChannelFlowBatch:
+    """Cases of one size class stepped together (cf_batch_*): every option
+    but nu, the forcing and the fallback stress shared, one grid.  Each step
+    kernel is launched once for all members; ``members[m]`` is a ChannelFlow
+    for everything but the step (initialise, get_state, statistics, ...)."""
+
+    def __init__(self, cfgs, x=None, y=None, z=None, device=0):
+        self.lib = _lib.load()
+        cfgs = list(cfgs)
+        if x is None:
+            x, y, z = cfgs[0].grid()
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        n = len(cfgs)
+        arr = (_lib.CfParams * n)()
+        for m, c in enumerate(cfgs):
+            arr[m] = case_params(c, x, y, z, device)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.cf_batch_create(n, arr, _lib.dptr(x), _lib.dptr(y), _lib.dptr(z), ctypes.byref(h)),
+                   "cf_batch_create")
+        self.h = h
+        self.members = []
+        for m, c in enumerate(cfgs):
+            mh = ctypes.c_void_p()
+            _lib.check(self.lib.cf_batch_member(self.h, m, ctypes.byref(mh)), "cf_batch_member")
+            self.members.append(ChannelFlow(c, x, y, z, device, _handle=mh))
+
+    def run(self, nsteps, graph=True):
+        """nsteps RK3 steps of every member (main.f90:70-73 per case)."""
+        _lib.check(self.lib.cf_batch_run_steps(self.h, int(nsteps), 1 if graph else 0), "cf_batch_run_steps")
+        if self.members:
+            self.members[0].sync()
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            for m in self.members:
+                m.h = ctypes.c_void_p()
+            self.lib.cf_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

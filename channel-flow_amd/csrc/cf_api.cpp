@@ -26,6 +26,10 @@ namespace cf {
 thread_local std::string g_err;
 
 void set_error(const std::string &m) { g_err = m; }
+Batch &launch_batch() {
+    static thread_local Batch b;
+    return b;
+}
 std::string last_error_copy() { return g_err; }
 
 #define HCK(call)                                                                              \
@@ -369,6 +373,8 @@ int Ctx<S>::init() {
     s0.norm_u = h.norm_u;
     s0.norm_v = h.norm_v;
     s0.ucl = h.prm.u_centerline;
+    s0.nu = h.prm.nu;
+    s0.dt_vis = h.dt_vis;
     HCK(hipMemcpy(sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice));
     HCK(hipStreamSynchronize(stream));
     return 0;
@@ -591,6 +597,10 @@ void Ctx<S>::step_head() {
 template <typename S>
 int Ctx<S>::run_steps(int nsteps, bool use_graph) {
     HCK(hipSetDevice(dev));
+    if (batch_member && !batch_running) {
+        set_error("a batch member is stepped through its batch (cf_batch_run_steps)");
+        return -1;
+    }
     for (int n = 0; n < nsteps; n++) {
         if (fuse_dt() && !dt_fresh) dt_launch();
         if (use_graph && !timing) {
@@ -598,6 +608,7 @@ int Ctx<S>::run_steps(int nsteps, bool use_graph) {
             const int start = cur;
             if (!ge) {
                 hipGraph_t g;
+                BatchScope bsc(bn, bcs);
                 HCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 step_head();
                 step_launch();
@@ -609,6 +620,7 @@ int Ctx<S>::run_steps(int nsteps, bool use_graph) {
             HCK(hipGraphLaunch(ge, stream));
             cur = 1 - start;  // three substeps flip the ping-pong parity
         } else {
+            BatchScope bsc(bn, bcs);
             step_head();
             step_launch();
         }
@@ -622,6 +634,7 @@ int Ctx<S>::run_steps(int nsteps, bool use_graph) {
 template <typename S>
 int Ctx<S>::prepare_graphs() {
     HCK(hipSetDevice(dev));
+    if (batch_member) { set_error("a batch member is stepped through its batch"); return -1; }
     const int keep = cur;
     for (int p = 0; p < 2; p++) {
         if (graphs[p]) continue;
@@ -640,9 +653,47 @@ int Ctx<S>::prepare_graphs() {
 
 // compute_time_step_RK3 with the dt the last compute_dt set, as the
 // reference's step uses the module's dt
+// Step a batch (this context first, then the other members, all of one
+// size class with their slabs at the byte stride cs): a member whose state
+// changed since its last step takes its own compute_dt first, on the shared
+// stream, then each kernel of the step runs once for every member.  The
+// members keep their ping-pong parity and fresh-dt flag in step with this one.
+template <typename S>
+int Ctx<S>::batch_run(const std::vector<CtxBase *> &members, int64_t cs, int nsteps, bool graph) {
+    HCK(hipSetDevice(dev));
+    const int n = (int)members.size();
+    if (n < 1 || members[0] != this) { set_error("batch_run: the first member leads the batch"); return -1; }
+    if (bn != n || bcs != cs) {
+        invalidate_graphs();
+        bn = n;
+        bcs = cs;
+    }
+    std::vector<Ctx<S> *> mem;
+    for (CtxBase *b : members) mem.push_back(static_cast<Ctx<S> *>(b));
+    for (int q = 0; q < nsteps; q++) {
+        for (Ctx<S> *c : mem) {
+            if (c->cur != cur || c->stream != stream) { set_error("batch_run: members out of step"); return -1; }
+            if (c->fuse_dt() && !c->dt_fresh) {
+                c->dt_launch();
+                c->dt_fresh = true;
+            }
+        }
+        batch_running = true;
+        const int rc = run_steps(1, graph);
+        batch_running = false;
+        if (rc) return -1;
+        for (Ctx<S> *c : mem) {
+            c->cur = cur;
+            c->dt_fresh = dt_fresh;
+        }
+    }
+    return 0;
+}
+
 template <typename S>
 int Ctx<S>::rk3_step() {
     HCK(hipSetDevice(dev));
+    if (batch_member) { set_error("a batch member is stepped through its batch (cf_batch_run_steps)"); return -1; }
     step_launch();
     HCK(hipGetLastError());
     dt_fresh = fuse_dt();
@@ -1319,7 +1370,130 @@ long long cf_footprint(const cf_params *p, const double *x, const double *y, con
 
 int cf_destroy(cf_ctx *c) {
     if (!c) return 0;
+    if (c->impl && c->impl->batch_member) {
+        set_error("cf_destroy: a batch member is destroyed with its batch (cf_batch_destroy)");
+        return -1;
+    }
     delete c;
+    return 0;
+}
+
+// ---- batches of one size class ----
+struct cf_batch {
+    int dev = 0;
+    size_t fp = 0;
+    char *base = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<cf::Slab> slabs;
+    std::vector<cf_ctx *> members;
+};
+
+static void batch_free(cf_batch *b) {
+    if (!b) return;
+    hipSetDevice(b->dev);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (cf_ctx *c : b->members) delete c;
+    b->members.clear();
+    if (b->base) hipFree(b->base);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// the options a batch's members must share (all but nu, the forcing and the
+// fallback stress), and the ones the batched step supports
+static std::string batch_class_check(const cf_params &a, const cf_params &b) {
+    // compared as bytes: the struct has no padding (4 ints, 5 doubles, 6 ints,
+    // 1 double, 2 ints, 3 doubles, 2 ints, 1 double)
+    static_assert(sizeof(cf_params) == 136, "cf_params layout changed: revisit the member comparison");
+    cf_params x = b;
+    x.nu = a.nu;
+    x.dPdx = a.dPdx;
+    x.dPdz = a.dPdz;
+    x.u_centerline = a.u_centerline;
+    x.tau_mean_eqwm = a.tau_mean_eqwm;
+    if (std::memcmp(&x, &a, sizeof(cf_params)) != 0)
+        return "members differ in an option other than nu, dPdx, dPdz, u_centerline and tau_mean_eqwm";
+    return "";
+}
+
+int cf_batch_create(int n, const cf_params *p, const double *x, const double *y, const double *z, cf_batch **out) {
+    if (n < 1 || !p || !x || !y || !z || !out) { set_error("cf_batch_create: bad arguments"); return -1; }
+    *out = nullptr;
+    const cf_params &p0 = p[0];
+    if (p0.precision != 0 || p0.les_model != 0 || p0.iwall_model != 0 || p0.y_mass_cte != 0 ||
+        p0.wall_surrogate != 0) {
+        set_error("cf_batch_create: the batched step takes FP64 DNS cases (no eddy viscosity, no wall model, "
+                  "no y mass flow)");
+        return -1;
+    }
+    std::vector<cf::Host> hosts;
+    hosts.reserve(n);
+    for (int m = 0; m < n; m++) {
+        if (m > 0) {
+            const std::string e = batch_class_check(p0, p[m]);
+            if (!e.empty()) { set_error("cf_batch_create: " + e); return -1; }
+        }
+        hosts.emplace_back(p[m], x, y, z);
+        const std::string v = hosts.back().validate();
+        if (!v.empty()) { set_error("cf_batch_create: " + v); return -1; }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("cf_batch_create: no HIP device"); return -1; }
+    if (p0.device < 0 || p0.device >= ndev) { set_error("cf_batch_create: bad device ordinal"); return -1; }
+    const size_t fp = cf::ctx_footprint(hosts[0]);
+    if (fp == 0) return -1;
+    for (int m = 1; m < n; m++)
+        if (cf::ctx_footprint(hosts[m]) != fp) { set_error("cf_batch_create: member footprints differ"); return -1; }
+    cf_batch *b = new cf_batch;
+    b->dev = p0.device;
+    b->fp = fp;
+    b->slabs.resize(n);
+    if (hipSetDevice(b->dev) != hipSuccess || hipMalloc(&b->base, fp * (size_t)n) != hipSuccess) {
+        b->base = nullptr;
+        batch_free(b);
+        set_error("cf_batch_create: device allocation failed");
+        return -1;
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        b->stream = nullptr;
+        batch_free(b);
+        set_error("cf_batch_create: stream creation failed");
+        return -1;
+    }
+    for (int m = 0; m < n; m++) {
+        b->slabs[m].base = b->base + (size_t)m * fp;
+        b->slabs[m].cap = fp;
+        auto impl = cf::make_ctx(hosts[m], &b->slabs[m], b->stream);
+        if (!impl) { batch_free(b); return -1; }
+        if (b->slabs[m].used != fp) {
+            batch_free(b);
+            set_error("cf_batch_create: member layout differs from the footprint");
+            return -1;
+        }
+        impl->batch_member = true;
+        cf_ctx *c = new cf_ctx;
+        c->impl = std::move(impl);
+        b->members.push_back(c);
+    }
+    *out = b;
+    return 0;
+}
+
+int cf_batch_member(cf_batch *b, int m, cf_ctx **ctx) {
+    if (!b || !ctx || m < 0 || m >= (int)b->members.size()) { set_error("cf_batch_member: bad arguments"); return -1; }
+    *ctx = b->members[m];
+    return 0;
+}
+
+int cf_batch_run_steps(cf_batch *b, int nsteps, int use_graph) {
+    if (!b || nsteps < 0) { set_error("cf_batch_run_steps: bad arguments"); return -1; }
+    std::vector<cf::CtxBase *> mem;
+    for (cf_ctx *c : b->members) mem.push_back(c->impl.get());
+    return mem[0]->batch_run(mem, (int64_t)b->fp, nsteps, use_graph != 0);
+}
+
+int cf_batch_destroy(cf_batch *b) {
+    batch_free(b);
     return 0;
 }
 

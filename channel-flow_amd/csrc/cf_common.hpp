@@ -51,6 +51,8 @@ struct Scalars {
     double corr_v;    // last y mass-flow correction (V increment, mass_flow.f90:174)
     double dt_next;   // compute_dt of the state at the end of the last step
     double ucl;       // centreline target Ucenterline_x (x_mass_cte = 2)
+    double nu;        // kinematic viscosity of the case (read by step kernels of a batch)
+    double dt_vis;    // viscous limit of compute_dt (time_integration.f90:205-212)
     double pad[3];    // [0] last bulk (or centreline) mean of U, [1] last bulk mean of V
     int n_nan_eqwm;
     int n_iter_eqwm;
@@ -69,6 +71,19 @@ struct DtParts {
     const Scalars *sc;
     int add_corr;
 };
+
+// A batch of cases of one size class keeps every per-case buffer at one byte
+// stride cs from the first case's: the step kernels take the case index from
+// blockIdx.y and move each per-case pointer by its offset (cs = 0: one case).
+// Grid metrics, transform tables and the y operator are the class's own and
+// are read from the first case.
+template <typename P>
+__device__ __forceinline__ P *case_ptr(P *p, int64_t cs) {
+    return (p && cs) ? reinterpret_cast<P *>(reinterpret_cast<char *>(const_cast<void *>(
+                                                 static_cast<const void *>(p))) +
+                                             (int64_t)blockIdx.y * cs)
+                     : p;
+}
 
 __host__ __device__ inline int64_t i3(int n1, int n2, int i, int j, int k) {
     return (int64_t)(i - 1) + (int64_t)n1 * ((int64_t)(j - 1) + (int64_t)n2 * (int64_t)(k - 1));

@@ -331,8 +331,10 @@ __global__ void __launch_bounds__(WG)
 k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
            const S *__restrict__ W, Cx<T> *__restrict__ spec, FFTPlan pM,
            const Cx<T> *__restrict__ twM, const Cx<T> *__restrict__ twN, int rb,
-           const double *__restrict__ ysrc) {
+           const double *__restrict__ ysrc, int64_t cs) {
     extern __shared__ unsigned char smem_raw[];
+    U = case_ptr(U, cs); V = case_ptr(V, cs); W = case_ptr(W, cs);
+    spec = case_ptr(spec, cs); ysrc = case_ptr(ysrc, cs);
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, ls = fft_line<T>(M);
     const int nrows = d.nyp * d.nzp;
@@ -414,8 +416,9 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
 template <typename T, int NC, int SIGNC>
 __global__ void __launch_bounds__(512)
 k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ twZ, int bcols,
-       int sign) {
+       int sign, int64_t cs) {
     extern __shared__ unsigned char smem_raw[];
+    spec = case_ptr(spec, cs);
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int n = NC ? NC : pZ.n, ls = fft_line<T>(n) + 1;
     const int64_t ncols = (int64_t)d.nyp * d.mh;
@@ -466,8 +469,10 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const double *__restrict__ kxx,
               const double *__restrict__ kzz, const double *__restrict__ dd, const double *__restrict__ dl,
-              const double *__restrict__ du) {
+              const double *__restrict__ du, int64_t cs) {
     extern __shared__ double sco[];  // dd | dl | du
+    spec = case_ptr(spec, cs);
+    ckpt = case_ptr(ckpt, cs);
     const int nzp = d.nzp, nyp = d.nyp;
     double *s_dd = sco, *s_dl = sco + nyp, *s_du = sco + 2 * nyp;
     for (int e = threadIdx.x; e < nyp; e += blockDim.x) { s_dd[e] = dd[e]; s_dl[e] = dl[e]; s_du[e] = du[e]; }
@@ -482,8 +487,8 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
     const int64_t js = 2 * (int64_t)nmodes;          // row stride in T
     T *col = reinterpret_cast<T *>(spec) + t;        // this lane's part of mode q, row 0
     const int nb = (nyp + TH_LB - 1) / TH_LB;
-    const int64_t cs = 2 * (int64_t)nmodes;          // checkpoint stride per block
-    double *ck_c = ckpt, *ck_d = ckpt + (int64_t)nb * cs;
+    const int64_t cks = 2 * (int64_t)nmodes;         // checkpoint stride per block
+    double *ck_c = ckpt, *ck_d = ckpt + (int64_t)nb * cks;
     auto Dj = [&](int j) {
         const double D = (s_dd[j] + kx2) + kz2;
         return (j == 0 && zm) ? 1.5 * D : D;
@@ -511,22 +516,22 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
                 cp = (j < nyp - 1) ? s_du[j] * r : 0.0;
             }
         }
-        ck_c[(int64_t)b * cs + t] = cp;
-        ck_d[(int64_t)b * cs + t] = dp;
+        ck_c[(int64_t)b * cks + t] = cp;
+        ck_d[(int64_t)b * cks + t] = dp;
 #pragma unroll
         for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
     }
     // back substitution, block by block from the top
     double x = dp;  // row nyp-1: x = d'
     loadblk(nb - 1, cur);
-    double c0n = nb >= 2 ? ck_c[(int64_t)(nb - 2) * cs + t] : 0.0;
-    double d0n = nb >= 2 ? ck_d[(int64_t)(nb - 2) * cs + t] : 0.0;
+    double c0n = nb >= 2 ? ck_c[(int64_t)(nb - 2) * cks + t] : 0.0;
+    double d0n = nb >= 2 ? ck_d[(int64_t)(nb - 2) * cks + t] : 0.0;
     for (int b = nb - 1; b >= 0; b--) {
         double cprev = c0n, dprev = d0n;  // c', d' at the row before the block
         if (b > 0) {
             loadblk(b - 1, nxt);
-            c0n = b >= 2 ? ck_c[(int64_t)(b - 2) * cs + t] : 0.0;
-            d0n = b >= 2 ? ck_d[(int64_t)(b - 2) * cs + t] : 0.0;
+            c0n = b >= 2 ? ck_c[(int64_t)(b - 2) * cks + t] : 0.0;
+            d0n = b >= 2 ? ck_d[(int64_t)(b - 2) * cks + t] : 0.0;
         }
         double c[TH_LB], dv[TH_LB];
 #pragma unroll
@@ -580,9 +585,10 @@ template <typename T, int MS>
 __global__ void __launch_bounds__(512)
 k_thomas(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ kxx, const double *__restrict__ kzz,
          const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
-         int KZ, int P) {
+         int KZ, int P, int64_t cs) {
     // [8 fields per (segment, kz), field-major: sh[f * NT + slot]] | dd | dl | du | kxx | kzz
     extern __shared__ double sh[];
+    spec = case_ptr(spec, cs);
     const int nzp = d.nzp, nyp = d.nyp, mh = d.mh;
     const int nkzb = (nzp + KZ - 1) / KZ, ntiles = mh * nkzb;
     const int tid = threadIdx.x, kzl = tid % KZ, sg = tid / KZ;
@@ -728,8 +734,11 @@ __global__ void __launch_bounds__(WG)
 k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__ phi,
               S *__restrict__ U, FFTPlan pM, const Cx<T> *__restrict__ twM,
               const Cx<T> *__restrict__ twN, int rb, double norm, const Scalars *__restrict__ csc, DtParts dtp,
-              int gcopy) {
+              int gcopy, int64_t cs) {
     extern __shared__ unsigned char smem_raw[];
+    spec = case_ptr(spec, cs); phi = case_ptr(phi, cs); U = case_ptr(U, cs); csc = case_ptr(csc, cs);
+    dtp.pu = case_ptr(dtp.pu, cs); dtp.pv = case_ptr(dtp.pv, cs); dtp.pw = case_ptr(dtp.pw, cs);
+    dtp.ubar = case_ptr(dtp.ubar, cs); dtp.sc = case_ptr(dtp.sc, cs);
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
     const int nrows = d.nyp * d.nzp;
@@ -954,22 +963,23 @@ void launch_div_fftx(const Dims &d, const Metrics &m, const S *U, const S *V, co
     const int rb = fft_rows_per_wg(pM.n, pM.n, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n) + pM.n) * sizeof(Cx<T>) + (size_t)rb * sizeof(XRow);
-    const dim3 grid((nrows + rb - 1) / rb);
+    const dim3 grid((nrows + rb - 1) / rb, launch_batch().n);
+    const int64_t cs = launch_batch().cs;
     if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_div_fftx<S, T, 512, 0>), grid, dim3(512), shm, st, d, m, U, V, W, spec, pM, twM, twN,
-                           rb, ysrc);
+                           rb, ysrc, cs);
         return;
     }
 #define CF_DIVX(MC)                                                                                        \
     case MC:                                                                                               \
         hipLaunchKernelGGL((k_div_fftx<S, T, 256, MC>), grid, dim3(256), shm, st, d, m, U, V, W, spec, pM, \
-                           twM, twN, rb, ysrc);                                                            \
+                           twM, twN, rb, ysrc, cs);                                                            \
         return;
     switch (fft_static_sizes() ? pM.n : 0) {
         CF_X_LENGTHS(CF_DIVX)
         default:
             hipLaunchKernelGGL((k_div_fftx<S, T, 256, 0>), grid, dim3(256), shm, st, d, m, U, V, W, spec, pM, twM,
-                               twN, rb, ysrc);
+                               twN, rb, ysrc, cs);
     }
 #undef CF_DIVX
 }
@@ -980,18 +990,19 @@ void launch_fftz(const Dims &d, Cx<T> *spec, const FFTPlan &pZ, const Cx<T> *twZ
     const int b = fft_cols_per_wg(pZ.n, (int)sizeof(Cx<T>));
     const int64_t ncols = (int64_t)d.nyp * d.mh;
     const size_t shm = ((size_t)b * (fft_line<T>(pZ.n) + 1) + pZ.n) * sizeof(Cx<T>);
-    const dim3 grid((unsigned)((ncols + b - 1) / b));
+    const dim3 grid((unsigned)((ncols + b - 1) / b), launch_batch().n);
+    const int64_t cs = launch_batch().cs;
 #define CF_FFTZ(NC)                                                                                           \
     case NC:                                                                                                  \
         if (sign < 0)                                                                                         \
-            hipLaunchKernelGGL((k_fftz<T, NC, -1>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign);     \
+            hipLaunchKernelGGL((k_fftz<T, NC, -1>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign, cs);     \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_fftz<T, NC, 1>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign);      \
+            hipLaunchKernelGGL((k_fftz<T, NC, 1>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign, cs);      \
         return;
     switch (fft_static_sizes() ? pZ.n : 0) {
         CF_Z_LENGTHS(CF_FFTZ)
         default:
-            hipLaunchKernelGGL((k_fftz<T, 0, 0>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign);
+            hipLaunchKernelGGL((k_fftz<T, 0, 0>), grid, dim3(512), shm, st, d, spec, pZ, twZ, b, sign, cs);
     }
 #undef CF_FFTZ
 }
@@ -1005,8 +1016,8 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
                    const double *dd, const double *dl, const double *du, hipStream_t st) {
     const int nmodes = d.mh * d.nzp;
     if (cscr && thomas_uses_scratch(nmodes)) {
-        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((2 * nmodes + 255) / 256), dim3(256), 3 * d.nyp * sizeof(double),
-                           st, d, spec, cscr, kxx, kzz, dd, dl, du);
+        hipLaunchKernelGGL((k_thomas_lane<T>), dim3((2 * nmodes + 255) / 256, launch_batch().n), dim3(256),
+                           3 * d.nyp * sizeof(double), st, d, spec, cscr, kxx, kzz, dd, dl, du, launch_batch().cs);
         return;
     }
     int KZ, P;
@@ -1024,11 +1035,11 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
     }
     const int grid = std::min(d.mh * nkzb, ncu * std::max(1, env_int("CF_TH_WGS_PER_CU", 2)));
     if (ms == 4)
-        hipLaunchKernelGGL((k_thomas<T, 4>), dim3(grid), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl, du, KZ,
-                           P);
+        hipLaunchKernelGGL((k_thomas<T, 4>), dim3(grid, launch_batch().n), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
+                           du, KZ, P, launch_batch().cs);
     else
-        hipLaunchKernelGGL((k_thomas<T, 8>), dim3(grid), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl, du, KZ,
-                           P);
+        hipLaunchKernelGGL((k_thomas<T, 8>), dim3(grid, launch_batch().n), dim3(threads), shm, st, d, spec, kxx, kzz, dd, dl,
+                           du, KZ, P, launch_batch().cs);
 }
 
 int ifftx_blocks(const Dims &d, int esize) {
@@ -1045,22 +1056,23 @@ void launch_ifftx_proju(const Dims &d, const Metrics &m, const Cx<T> *spec, P *p
     const int rb = fft_rows_per_wg(pM.n, pM.n + 1, (int)sizeof(Cx<T>));
     const int nrows = d.nyp * d.nzp;
     const size_t shm = ((size_t)rb * fft_line<T>(pM.n + 1) + pM.n) * sizeof(Cx<T>) + (size_t)rb * sizeof(XRow);
-    const dim3 grid((nrows + rb - 1) / rb);
+    const dim3 grid((nrows + rb - 1) / rb, launch_batch().n);
+    const int64_t cs = launch_batch().cs;
     if (fft_x_wg() == 512) {
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 512, 0>), grid, dim3(512), shm, st, d, m, spec, phi, U, pM, twM,
-                           twN, rb, norm, corr, dtp, gcopy);
+                           twN, rb, norm, corr, dtp, gcopy, cs);
         return;
     }
 #define CF_IFFTX(MC)                                                                                          \
     case MC:                                                                                                  \
         hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, MC>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM, \
-                           twM, twN, rb, norm, corr, dtp, gcopy);                                                    \
+                           twM, twN, rb, norm, corr, dtp, gcopy, cs);                                                    \
         return;
     switch (fft_static_sizes() ? pM.n : 0) {
         CF_X_LENGTHS(CF_IFFTX)
         default:
             hipLaunchKernelGGL((k_ifftx_proju<S, T, P, 256, 0>), grid, dim3(256), shm, st, d, m, spec, phi, U, pM,
-                               twM, twN, rb, norm, corr, dtp, gcopy);
+                               twM, twN, rb, norm, corr, dtp, gcopy, cs);
     }
 #undef CF_IFFTX
 }

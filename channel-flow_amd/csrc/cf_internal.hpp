@@ -97,6 +97,11 @@ struct CtxBase {
     virtual int attach_mlp(Mlp *m) = 0;
     virtual int wall_fallbacks() = 0;
     virtual int prepare_graphs() = 0;
+    // A batch member (cf_batch) is stepped only by batch_run on the first
+    // member: each step kernel is launched once for all members, whose slabs
+    // sit at the byte stride cs.
+    bool batch_member = false;
+    virtual int batch_run(const std::vector<CtxBase *> &members, int64_t cs, int nsteps, bool graph) = 0;
 };
 
 // a context for one case: device buffers from `slab` when given (else
@@ -148,6 +153,9 @@ struct Ctx : CtxBase {
     Cx<S> *twM = nullptr, *twN = nullptr, *twZ = nullptr;
     double *alpha_x = nullptr, *alpha_z = nullptr, *azt = nullptr, *tau_init = nullptr;
     hipGraphExec_t graphs[2] = {nullptr, nullptr};
+    int bn = 1;            // cases the step kernels launch for (batch_run)
+    int64_t bcs = 0;       // their byte stride
+    bool batch_running = false;
     // optional learned wall-model hook: fills predicted stresses and a mask
     std::function<void(Ctx<S> *, S *const *, const double **, const int **)> surrogate;
 
@@ -223,6 +231,7 @@ struct Ctx : CtxBase {
     int attach_mlp(Mlp *m) override;
     int wall_fallbacks() override;
     int prepare_graphs() override;
+    int batch_run(const std::vector<CtxBase *> &members, int64_t cs, int nsteps, bool graph) override;
 };
 
 }  // namespace cf

@@ -14,6 +14,21 @@ struct FFTPlan {
 // radix 8 in the plan of the x transforms (the z transforms always use it)
 constexpr bool FFT_X_RADIX8 = true;
 
+// ---- batches (several cases of one size class stepped by each launch) ----
+// The launchers of the step kernels launch launch_batch().n copies of their
+// grid along y (one per case) and pass the byte stride cs between cases;
+// BatchScope sets them for the launches of one batched step on this thread.
+struct Batch {
+    int n = 1;
+    int64_t cs = 0;
+};
+Batch &launch_batch();
+struct BatchScope {
+    Batch prev;
+    BatchScope(int n, int64_t cs) : prev(launch_batch()) { launch_batch() = Batch{n, cs}; }
+    ~BatchScope() { launch_batch() = prev; }
+};
+
 // ---- dynamic Smagorinsky (cf_dsm.hip) ----
 struct DsmWork {
     double *Q;     // 10 centre arrays (nxg*nyg*nzg each): centre velocities, |S|Sij, |S|

@@ -226,8 +226,13 @@ k_rhs_rk(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
          S *__restrict__ Vo, S *__restrict__ Wo, S *__restrict__ Gu, S *__restrict__ Gv,
          S *__restrict__ Gw, const Scalars *__restrict__ sc, const double *__restrict__ ubar,
          double nu_d, double om_d, double a_new, double a_g, int tiles_x, int tiles_y,
-         int nchunks, int ntiles, double *__restrict__ mfpart, int mf_mode) {
+         int nchunks, int ntiles, double *__restrict__ mfpart, int mf_mode, int64_t cs) {
     constexpr int LX = RK_TX + 2, PL = LX * (TY + 2), NT_ = RK_TX * TY;
+    U = case_ptr(U, cs); V = case_ptr(V, cs); W = case_ptr(W, cs); NT = case_ptr(NT, cs);
+    Uo = case_ptr(Uo, cs); Vo = case_ptr(Vo, cs); Wo = case_ptr(Wo, cs);
+    Gu = case_ptr(Gu, cs); Gv = case_ptr(Gv, cs); Gw = case_ptr(Gw, cs);
+    sc = case_ptr(sc, cs); ubar = case_ptr(ubar, cs); mfpart = case_ptr(mfpart, cs);
+    if (cs) nu_d = sc->nu;
     constexpr int NPF = (PL + NT_ - 1) / NT_;  // plane elements per thread
     extern __shared__ unsigned char smem_raw[];
     R *ring = reinterpret_cast<R *>(smem_raw);
@@ -588,7 +593,9 @@ template <typename S, typename P, int KM, bool GC>
 __global__ void __launch_bounds__(256)
 k_project_vw(Dims d, Metrics m, const P *__restrict__ phi, S *__restrict__ V,
              S *__restrict__ W, int tiles_x, int tiles_y, int ntiles, int kc, double *__restrict__ dtv,
-             double *__restrict__ dtw) {
+             double *__restrict__ dtw, int64_t cs) {
+    phi = case_ptr(phi, cs); V = case_ptr(V, cs); W = case_ptr(W, cs);
+    dtv = case_ptr(dtv, cs); dtw = case_ptr(dtw, cs);
     const int t = xcd_remap(blockIdx.x, ntiles);
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tc = t / (tiles_x * tiles_y);
     const int i = 2 + tx * 64 + (threadIdx.x & 63);
@@ -723,8 +730,10 @@ k_massflow_u_partial(Dims d, Metrics m, const S *__restrict__ U, const double *_
 // final: Int_U over k = 2..nzg-1, i.e. the unique planes plus the duplicate
 // plane nzg-1 (= plane 2 after the periodic copy, mass_flow.f90:116)
 __global__ void k_massflow_u_final(const double *__restrict__ part, int nparts,
-                                   Scalars *sc, int apply) {
+                                   Scalars *sc, int apply, int64_t cs) {
     __shared__ double sh[8];
+    part = case_ptr(part, cs);
+    sc = case_ptr(sc, cs);
     double acc = 0.0;
     for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += part[q];
     acc = block_reduce(acc, OpAdd(), sh);
@@ -807,8 +816,11 @@ k_massflow_v_partial(Dims d, const double *__restrict__ y, const S *__restrict__
 // Qflow_y_0 - bulk, dPdy = corr/dt (time_integration.f90:166-169).
 enum { MF_CENTERLINE_MEAN = 0, MF_CENTERLINE = 1, MF_V = 2, MF_V_APPLY = 3 };
 __global__ void k_flow_final(const double *__restrict__ part, int nparts, Scalars *sc,
-                             const double *__restrict__ ubar, int jc, double count, int mode) {
+                             const double *__restrict__ ubar, int jc, double count, int mode, int64_t cs) {
     __shared__ double sh[8];
+    part = case_ptr(part, cs);
+    sc = case_ptr(sc, cs);
+    ubar = case_ptr(ubar, cs);
     double acc = 0.0;
     for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += part[q];
     acc = block_reduce(acc, OpAdd(), sh);
@@ -970,8 +982,10 @@ constexpr int DTF_NB = 128;  // workgroups of the first level
 __global__ void __launch_bounds__(256)
 k_dt_fused_mid(Dims d, const double *__restrict__ pu, int nu_, const double *__restrict__ pv,
                const double *__restrict__ pw, int nvw, const double *__restrict__ dyc, const void *__restrict__ Vp,
-               int vfp32, double *__restrict__ mid) {
+               int vfp32, double *__restrict__ mid, int64_t cs) {
     __shared__ double sh[8];
+    pu = case_ptr(pu, cs); pv = case_ptr(pv, cs); pw = case_ptr(pw, cs);
+    Vp = case_ptr(Vp, cs); mid = case_ptr(mid, cs);
     const int g = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
     double lu = 0.0, lv = 0.0, lw = 0.0, vmin = INFINITY;
     for (int q = g; q < nu_; q += gs) lu = fmax(lu, pu[q]);
@@ -996,8 +1010,11 @@ k_dt_fused_mid(Dims d, const double *__restrict__ pu, int nu_, const double *__r
 }
 __global__ void __launch_bounds__(256)
 k_dt_fused_final(Dims d, const double *__restrict__ mid, const double *__restrict__ dyc, Scalars *sc, double cfl,
-                 double dt_vis) {
+                 double dt_vis, int64_t cs) {
     __shared__ double sh[8];
+    mid = case_ptr(mid, cs);
+    sc = case_ptr(sc, cs);
+    if (cs) dt_vis = sc->dt_vis;
     double lu = 0.0, lv = 0.0, lw = 0.0, vmin = INFINITY;
     for (int q = threadIdx.x; q < DTF_NB; q += blockDim.x) {
         lu = fmax(lu, mid[4 * q]);
@@ -1019,19 +1036,24 @@ k_dt_fused_final(Dims d, const double *__restrict__ mid, const double *__restric
         sc->dt_conv[0] = lu; sc->dt_conv[1] = lv; sc->dt_conv[2] = lw;
     }
 }
-__global__ void k_dt_take_next(Scalars *sc) { sc->dt = sc->dt_next; }
+__global__ void k_dt_take_next(Scalars *sc, int64_t cs) {
+    sc = case_ptr(sc, cs);
+    sc->dt = sc->dt_next;
+}
 
 size_t dt_fused_mid_doubles() { return 4 * (size_t)DTF_NB; }
 template <typename S>
 void launch_dt_fused_final(const Dims &d, const DtParts &p, int nu_, int nvw, const double *dyc, const S *V,
                            Scalars *sc, double cfl, double dt_vis, hipStream_t st) {
     double *mid = p.pw + nvw;  // after the partial arrays (dt_fused_mid_doubles of room)
-    hipLaunchKernelGGL(k_dt_fused_mid, dim3(DTF_NB), dim3(256), 0, st, d, p.pu, nu_, p.pv, p.pw, nvw, dyc,
-                       (const void *)V, sizeof(S) == 4 ? 1 : 0, mid);
-    hipLaunchKernelGGL(k_dt_fused_final, dim3(1), dim3(256), 0, st, d, mid, dyc, sc, cfl, dt_vis);
+    const int nb = launch_batch().n;
+    const int64_t cs = launch_batch().cs;
+    hipLaunchKernelGGL(k_dt_fused_mid, dim3(DTF_NB, nb), dim3(256), 0, st, d, p.pu, nu_, p.pv, p.pw, nvw, dyc,
+                       (const void *)V, sizeof(S) == 4 ? 1 : 0, mid, cs);
+    hipLaunchKernelGGL(k_dt_fused_final, dim3(1, nb), dim3(256), 0, st, d, mid, dyc, sc, cfl, dt_vis, cs);
 }
 void launch_dt_take_next(Scalars *sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_dt_take_next, dim3(1), dim3(1), 0, st, sc);
+    hipLaunchKernelGGL(k_dt_take_next, dim3(1, launch_batch().n), dim3(1), 0, st, sc, launch_batch().cs);
 }
 template void launch_dt_fused_final<double>(const Dims &, const DtParts &, int, int, const double *,
                                             const double *, Scalars *, double, double, hipStream_t);
@@ -1299,10 +1321,12 @@ void launch_rhs_rk(const Dims &d, const Metrics &m, const S *U, const S *V, cons
     const int tx = (d.nxp + RK_TX - 1) / RK_TX, ty = (d.nyp + TYv - 1) / TYv;
     const int nch = (d.nzp + RK_KC - 1) / RK_KC, nt = tx * ty * nch;
     const size_t shm = (size_t)(nut ? 4 : 3) * RK_NSLOT * (RK_TX + 2) * (TYv + 2) * sizeof(R);
+    const int nb = launch_batch().n;
+    const int64_t cs = launch_batch().cs;
 #define L(NUTB, ROTB, SUBV, TYB)                                                                           \
-    hipLaunchKernelGGL((k_rhs_rk<S, R, NUTB, ROTB, SUBV, TYB>), dim3(nt), dim3(RK_TX, TYB), shm, st, d, m, U, V, \
+    hipLaunchKernelGGL((k_rhs_rk<S, R, NUTB, ROTB, SUBV, TYB>), dim3(nt, nb), dim3(RK_TX, TYB), shm, st, d, m, U, V, \
                        W, NT, Uo, Vo, Wo, Gu, Gv, Gw, sc, ubar, nu, om, a_new, a_g, tx, ty, nch, nt,  \
-                       mfpart, mf_mode)
+                       mfpart, mf_mode, cs)
     if (sub == 1) {
         if (nut) { if (rot) L(true, true, 1, 4); else L(true, false, 1, 4); }
         else { if (rot) L(false, true, 1, 8); else L(false, false, 1, 8); }
@@ -1373,12 +1397,14 @@ void launch_project_vw(const Dims &d, const Metrics &m, const P *phi, S *V, S *W
     // minima tracked for as many planes as the default march has (FP64: 1,
     // FP32: 8)
     constexpr int KM = sizeof(S) == 4 ? 8 : 1;
+    const dim3 grid(nt, launch_batch().n);
+    const int64_t cs = launch_batch().cs;
     if (gcopy)
-        hipLaunchKernelGGL((k_project_vw<S, P, KM, true>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc,
-                           dtv, dtw);
+        hipLaunchKernelGGL((k_project_vw<S, P, KM, true>), grid, dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt, kc,
+                           dtv, dtw, cs);
     else
-        hipLaunchKernelGGL((k_project_vw<S, P, KM, false>), dim3(nt), dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt,
-                           kc, dtv, dtw);
+        hipLaunchKernelGGL((k_project_vw<S, P, KM, false>), grid, dim3(256), 0, st, d, m, phi, V, W, tx, ty, nt,
+                           kc, dtv, dtw, cs);
 }
 
 template <typename S>
@@ -1386,7 +1412,7 @@ void launch_massflow(const Dims &d, const Metrics &m, const S *U, double *part, 
                      int apply, hipStream_t st) {
     const int nb = massflow_parts(d);
     hipLaunchKernelGGL((k_massflow_u_partial<S>), dim3(nb), dim3(256), 0, st, d, m, U, nullptr, part);
-    hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, part, nb, sc, apply);
+    hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, part, nb, sc, apply, (int64_t)0);
 }
 static int massflow_u_parts(const Dims &d) {
     const int64_t n = (int64_t)(d.nx - 2) * (d.nzg - 2);
@@ -1402,11 +1428,13 @@ int massflow_parts(const Dims &d) { return std::max(massflow_u_parts(d), massflo
 // 2: centreline); writes corr and dPdx
 void launch_massflow_fused_final(const Dims &d, const double *mfpart, int nparts, int mf_mode, const double *ubar,
                                  Scalars *sc, hipStream_t st) {
+    const dim3 grid(1, launch_batch().n);
+    const int64_t cs = launch_batch().cs;
     if (mf_mode == 1)
-        hipLaunchKernelGGL(k_massflow_u_final, dim3(1), dim3(512), 0, st, mfpart, nparts, sc, 1);
+        hipLaunchKernelGGL(k_massflow_u_final, grid, dim3(512), 0, st, mfpart, nparts, sc, 1, cs);
     else
-        hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(512), 0, st, mfpart, nparts, sc, ubar, d.ny / 2 + 1,
-                           (double)((d.nx - 2) * (d.nzg - 2)), (int)MF_CENTERLINE);
+        hipLaunchKernelGGL(k_flow_final, grid, dim3(512), 0, st, mfpart, nparts, sc, ubar, d.ny / 2 + 1,
+                           (double)((d.nx - 2) * (d.nzg - 2)), (int)MF_CENTERLINE, cs);
 }
 
 template <typename S>
@@ -1415,7 +1443,8 @@ void launch_centerline_u(const Dims &d, const S *U, const double *ubar, double *
     const int nb = massflow_u_parts(d), jc = d.ny / 2 + 1;
     hipLaunchKernelGGL((k_centerline_partial<S>), dim3(nb), dim3(256), 0, st, d, U, jc, part);
     hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(512), 0, st, part, nb, sc, ubar, jc,
-                       (double)((d.nx - 2) * (d.nzg - 2)), apply ? (int)MF_CENTERLINE : (int)MF_CENTERLINE_MEAN);
+                       (double)((d.nx - 2) * (d.nzg - 2)), apply ? (int)MF_CENTERLINE : (int)MF_CENTERLINE_MEAN,
+                       (int64_t)0);
 }
 template <typename S>
 void launch_massflow_v(const Dims &d, const double *y, const S *V, double *part, Scalars *sc, int apply,
@@ -1423,7 +1452,7 @@ void launch_massflow_v(const Dims &d, const double *y, const S *V, double *part,
     const int nb = massflow_v_parts(d);
     hipLaunchKernelGGL((k_massflow_v_partial<S>), dim3(nb), dim3(256), 0, st, d, y, V, part);
     hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(512), 0, st, part, nb, sc, (const double *)nullptr, 0, 0.0,
-                       apply ? (int)MF_V_APPLY : (int)MF_V);
+                       apply ? (int)MF_V_APPLY : (int)MF_V, (int64_t)0);
 }
 template <typename S>
 void launch_add_corr_v(const Dims &d, S *V, const Scalars *sc, hipStream_t st) {
@@ -1631,9 +1660,15 @@ __global__ void k_reset_counters(Scalars *sc) {
     sc->n_iter_eqwm = 0;
     sc->n_fallback_mlp = 0;
 }
-__global__ void k_dt_fixed(Scalars *sc, double cfl) { sc->dt = -cfl; }
+__global__ void k_dt_fixed(Scalars *sc, double cfl, int64_t cs) {
+    sc = case_ptr(sc, cs);
+    sc->dt = -cfl;
+}
 // t = to + rk_t(3)*dt with rk_t(3) = 1 (time_integration.f90:146)
-__global__ void k_advance_time(Scalars *sc) { sc->t = sc->t + 1.0 * sc->dt; }
+__global__ void k_advance_time(Scalars *sc, int64_t cs) {
+    sc = case_ptr(sc, cs);
+    sc->t = sc->t + 1.0 * sc->dt;
+}
 
 // mass-flow correction in the deviation storage: U = ubar(j) + U', and the
 // correction adds corr to U on the interior (time_integration.f90:155), i.e.
@@ -1658,10 +1693,10 @@ void launch_reset_counters(Scalars *sc, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, sc);
 }
 void launch_dt_final_only(Scalars *sc, double cfl, hipStream_t st) {
-    hipLaunchKernelGGL(k_dt_fixed, dim3(1), dim3(1), 0, st, sc, cfl);
+    hipLaunchKernelGGL(k_dt_fixed, dim3(1, launch_batch().n), dim3(1), 0, st, sc, cfl, launch_batch().cs);
 }
 void launch_advance_time(Scalars *sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, st, sc);
+    hipLaunchKernelGGL(k_advance_time, dim3(1, launch_batch().n), dim3(1), 0, st, sc, launch_batch().cs);
 }
 
 }  // namespace cf

@@ -194,6 +194,25 @@ int cf_stats_reset(cf_ctx *c);
  * serving arena admits cases by this figure).  No device is touched. */
 long long cf_footprint(const cf_params *p, const double *x, const double *y, const double *z);
 
+/* ---- batches of one size class ----
+ * n cases that share every option but nu, dPdx, dPdz, u_centerline and
+ * tau_mean_eqwm (same grid x, y, z), with their device buffers in one
+ * allocation at a fixed stride, stepped together: each kernel of the RK3
+ * step is launched once for all n cases (the case index is the launch
+ * grid's second dimension), so small grids fill the GPU.  The reference has
+ * no counterpart (one case is one run of main.f90); each member computes
+ * the same bits as the same case alone.  The batched step takes FP64 DNS
+ * cases (no eddy viscosity, wall model or y mass flow).
+ * cf_batch_member gives member m as an ordinary context for everything but
+ * the step (set_state, get_state, statistics, ...); it is owned by the batch
+ * and is stepped only by cf_batch_run_steps. */
+typedef struct cf_batch cf_batch;
+int cf_batch_create(int n, const cf_params *p, const double *x, const double *y, const double *z,
+                    cf_batch **out);
+int cf_batch_member(cf_batch *b, int m, cf_ctx **ctx);
+int cf_batch_run_steps(cf_batch *b, int nsteps, int use_graph);
+int cf_batch_destroy(cf_batch *b);
+
 /* ---- serving ----
  * Many cases of different grids, Reynolds numbers and models on one GPU
  * with continuous batching.  The reference has no counterpart: there one
