@@ -375,6 +375,7 @@ int Ctx<S>::init() {
     s0.ucl = h.prm.u_centerline;
     s0.nu = h.prm.nu;
     s0.dt_vis = h.dt_vis;
+    s0.tau_mean = h.prm.tau_mean_eqwm;
     HCK(hipMemcpy(sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice));
     HCK(hipStreamSynchronize(stream));
     return 0;
@@ -1253,6 +1254,7 @@ int Ctx<S>::compute_wall_model(int *n_nan, int *n_iter) {
 template <typename S>
 int Ctx<S>::attach_mlp(Mlp *m) {
     HCK(hipSetDevice(dev));
+    if (batch_member && m) { set_error("the learned wall model is not taken by a batch member"); return -1; }
     invalidate_graphs();
     if (!m) { surrogate = nullptr; return 0; }
     const Dims &d = h.d;
@@ -1420,10 +1422,9 @@ int cf_batch_create(int n, const cf_params *p, const double *x, const double *y,
     if (n < 1 || !p || !x || !y || !z || !out) { set_error("cf_batch_create: bad arguments"); return -1; }
     *out = nullptr;
     const cf_params &p0 = p[0];
-    if (p0.precision != 0 || p0.les_model != 0 || p0.iwall_model != 0 || p0.y_mass_cte != 0 ||
-        p0.wall_surrogate != 0) {
-        set_error("cf_batch_create: the batched step takes FP64 DNS cases (no eddy viscosity, no wall model, "
-                  "no y mass flow)");
+    if (p0.les_model == 2 || p0.y_mass_cte != 0 || p0.wall_surrogate != 0) {
+        set_error("cf_batch_create: the batched step takes no dynamic Smagorinsky model, y mass flow or learned "
+                  "wall model");
         return -1;
     }
     std::vector<cf::Host> hosts;

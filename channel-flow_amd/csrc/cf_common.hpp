@@ -53,6 +53,7 @@ struct Scalars {
     double ucl;       // centreline target Ucenterline_x (x_mass_cte = 2)
     double nu;        // kinematic viscosity of the case (read by step kernels of a batch)
     double dt_vis;    // viscous limit of compute_dt (time_integration.f90:205-212)
+    double tau_mean;  // wall-model fallback stress of the case (tau_mean_eqwm)
     double pad[3];    // [0] last bulk (or centreline) mean of U, [1] last bulk mean of V
     int n_nan_eqwm;
     int n_iter_eqwm;

@@ -148,7 +148,14 @@ __global__ void __launch_bounds__(EQWM_WG) k_wall_model(Dims d, const S *__restr
                              double *__restrict__ tau_init, double *__restrict__ alpha_x,
                              double *__restrict__ azt, Scalars *sc, EqwmParams p, double nu,
                              double h, int jref, double tau_mean, const double *__restrict__ tau_pred,
-                             const int *__restrict__ use_pred, const double *__restrict__ ubar) {
+                             const int *__restrict__ use_pred, const double *__restrict__ ubar, int64_t cs) {
+    U = case_ptr(U, cs); W = case_ptr(W, cs); NT = case_ptr(NT, cs);
+    tau_init = case_ptr(tau_init, cs); alpha_x = case_ptr(alpha_x, cs); azt = case_ptr(azt, cs);
+    sc = case_ptr(sc, cs); ubar = case_ptr(ubar, cs);
+    if (cs) {
+        nu = sc->nu;
+        tau_mean = sc->tau_mean;
+    }
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
     EqwmLds s = eqwm_lds(p.ncv);
     eqwm_grid(p.ncv, p.stretch, s.g_fa, s.g_cv, s.r_cv);
@@ -205,7 +212,9 @@ __global__ void __launch_bounds__(EQWM_WG) k_wall_model(Dims d, const S *__restr
 
 // wallmodel.f90:667-674: alpha_z from the 4-point average of alpha_z_temp,
 // with the reference's periodic copies folded into the index map.
-__global__ void k_alpha_z(Dims d, const double *__restrict__ azt, double *__restrict__ az) {
+__global__ void k_alpha_z(Dims d, const double *__restrict__ azt, double *__restrict__ az, int64_t cs) {
+    azt = case_ptr(azt, cs);
+    az = case_ptr(az, cs);
     const int nxg = d.nxg, nz = d.nz, nx = d.nx;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nxg * 2 * nz) return;
@@ -225,11 +234,13 @@ void launch_wall_model(const Dims &d, const S *U, const S *W, const S *NT, const
                        const double *tau_pred, const int *use_pred, const double *ubar, hipStream_t st) {
     const int n = d.nx * d.nzg * 2;
     const int wg = eqwm_block(p.ncv);
-    hipLaunchKernelGGL((k_wall_model<S>), dim3((n + wg - 1) / wg), dim3(wg), eqwm_lds_bytes(p.ncv, wg), st, d,
+    const int nb = launch_batch().n;
+    const int64_t cs = launch_batch().cs;
+    hipLaunchKernelGGL((k_wall_model<S>), dim3((n + wg - 1) / wg, nb), dim3(wg), eqwm_lds_bytes(p.ncv, wg), st, d,
                        U, W, NT, t, tau_init,
-                       alpha_x, azt, sc, p, nu, h, jref, tau_mean, tau_pred, use_pred, ubar);
+                       alpha_x, azt, sc, p, nu, h, jref, tau_mean, tau_pred, use_pred, ubar, cs);
     const int n2 = d.nxg * 2 * d.nz;
-    hipLaunchKernelGGL(k_alpha_z, dim3((n2 + 255) / 256), dim3(256), 0, st, d, azt, az);
+    hipLaunchKernelGGL(k_alpha_z, dim3((n2 + 255) / 256, nb), dim3(256), 0, st, d, azt, az, cs);
 }
 
 // input of the wall model at every wall point, for the learned surrogate:

@@ -541,7 +541,11 @@ struct GhostSet {
     int n;
 };
 template <typename S>
-__global__ void k_ghosts(Dims d, GhostSet gs, double dyg_bot, double dyg_top) {
+__global__ void k_ghosts(Dims d, GhostSet gs, double dyg_bot, double dyg_top, int64_t cs) {
+    for (int f = 0; f < gs.n; f++) {
+        gs.F[f] = case_ptr(gs.F[f], cs);
+        gs.alpha[f] = case_ptr(gs.alpha[f], cs);
+    }
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t beg = 0;
     for (int f = 0; f < gs.n; f++) {
@@ -1097,7 +1101,10 @@ k_smagorinsky(Dims d, Metrics m, const double *__restrict__ y, const double *__r
               const double *__restrict__ delta, const S *__restrict__ U, const S *__restrict__ V,
               const S *__restrict__ W, S *__restrict__ NT, const Scalars *__restrict__ sc,
               const double *__restrict__ ubar, double cs, double nu, int damping, int tiles_x,
-              int tiles_y, int ntiles) {
+              int tiles_y, int ntiles, int64_t ccs) {
+    U = case_ptr(U, ccs); V = case_ptr(V, ccs); W = case_ptr(W, ccs); NT = case_ptr(NT, ccs);
+    sc = case_ptr(sc, ccs); ubar = case_ptr(ubar, ccs);
+    if (ccs) nu = sc->nu;
     const int t = xcd_remap(blockIdx.x, ntiles);
     const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
     const int i = 2 + tx * 64 + (threadIdx.x & 63);
@@ -1375,8 +1382,8 @@ void launch_ghosts3(const Dims &d, S *const *F, const int *field, const int *wal
     }
     gs.n = n;
     if (tot == 0) return;
-    hipLaunchKernelGGL((k_ghosts<S>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, d, gs, dyg_bot,
-                       dyg_top);
+    hipLaunchKernelGGL((k_ghosts<S>), dim3((unsigned)((tot + 255) / 256), launch_batch().n), dim3(256), 0, st, d, gs,
+                       dyg_bot, dyg_top, launch_batch().cs);
 }
 
 int project_vw_blocks(const Dims &d, int esize) {
@@ -1498,8 +1505,8 @@ void launch_smagorinsky(const Dims &d, const Metrics &m, const double *y, const 
                         const S *U, const S *V, const S *W, S *NT, const Scalars *sc, const double *ubar,
                         double cs, double nu, int damping, hipStream_t st) {
     const int tx = tiles_x(d), ty = tiles_y(d), nt = tx * ty * d.nzp;
-    hipLaunchKernelGGL((k_smagorinsky<S, S>), dim3(nt), dim3(256), 0, st, d, m, y, yg, delta, U, V, W, NT, sc,
-                       ubar, cs, nu, damping, tx, ty, nt);
+    hipLaunchKernelGGL((k_smagorinsky<S, S>), dim3(nt, launch_batch().n), dim3(256), 0, st, d, m, y, yg, delta, U,
+                       V, W, NT, sc, ubar, cs, nu, damping, tx, ty, nt, launch_batch().cs);
 }
 
 // each plane is split over STATS_NB workgroups, so the launch fills the GPU;
@@ -1655,7 +1662,8 @@ template void launch_project_vw<float, float>(const Dims &, const Metrics &, con
 // ---------------------------------------------------------------------------
 namespace cf {
 
-__global__ void k_reset_counters(Scalars *sc) {
+__global__ void k_reset_counters(Scalars *sc, int64_t cs) {
+    sc = case_ptr(sc, cs);
     sc->n_nan_eqwm = 0;  // wallmodel.f90:613-614
     sc->n_iter_eqwm = 0;
     sc->n_fallback_mlp = 0;
@@ -1674,7 +1682,9 @@ __global__ void k_advance_time(Scalars *sc, int64_t cs) {
 // correction adds corr to U on the interior (time_integration.f90:155), i.e.
 // to ubar(2..nyg-1); the ghost entries follow the wall condition the ghost
 // kernel applies to U' (Dirichlet: antisymmetric, Neumann: symmetric).
-__global__ void __launch_bounds__(256) k_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann) {
+__global__ void __launch_bounds__(256) k_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann, int64_t cs) {
+    sc = case_ptr(sc, cs);
+    ubar = case_ptr(ubar, cs);
     const double c = sc->corr;
     __syncthreads();  // every thread has read corr before thread 0 moves on
     for (int j = 2 + threadIdx.x; j <= nyg - 1; j += blockDim.x) ubar[j] += c;
@@ -1686,11 +1696,12 @@ __global__ void __launch_bounds__(256) k_ubar_shift(Scalars *sc, double *ubar, i
     }
 }
 void launch_ubar_shift(Scalars *sc, double *ubar, int nyg, int neumann, hipStream_t st) {
-    hipLaunchKernelGGL(k_ubar_shift, dim3(1), dim3(256), 0, st, sc, ubar, nyg, neumann);
+    hipLaunchKernelGGL(k_ubar_shift, dim3(1, launch_batch().n), dim3(256), 0, st, sc, ubar, nyg, neumann,
+                       launch_batch().cs);
 }
 
 void launch_reset_counters(Scalars *sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, st, sc);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1, launch_batch().n), dim3(1), 0, st, sc, launch_batch().cs);
 }
 void launch_dt_final_only(Scalars *sc, double cfl, hipStream_t st) {
     hipLaunchKernelGGL(k_dt_fixed, dim3(1, launch_batch().n), dim3(1), 0, st, sc, cfl, launch_batch().cs);

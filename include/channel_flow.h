@@ -201,8 +201,9 @@ long long cf_footprint(const cf_params *p, const double *x, const double *y, con
  * step is launched once for all n cases (the case index is the launch
  * grid's second dimension), so small grids fill the GPU.  The reference has
  * no counterpart (one case is one run of main.f90); each member computes
- * the same bits as the same case alone.  The batched step takes FP64 DNS
- * cases (no eddy viscosity, wall model or y mass flow).
+ * the same bits as the same case alone.  The batched step takes both
+ * precisions, the Smagorinsky model and the equilibrium wall model; not the
+ * dynamic Smagorinsky model, y mass flow or the learned wall model.
  * cf_batch_member gives member m as an ordinary context for everything but
  * the step (set_state, get_state, statistics, ...); it is owned by the batch
  * and is stepped only by cf_batch_run_steps. */

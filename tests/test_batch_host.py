@@ -1,6 +1,7 @@
 """Batch creation checks that run before any device is touched: the members
 of a batch share every option but nu, dPdx, dPdz, u_centerline and
-tau_mean_eqwm, and the batched step takes FP64 DNS cases only."""
+tau_mean_eqwm, and the batched step takes neither the dynamic Smagorinsky
+model, nor y mass flow, nor the learned wall model."""
 import pytest
 
 from channel_flow.serving import ensemble_case
@@ -11,10 +12,12 @@ def _make(cfgs):
     return ChannelFlowBatch(cfgs)
 
 
-def test_mixed_precision_and_wall_model_classes_are_refused():
-    wm = ensemble_case((64, 64, 64), 1000.0, True)
-    with pytest.raises(RuntimeError, match="FP64 DNS"):
-        _make([wm, wm])
+@pytest.mark.parametrize("option", ["les_model", "y_mass_cte", "wall_surrogate"])
+def test_unbatched_options_are_refused(option):
+    c = ensemble_case((64, 64, 64), 1000.0, True)
+    setattr(c, option, 2 if option == "les_model" else 1)
+    with pytest.raises(RuntimeError, match="batched step takes no"):
+        _make([c, c])
 
 
 def test_members_differing_in_a_shared_option_are_refused():

@@ -2,10 +2,12 @@
 launched once for all members, whose buffers sit at one stride in one
 allocation.  Every member computes the same bits as the same case alone
 (state, ghosts included, and scalars) after ten steps, for the serving
-ensemble's 64^3 DNS class (Re_tau spread, seeds differ, CFL time steps, bulk
-mass flow, lane y solve switched on by the mode count where it applies) and
-for a small ragged class (partitioned y solve, fixed dt, centreline mass
-flow) stepped without graphs.  A member is stepped only through its batch."""
+ensemble's 64^3 DNS class (FP64, Re_tau spread, seeds differ, CFL time
+steps, bulk mass flow), its wall-modelled LES class (mixed precision,
+Smagorinsky, equilibrium wall model with each case's own nu and fallback
+stress, deviation profile), and a small ragged class (partitioned y solve,
+fixed dt, centreline mass flow) stepped without graphs.  A member is stepped
+only through its batch."""
 import numpy as np
 import pytest
 
@@ -54,6 +56,17 @@ def test_batch_of_ensemble_dns_class_is_bitwise_each_case_alone():
     got = _check(cfgs, 10, True)
     # the members really differ (no cross-talk would hide behind equal inputs)
     assert not np.array_equal(got[0][0][0], got[1][0][0])
+
+
+def test_batch_of_ensemble_wmles_class_is_bitwise_each_case_alone():
+    rng = np.random.default_rng(4)
+    cfgs = []
+    for q in range(3):
+        c = ensemble_case((64, 64, 64), float(rng.uniform(180.0, 2000.0)), True, name="w%d" % q)
+        c.seed = 200 + q
+        cfgs.append(c)
+    print("batch 64^3 WMLES x3: nu %s" % [round(c.nu, 9) for c in cfgs], flush=True)
+    _check(cfgs, 10, True)
 
 
 def test_batch_of_ragged_class_without_graphs():
