@@ -52,9 +52,11 @@ class Server:
 
     hbm_budget: device bytes reserved for all running cases; max_active:
     cases running at once (one HIP stream each); depth: steps in flight per
-    case."""
+    case.  batch > 0: cases of one size class are stepped together, up to
+    ``batch`` per group, in stretches of at most ``chunk`` steps (one stream
+    per group)."""
 
-    def __init__(self, hbm_budget, max_active=8, depth=3, device=0):
+    def __init__(self, hbm_budget, max_active=8, depth=3, device=0, batch=0, chunk=25):
         self.lib = _lib.load()
         self.device = device
         h = ctypes.c_void_p()
@@ -62,6 +64,8 @@ class Server:
                                              ctypes.byref(h)), "cf_server_create")
         self.h = h
         self.cases = {}
+        if batch:
+            _lib.check(self.lib.cf_server_set_batching(self.h, int(batch), int(chunk)), "cf_server_set_batching")
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -138,10 +142,10 @@ class Server:
         return ms[:m].astype(np.float64)
 
     def summary(self):
-        v = np.zeros(6)
-        _lib.check(self.lib.cf_server_summary(self.h, _lib.dptr(v), 6), "cf_server_summary")
+        v = np.zeros(8)
+        _lib.check(self.lib.cf_server_summary(self.h, _lib.dptr(v), 8), "cf_server_summary")
         return dict(zip(["makespan_s", "peak_bytes", "peak_running", "waited_for_memory", "arena_bytes",
-                         "cases"], v))
+                         "cases", "groups", "peak_members"], v))
 
 
 # ---------------------------------------------------------------- workloads

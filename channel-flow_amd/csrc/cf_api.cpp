@@ -1309,6 +1309,20 @@ std::unique_ptr<CtxBase> make_ctx(const Host &h, Slab *slab, hipStream_t stream)
     return c;
 }
 
+std::string batch_class_key(const Host &h) {
+    // the struct is read as bytes: it has no padding (4 ints, 5 doubles,
+    // 6 ints, 1 double, 2 ints, 3 doubles, 2 ints, 1 double)
+    static_assert(sizeof(cf_params) == 136, "cf_params layout changed: revisit batch_class_key");
+    const cf_params &p = h.prm;
+    if (p.les_model == 2 || p.y_mass_cte != 0 || p.wall_surrogate != 0) return "";
+    cf_params q = p;
+    q.nu = q.dPdx = q.dPdz = q.u_centerline = q.tau_mean_eqwm = 0.0;
+    std::string k(reinterpret_cast<const char *>(&q), sizeof q);
+    for (const std::vector<double> *v : {&h.x, &h.y, &h.z})
+        k.append(reinterpret_cast<const char *>(v->data()), v->size() * sizeof(double));
+    return k;
+}
+
 size_t ctx_footprint(const Host &h) {
     Slab dry;
     dry.dry = true;
@@ -1401,42 +1415,28 @@ static void batch_free(cf_batch *b) {
     delete b;
 }
 
-// the options a batch's members must share (all but nu, the forcing and the
-// fallback stress), and the ones the batched step supports
-static std::string batch_class_check(const cf_params &a, const cf_params &b) {
-    // compared as bytes: the struct has no padding (4 ints, 5 doubles, 6 ints,
-    // 1 double, 2 ints, 3 doubles, 2 ints, 1 double)
-    static_assert(sizeof(cf_params) == 136, "cf_params layout changed: revisit the member comparison");
-    cf_params x = b;
-    x.nu = a.nu;
-    x.dPdx = a.dPdx;
-    x.dPdz = a.dPdz;
-    x.u_centerline = a.u_centerline;
-    x.tau_mean_eqwm = a.tau_mean_eqwm;
-    if (std::memcmp(&x, &a, sizeof(cf_params)) != 0)
-        return "members differ in an option other than nu, dPdx, dPdz, u_centerline and tau_mean_eqwm";
-    return "";
-}
 
 int cf_batch_create(int n, const cf_params *p, const double *x, const double *y, const double *z, cf_batch **out) {
     if (n < 1 || !p || !x || !y || !z || !out) { set_error("cf_batch_create: bad arguments"); return -1; }
     *out = nullptr;
     const cf_params &p0 = p[0];
-    if (p0.les_model == 2 || p0.y_mass_cte != 0 || p0.wall_surrogate != 0) {
-        set_error("cf_batch_create: the batched step takes no dynamic Smagorinsky model, y mass flow or learned "
-                  "wall model");
-        return -1;
-    }
     std::vector<cf::Host> hosts;
     hosts.reserve(n);
     for (int m = 0; m < n; m++) {
-        if (m > 0) {
-            const std::string e = batch_class_check(p0, p[m]);
-            if (!e.empty()) { set_error("cf_batch_create: " + e); return -1; }
-        }
         hosts.emplace_back(p[m], x, y, z);
         const std::string v = hosts.back().validate();
         if (!v.empty()) { set_error("cf_batch_create: " + v); return -1; }
+        const std::string k = cf::batch_class_key(hosts.back());
+        if (k.empty()) {
+            set_error("cf_batch_create: the batched step takes no dynamic Smagorinsky model, y mass flow or learned "
+                      "wall model");
+            return -1;
+        }
+        if (m > 0 && k != cf::batch_class_key(hosts[0])) {
+            set_error("cf_batch_create: members differ in an option other than nu, dPdx, dPdz, u_centerline and "
+                      "tau_mean_eqwm");
+            return -1;
+        }
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("cf_batch_create: no HIP device"); return -1; }

@@ -102,7 +102,15 @@ struct CtxBase {
     // sit at the byte stride cs.
     bool batch_member = false;
     virtual int batch_run(const std::vector<CtxBase *> &members, int64_t cs, int nsteps, bool graph) = 0;
+    // ping-pong parity of the state buffers (members of a batch share it)
+    virtual int parity() const = 0;
+    virtual void set_parity(int p) = 0;
 };
+
+// The size class of a case for batched stepping: its options less the
+// per-case ones (nu, forcing, centreline target, fallback stress) and its
+// grids, as bytes; empty when the batched step does not take the case.
+std::string batch_class_key(const Host &h);
 
 // a context for one case: device buffers from `slab` when given (else
 // hipMalloc), work on `stream` when given (else its own); null on failure
@@ -232,6 +240,8 @@ struct Ctx : CtxBase {
     int wall_fallbacks() override;
     int prepare_graphs() override;
     int batch_run(const std::vector<CtxBase *> &members, int64_t cs, int nsteps, bool graph) override;
+    int parity() const override { return cur; }
+    void set_parity(int p) override { cur = p; }
 };
 
 }  // namespace cf

@@ -16,6 +16,17 @@
 //   arena, so a case computes the same bits alone and in a mix.
 // - Step latency of a case is the GPU time between its consecutive step
 //   completion events; case latency is finish minus arrival.
+// - With batching on (cf_server_set_batching), cases of one size class
+//   (batch_class_key: the same options but nu and the forcing, the same
+//   grid) share a group: one arena region of cap member slabs at one stride
+//   and one stream, stepped by the batched step (each kernel launched once
+//   for all members).  A group runs in stretches of at most `chunk` steps,
+//   none longer than its members' remaining steps; between stretches (the
+//   group drained) finished members are read back and retired, and cases
+//   that arrived for the group are prepared into free member slots, so
+//   requests join a running group as members leave.  A member's step
+//   latency is the group step's GPU time.  Cases outside any batchable class
+//   run one per stream as above.
 //
 // The reference has no serving layer; one case is one run of main.f90.
 #include <hip/hip_runtime.h>
@@ -70,9 +81,16 @@ struct Arena {
         for (auto &f : free_) s += f.second;
         return s;
     }
+    size_t largest() const {
+        size_t s = 0;
+        for (auto &f : free_) s = std::max(s, f.second);
+        return s;
+    }
 };
 
 enum CaseState { PENDING = 0, PREPARING = 1, RUNNING = 2, DONE = 3, CANCELLED = 4, FAILED = 5 };
+
+struct Group;
 
 struct Case {
     long long id = 0;
@@ -93,6 +111,24 @@ struct Case {
     std::vector<double> Uo, Vo, Wo;
     double scal[7] = {0, 0, 0, 0, 0, 0, 0};
     std::string err;
+    std::string bkey;        // batch class (empty: not batchable)
+    Group *grp = nullptr;    // group and member slot while in one
+    int mslot = -1;
+};
+
+// cases of one size class stepped together (see the file comment)
+struct Group {
+    std::string key;
+    int cap = 0, slot = -1;
+    size_t fp = 0, off = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Slab> slabs;
+    std::vector<std::unique_ptr<CtxBase>> ctx;  // per member slot, kept after its case leaves
+    std::vector<Case *> mem;                    // case in each member slot (null: free)
+    std::vector<Case *> run;                    // members stepped by the current stretch
+    std::vector<hipEvent_t> ev;
+    int launched = 0, completed = 0;
+    std::unique_ptr<Host> proto;  // the first case's host (fills a slot that has no context)
 };
 
 struct Server {
@@ -116,6 +152,10 @@ struct Server {
     int peak_running = 0;
     long long n_blocked_mem = 0;  // cases that reached the head of the queue and waited for memory
     long long last_blocked = -1;
+    int batch_cap = 0, chunk = 25;  // batching: members per group (0: off), steps per stretch
+    std::vector<std::unique_ptr<Group>> groups;
+    long long groups_formed = 0;
+    int peak_members = 0;
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -123,6 +163,10 @@ struct Server {
     void scheduler();
     void prepare_loop();
     void retire(Case *c, int final_state);
+    bool admit_to_group(Case *c, double t);
+    bool step_groups();
+    int group_boundary(Group *g);
+    void group_free(Group *g);
 };
 
 static int prepare_case(Server *sv, Case *c) {
@@ -204,6 +248,242 @@ void Server::retire(Case *c, int final_state) {
     cv_done.notify_all();
 }
 
+// ---- groups ----
+// admission of a batchable case (under mu): a free member slot of a group of
+// its class, else a new group sized for the arrived cases of the class (up
+// to batch_cap, as many as the largest free arena region holds) on a free
+// stream slot.  The case is prepared at the group's next boundary.
+bool Server::admit_to_group(Case *c, double t) {
+    Group *g = nullptr;
+    int m = -1;
+    for (auto &gp : groups) {
+        if (gp->key != c->bkey) continue;
+        for (int q = 0; q < gp->cap; q++)
+            if (!gp->mem[q]) { g = gp.get(); m = q; break; }
+        if (g) break;
+    }
+    if (!g) {
+        if (free_slots.empty()) return false;
+        int want = 0;
+        for (Case *p : pending)
+            if (p->bkey == c->bkey && p->arrival <= t && !p->cancel.load()) want++;
+        const size_t fit = arena.largest() / c->fp;
+        const int cap = (int)std::min<size_t>((size_t)std::min(batch_cap, std::max(want, 1)), fit);
+        size_t off = 0;
+        if (cap < 1 || !arena.alloc((size_t)cap * c->fp, &off)) {
+            if (c->id != last_blocked) { n_blocked_mem++; last_blocked = c->id; }
+            return false;
+        }
+        std::unique_ptr<Group> ng(new Group);
+        ng->key = c->bkey;
+        ng->cap = cap;
+        ng->fp = c->fp;
+        ng->off = off;
+        ng->slot = free_slots.back();
+        free_slots.pop_back();
+        ng->stream = streams[ng->slot];
+        ng->slabs.resize(cap);
+        ng->ctx.resize(cap);
+        ng->mem.assign(cap, nullptr);
+        ng->ev.assign(chunk + 1, nullptr);
+        ng->proto.reset(new Host(*c->host));
+        g = ng.get();
+        m = 0;
+        groups.push_back(std::move(ng));
+        groups_formed++;
+        peak_used = std::max(peak_used, arena.cap - arena.free_bytes());
+    }
+    g->mem[m] = c;
+    c->grp = g;
+    c->mslot = m;
+    c->state = PREPARING;
+    c->t_admit = t;
+    return true;
+}
+
+// a member's state and scalars into its case record, then out of the group
+static int group_fetch(Group *g, int m, Case *c) {
+    const Dims &d = c->host->d;
+    c->Uo.resize((size_t)d.nx * d.nyg * d.nzg);
+    c->Vo.resize((size_t)d.nxg * d.ny * d.nzg);
+    c->Wo.resize((size_t)d.nxg * d.nyg * d.nz);
+    if (g->ctx[m]->get_state(c->Uo.data(), c->Vo.data(), c->Wo.data()) || g->ctx[m]->get_scalars(c->scal)) {
+        c->err = last_error_copy();
+        return -1;
+    }
+    return 0;
+}
+
+// a fresh context for member slot m with host h (the slot's previous one,
+// if any, is dropped), its state parity that of the running members
+static int group_new_ctx(Server *sv, Group *g, int m, const Host &h) {
+    g->ctx[m].reset();
+    g->slabs[m].base = sv->base + g->off + (size_t)m * g->fp;
+    g->slabs[m].cap = g->fp;
+    g->slabs[m].used = 0;
+    g->ctx[m] = make_ctx(h, &g->slabs[m], g->stream);
+    if (!g->ctx[m]) return -1;
+    if (g->slabs[m].used != g->fp) {
+        g->ctx[m].reset();
+        set_error("group member layout differs from its footprint");
+        return -1;
+    }
+    g->ctx[m]->batch_member = true;
+    for (int q = 0; q < g->cap; q++)
+        if (q != m && g->ctx[q] && g->mem[q] && g->mem[q]->state == RUNNING) {
+            g->ctx[m]->set_parity(g->ctx[q]->parity());
+            break;
+        }
+    return 0;
+}
+
+// the group has drained: retire finished and cancelled members, prepare the
+// admitted ones, then launch the next stretch.  Returns 1 when the group is
+// empty (nothing running or admitted), -1 on a device error.
+int Server::group_boundary(Group *g) {
+    for (int m = 0; m < g->cap; m++) {
+        Case *c = g->mem[m];
+        if (!c || c->state != RUNNING) continue;
+        const bool done = c->completed >= c->nsteps;
+        if (!done && !c->cancel.load()) continue;
+        int st = done ? DONE : CANCELLED;
+        if (done && group_fetch(g, m, c)) st = FAILED;
+        std::lock_guard<std::mutex> lk(mu);
+        g->mem[m] = nullptr;
+        c->grp = nullptr;
+        c->t_done = now();
+        c->state = st;
+        cv_done.notify_all();
+    }
+    for (int m = 0; m < g->cap; m++) {
+        Case *c = g->mem[m];
+        if (!c || c->state != PREPARING) continue;
+        int rc = c->cancel.load() ? 1 : group_new_ctx(this, g, m, *c->host);
+        CtxBase *x = rc == 0 ? g->ctx[m].get() : nullptr;
+        // initialization.f90:156-160, 460 on the group's stream (drained)
+        if (x && (x->set_state(c->U.data(), c->V.data(), c->W.data()) || x->apply_bc() || x->projection_step(0) ||
+                  x->apply_bc() || x->init_mass_flow() || x->sync()))
+            rc = -1;
+        std::lock_guard<std::mutex> lk(mu);
+        std::vector<double>().swap(c->U);
+        std::vector<double>().swap(c->V);
+        std::vector<double>().swap(c->W);
+        c->t_ready = now();
+        if (rc != 0) {
+            if (rc < 0) c->err = last_error_copy();
+            g->mem[m] = nullptr;
+            c->grp = nullptr;
+            c->t_done = now();
+            c->state = rc < 0 ? FAILED : CANCELLED;
+            cv_done.notify_all();
+            continue;
+        }
+        c->state = RUNNING;
+    }
+    // the stretch: members [0, hi) (a slot below hi with no context gets a
+    // filler with a zero field, so that every stepped slab is initialised)
+    int hi = 0, nrun = 0, steps = chunk;
+    g->run.clear();
+    for (int m = 0; m < g->cap; m++) {
+        Case *c = g->mem[m];
+        if (c && c->state == RUNNING) {
+            hi = m + 1;
+            nrun++;
+            steps = std::min(steps, c->nsteps - c->completed);
+            g->run.push_back(c);
+        }
+    }
+    if (nrun == 0) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Case *c : g->mem)
+            if (c) return 0;  // admitted, not yet prepared (cannot happen after the loop above)
+        return 1;
+    }
+    for (int m = 0; m < hi; m++) {
+        if (g->ctx[m]) continue;
+        if (group_new_ctx(this, g, m, *g->proto)) return -1;
+        const Dims &d = g->proto->d;
+        std::vector<double> zu((size_t)d.nx * d.nyg * d.nzg, 0.0), zv((size_t)d.nxg * d.ny * d.nzg, 0.0),
+            zw((size_t)d.nxg * d.nyg * d.nz, 0.0);
+        if (g->ctx[m]->set_state(zu.data(), zv.data(), zw.data())) return -1;
+    }
+    peak_members = std::max(peak_members, nrun);
+    // idle slots below hi follow the running members' parity (their data
+    // are stepped but read by no one)
+    int par = -1;
+    for (int m = 0; m < hi && par < 0; m++)
+        if (g->mem[m] && g->mem[m]->state == RUNNING) par = g->ctx[m]->parity();
+    std::vector<CtxBase *> mem;
+    for (int m = 0; m < hi; m++) {
+        if (g->ctx[m]->parity() != par) g->ctx[m]->set_parity(par);
+        mem.push_back(g->ctx[m].get());
+    }
+    for (auto &e : g->ev)
+        if (!e && hipEventCreate(&e) != hipSuccess) return -1;
+    g->launched = g->completed = 0;
+    if (hipEventRecord(g->ev[0], g->stream) != hipSuccess) return -1;
+    for (int q = 0; q < steps; q++) {
+        if (mem[0]->batch_run(mem, (int64_t)g->fp, 1, true)) return -1;
+        if (hipEventRecord(g->ev[q + 1], g->stream) != hipSuccess) return -1;
+        g->launched++;
+    }
+    return 0;
+}
+
+void Server::group_free(Group *g) {
+    if (g->stream) hipStreamSynchronize(g->stream);
+    for (auto &x : g->ctx) x.reset();
+    for (auto &e : g->ev)
+        if (e) hipEventDestroy(e);
+    g->ev.clear();
+}
+
+// completions of every group's stretch; at a drained group its boundary.
+// Returns whether anything happened.
+bool Server::step_groups() {
+    bool busy = false;
+    for (size_t gi = 0; gi < groups.size();) {
+        Group *g = groups[gi].get();
+        while (g->completed < g->launched) {
+            hipEvent_t e = g->ev[g->completed + 1];
+            const hipError_t st = hipEventQuery(e);
+            if (st == hipErrorNotReady) break;
+            float ms = 0.f;
+            if (st == hipSuccess) hipEventElapsedTime(&ms, g->ev[g->completed], e);
+            std::lock_guard<std::mutex> lk(mu);
+            for (Case *c : g->run) {
+                c->step_ms.push_back(ms);
+                c->completed++;
+            }
+            g->completed++;
+            busy = true;
+        }
+        if (g->completed < g->launched) { gi++; continue; }
+        const int rc = group_boundary(g);
+        if (rc == 0) { gi++; busy = busy || g->launched > 0; continue; }
+        // empty, or a device error: fail what is left, release the group
+        group_free(g);
+        std::lock_guard<std::mutex> lk(mu);
+        for (int m = 0; m < g->cap; m++) {
+            Case *c = g->mem[m];
+            if (!c) continue;
+            c->err = rc < 0 ? last_error_copy() : std::string("group released");
+            c->grp = nullptr;
+            c->t_done = now();
+            c->state = FAILED;
+        }
+        if (rc < 0) cv_done.notify_all();
+        free_slots.push_back(g->slot);
+        arena.release(g->off, (size_t)g->cap * g->fp);
+        groups.erase(groups.begin() + gi);
+        busy = true;
+    }
+    size_t nrun = running.size();
+    for (auto &g : groups) nrun += g->run.size();
+    peak_running = std::max(peak_running, (int)nrun);
+    return busy;
+}
+
 void Server::scheduler() {
     hipSetDevice(dev);
     while (!stop.load()) {
@@ -229,7 +509,14 @@ void Server::scheduler() {
             while (!pending.empty()) {
                 Case *c = pending.front();
                 if (c->cancel.load()) { pending.pop_front(); c->state = CANCELLED; c->t_done = t; cv_done.notify_all(); continue; }
-                if (c->arrival > t || free_slots.empty()) break;
+                if (c->arrival > t) break;
+                if (batch_cap > 0 && !c->bkey.empty()) {
+                    if (!admit_to_group(c, t)) break;
+                    pending.pop_front();
+                    busy = true;
+                    continue;
+                }
+                if (free_slots.empty()) break;
                 size_t off = 0;
                 if (!arena.alloc(c->fp, &off)) {
                     if (c->id != last_blocked) { n_blocked_mem++; last_blocked = c->id; }
@@ -298,6 +585,7 @@ void Server::scheduler() {
             }
             q++;
         }
+        if (!groups.empty() && step_groups()) busy = true;
         gpu.unlock();
         if (!busy) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -361,6 +649,14 @@ int cf_server_create(int device, long long hbm_budget, int max_active, int depth
     return 0;
 }
 
+int cf_server_set_batching(cf_server *sv, int max_members, int chunk) {
+    if (!sv || max_members < 0 || chunk < 1) { set_error("cf_server_set_batching: bad arguments"); return -1; }
+    if (sv->s.started.load()) { set_error("cf_server_set_batching: server already started"); return -1; }
+    sv->s.batch_cap = max_members;
+    sv->s.chunk = chunk;
+    return 0;
+}
+
 int cf_server_start(cf_server *sv) {
     if (!sv) { set_error("cf_server_start: null server"); return -1; }
     Server &s = sv->s;
@@ -383,7 +679,7 @@ int cf_server_destroy(cf_server *sv) {
         for (int spin = 0; spin < 200000; spin++) {
             {
                 std::lock_guard<std::mutex> lk(s.mu);
-                if (s.running.empty() && s.to_prepare.empty() && s.prepared.empty()) {
+                if (s.running.empty() && s.to_prepare.empty() && s.prepared.empty() && s.groups.empty()) {
                     bool any = false;
                     for (auto &kv : s.cases) any = any || kv.second->state == cf::PREPARING;
                     if (!any) break;
@@ -397,6 +693,8 @@ int cf_server_destroy(cf_server *sv) {
         if (s.worker.joinable()) s.worker.join();
     }
     hipSetDevice(s.dev);
+    for (auto &g : s.groups) s.group_free(g.get());
+    s.groups.clear();
     for (Case *c : s.running) {
         if (c->ctx) c->ctx->sync();
         c->ctx.reset();
@@ -427,6 +725,7 @@ int cf_server_submit(cf_server *sv, const cf_params *p, const double *x, const d
     const std::string v = c->host->validate();
     if (!v.empty()) { set_error("cf_server_submit: " + v); return -1; }
     c->fp = c->footprint = cf::ctx_footprint(*c->host);
+    c->bkey = cf::batch_class_key(*c->host);
     if (c->fp == 0) return -1;
     if (c->fp > s.arena.cap) { set_error("cf_server_submit: case footprint exceeds the HBM budget"); return -1; }
     const cf::Dims &d = c->host->d;
@@ -524,9 +823,10 @@ int cf_server_summary(cf_server *sv, double *out, int n) {
     std::lock_guard<std::mutex> lk(s.mu);
     double last = 0;
     for (auto &kv : s.cases) last = std::max(last, kv.second->t_done);
-    const double v[6] = {last, (double)s.peak_used, (double)s.peak_running, (double)s.n_blocked_mem,
-                         (double)s.arena.cap, (double)s.cases.size()};
-    for (int q = 0; q < n && q < 6; q++) out[q] = v[q];
+    const double v[8] = {last, (double)s.peak_used, (double)s.peak_running, (double)s.n_blocked_mem,
+                         (double)s.arena.cap, (double)s.cases.size(), (double)s.groups_formed,
+                         (double)s.peak_members};
+    for (int q = 0; q < n && q < 8; q++) out[q] = v[q];
     return 0;
 }
 

@@ -233,6 +233,13 @@ int cf_server_destroy(cf_server *s);
 int cf_server_submit(cf_server *s, const cf_params *p, const double *x, const double *y, const double *z,
                      const double *U, const double *V, const double *W, int nsteps, double arrival,
                      long long *id);
+/* batching (before cf_server_start): cases of one size class (every option
+ * but nu, dPdx, dPdz, u_centerline and tau_mean_eqwm shared, the same grid;
+ * see cf_batch_create for the classes taken) are stepped together, up to
+ * max_members per group, in stretches of at most chunk steps; cases join a
+ * running group between stretches.  max_members 0 (the default): one case
+ * per stream. */
+int cf_server_set_batching(cf_server *s, int max_members, int chunk);
 /* start the clock, the scheduler and the admission worker (cases may be
  * submitted before and after) */
 int cf_server_start(cf_server *s);
@@ -250,9 +257,9 @@ int cf_server_result(cf_server *s, long long id, double *U, double *V, double *W
 int cf_server_case_info(cf_server *s, long long id, double *info, int n);
 /* per-step GPU durations (ms) of a case; returns how many were written */
 int cf_server_step_times(cf_server *s, long long id, float *ms, int n);
-/* out (up to 6): last finish time, peak arena bytes in use, peak running
+/* out (up to 8): last finish time, peak arena bytes in use, peak running
  * cases, cases that waited for memory at the head of the queue, arena bytes,
- * cases submitted */
+ * cases submitted, batch groups formed, most members stepped together */
 int cf_server_summary(cf_server *s, double *out, int n);
 
 /* Per-kernel timing with HIP events recorded on the context's stream. */
