@@ -187,13 +187,17 @@ def other_configs(steps=50, warmup=5):
     return res
 
 
-def serving_bench(scale, slot_counts, budget_gb, n_cases=64, seed=5):
+def serving_bench(scale, legs, budget_gb, chunk=25, n_cases=64, seed=5):
     """Ensemble of channel cases (seeded Poisson arrivals; grids 64^3,
     96x64x96, 128x96x128, 192x128x192; Re_tau ~ U[180, 2000]; DNS/WMLES
-    50/50; 200-2000 steps times `scale`) through the serving loop with
-    continuous batching at each slot count in `slot_counts` (cases running
-    at once, one stream each), and the same cases one at a time (a server
-    with a single slot, same arrivals)."""
+    50/50; 200-2000 steps times `scale`) through the serving loop: the same
+    cases one at a time (a server with a single slot, same arrivals), then
+    each leg of `legs`: ("streams", S) continuous batching over S per-case
+    streams (cases running at once, one captured step graph each), or
+    ("groups", S, B, C) cases of one size class stepped together by the
+    batched step, groups of up to B members on S streams, stretches of at
+    most `chunk` steps, only cases of at most C unique cells batched (0: all;
+    larger cases on their own streams)."""
     from channel_flow import serving
     # steps and arrival gaps scale together: the offered load is the full ensemble's
     wl = serving.ensemble_workload(n=n_cases, seed=seed, steps_scale=scale, mean_gap=0.05 * scale)
@@ -206,21 +210,44 @@ def serving_bench(scale, slot_counts, budget_gb, n_cases=64, seed=5):
     recs1, summ1 = serving.serve(wl, budget, max_active=1, fields=fields)
     seq = serving.latency_report(recs1, summ1)
     log("serving: one-at-a-time run done in %.1f s: %s" % (time.perf_counter() - t0, seq))
-    by_slots = {}
-    for k in slot_counts:
+    runs = {}
+    for leg in legs:
         t0 = time.perf_counter()
-        recs, summ = serving.serve(wl, budget, max_active=k, fields=fields)
-        by_slots[k] = serving.latency_report(recs, summ)
-        log("serving: batched run, %d slots, done in %.1f s: %s" % (k, time.perf_counter() - t0, by_slots[k]))
-    best = max(by_slots, key=lambda k: by_slots[k]["mcell_per_s"])
-    mix = by_slots[best]
-    return {"workload": "%d cases, Poisson arrivals mean gap 50 ms, steps 200-2000, both x %g" % (n_cases, scale),
+        if leg[0] == "streams":
+            key = "streams_%d" % leg[1]
+            recs, summ = serving.serve(wl, budget, max_active=leg[1], fields=fields)
+        else:
+            _, sl, b, mc = leg
+            key = "groups_%dx%d" % (sl, b) + ("_upto_%d_cells" % mc if mc else "")
+            recs, summ = serving.serve(wl, budget, max_active=sl, fields=fields, batch=b, chunk=chunk,
+                                       batch_max_cells=mc)
+        runs[key] = serving.latency_report(recs, summ)
+        log("serving: %s done in %.1f s: %s" % (key, time.perf_counter() - t0, runs[key]))
+    best = max(runs, key=lambda k: runs[k]["mcell_per_s"])
+    mix = runs[best]
+    return {"workload": "%d cases, Poisson arrivals mean gap %g ms, steps %d-%d per case" % (
+                n_cases, 50 * scale, max(1, round(200 * scale)), max(1, round(2000 * scale))),
+            "steps_scale": scale,
             "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default (4)"),
-            "max_active": best, "hbm_budget_gb": budget_gb,
+            "best": best, "hbm_budget_gb": budget_gb, "group_chunk": chunk,
             "batched": mix, "one_at_a_time": seq,
-            "batched_by_slots": {str(k): v for k, v in by_slots.items()},
+            "runs": runs,
             "throughput_gain": round(mix["mcell_per_s"] / seq["mcell_per_s"], 3),
             "case_p99_gain": round(seq["case_s_p99"] / mix["case_s_p99"], 3)}
+
+
+def parse_legs(spec):
+    """'streams:8,groups:8:4:0' -> [("streams", 8), ("groups", 8, 4, 0)]"""
+    out = []
+    for item in spec.split(","):
+        f = item.split(":")
+        if f[0] == "streams" and len(f) == 2:
+            out.append(("streams", int(f[1])))
+        elif f[0] == "groups" and len(f) in (3, 4):
+            out.append(("groups", int(f[1]), int(f[2]), int(f[3]) if len(f) == 4 else 0))
+        else:
+            raise ValueError("bad serving leg %r" % item)
+    return out
 
 
 def serving_child(a):
@@ -232,7 +259,8 @@ def serving_child(a):
     if a.serve_hw_queues > 0:
         env["GPU_MAX_HW_QUEUES"] = str(min(a.serve_hw_queues, 32))
     cmd = [sys.executable, os.path.abspath(__file__), "--serve-only", "--serve-scale", str(a.serve_scale),
-           "--serve-slots", a.serve_slots, "--serve-budget-gb", str(a.serve_budget_gb)]
+           "--serve-legs", a.serve_legs, "--serve-budget-gb", str(a.serve_budget_gb),
+           "--serve-chunk", str(a.serve_chunk)]
     r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=1200)
     if r.returncode != 0:
         raise RuntimeError("serving leg failed with status %d" % r.returncode)
@@ -354,8 +382,11 @@ def main():
     ap.add_argument("--no-others", action="store_true", help="skip the 64^3 and LES step timings")
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="steps of the serving ensemble relative to 200-2000 per case")
-    ap.add_argument("--serve-slots", type=str, default="4,8",
-                    help="comma-separated slot counts (cases running at once) of the batched runs")
+    ap.add_argument("--serve-legs", type=str, default="streams:8,groups:8:4:0,groups:8:4:1572864",
+                    help="serving runs beside the one-at-a-time run: streams:S (S per-case streams) or "
+                         "groups:S:B[:C] (size-class groups of up to B members on S streams, cases of at most "
+                         "C unique cells batched, 0 all)")
+    ap.add_argument("--serve-chunk", type=int, default=25, help="steps per stretch of a group")
     ap.add_argument("--serve-hw-queues", type=int, default=0,
                     help="hardware queues of the serving process (GPU_MAX_HW_QUEUES, at most 32; 0: the "
                          "runtime's default).  Measured at 64 cases: 16 queues with 4, 8 or 16 slots gave "
@@ -382,8 +413,7 @@ def main():
     from channel_flow import cases
     cells = tuple(int(c) for c in a.cells.split(","))
     if a.serve_only:
-        slots = [int(k) for k in a.serve_slots.split(",")]
-        print(json.dumps(serving_bench(a.serve_scale, slots, a.serve_budget_gb)))
+        print(json.dumps(serving_bench(a.serve_scale, parse_legs(a.serve_legs), a.serve_budget_gb, a.serve_chunk)))
         return
     if a.dsm_only:
         print(json.dumps({"dsm": dsm_steps(cells, a.steps, a.warmup)}))

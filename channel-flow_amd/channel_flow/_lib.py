@@ -97,7 +97,7 @@ SIGNATURES = {
     "cf_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CfParams), _dp, _dp, _dp, _dp, _dp,
                                         _dp, ctypes.c_int, ctypes.c_double,
                                         ctypes.POINTER(ctypes.c_longlong)]),
-    "cf_server_set_batching": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "cf_server_set_batching": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_longlong]),
     "cf_server_start": (ctypes.c_int, [ctypes.c_void_p]),
     "cf_server_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "cf_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),

@@ -54,9 +54,10 @@ class Server:
     cases running at once (one HIP stream each); depth: steps in flight per
     case.  batch > 0: cases of one size class are stepped together, up to
     ``batch`` per group, in stretches of at most ``chunk`` steps (one stream
-    per group)."""
+    per group); when ``batch_max_cells`` is nonzero, only cases of at most
+    that many unique cells."""
 
-    def __init__(self, hbm_budget, max_active=8, depth=3, device=0, batch=0, chunk=25):
+    def __init__(self, hbm_budget, max_active=8, depth=3, device=0, batch=0, chunk=25, batch_max_cells=0):
         self.lib = _lib.load()
         self.device = device
         h = ctypes.c_void_p()
@@ -65,7 +66,8 @@ class Server:
         self.h = h
         self.cases = {}
         if batch:
-            _lib.check(self.lib.cf_server_set_batching(self.h, int(batch), int(chunk)), "cf_server_set_batching")
+            _lib.check(self.lib.cf_server_set_batching(self.h, int(batch), int(chunk), int(batch_max_cells)),
+                       "cf_server_set_batching")
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -196,10 +198,15 @@ class _FieldCache:
         return self.d[key]
 
 
-def serve(workload, hbm_budget, max_active, depth=3, timeout=1200.0, fields=None):
-    """Run a workload through one server; per-case records and the summary."""
+def serve(workload, hbm_budget, max_active, depth=3, timeout=1200.0, fields=None, batch=0, chunk=25,
+          batch_max_cells=0):
+    """Run a workload through one server; per-case records and the summary.
+    batch > 0 steps the cases of one size class together (up to ``batch``
+    members per group, stretches of at most ``chunk`` steps, cases of at most
+    ``batch_max_cells`` unique cells when nonzero)."""
     fields = fields or _FieldCache()
-    with Server(hbm_budget, max_active=max_active, depth=depth) as sv:
+    with Server(hbm_budget, max_active=max_active, depth=depth, batch=batch, chunk=chunk,
+                batch_max_cells=batch_max_cells) as sv:
         ids = [sv.submit(cfg, ns, arrival=t, field=fields(cfg)) for cfg, ns, t in workload]
         sv.start()
         if not sv.wait(timeout):
@@ -232,4 +239,6 @@ def latency_report(recs, summ):
         "peak_running": int(summ["peak_running"]),
         "peak_gb": float(summ["peak_bytes"] / 1e9),
         "waited_for_memory": int(summ["waited_for_memory"]),
+        "groups": int(summ["groups"]),
+        "peak_members": int(summ["peak_members"]),
     }

@@ -88,6 +88,8 @@ struct Arena {
     }
 };
 
+static long long unique_cells(const Dims &d) { return (long long)(d.nx - 2) * (d.ny - 1) * (d.nz - 2); }
+
 enum CaseState { PENDING = 0, PREPARING = 1, RUNNING = 2, DONE = 3, CANCELLED = 4, FAILED = 5 };
 
 struct Group;
@@ -153,6 +155,7 @@ struct Server {
     long long n_blocked_mem = 0;  // cases that reached the head of the queue and waited for memory
     long long last_blocked = -1;
     int batch_cap = 0, chunk = 25;  // batching: members per group (0: off), steps per stretch
+    long long batch_max_cells = 0;  // largest batched case in unique cells (0: any)
     std::vector<std::unique_ptr<Group>> groups;
     long long groups_formed = 0;
     int peak_members = 0;
@@ -250,9 +253,11 @@ void Server::retire(Case *c, int final_state) {
 
 // ---- groups ----
 // admission of a batchable case (under mu): a free member slot of a group of
-// its class, else a new group sized for the arrived cases of the class (up
-// to batch_cap, as many as the largest free arena region holds) on a free
-// stream slot.  The case is prepared at the group's next boundary.
+// its class, else a new group on a free stream slot with member slots for
+// the arrived cases of the class or, if more, as many as half of the largest
+// free arena region holds (never above batch_cap or what that region holds),
+// so that later arrivals of the class can join it.  The case is prepared at
+// the group's next boundary.
 bool Server::admit_to_group(Case *c, double t) {
     Group *g = nullptr;
     int m = -1;
@@ -264,11 +269,15 @@ bool Server::admit_to_group(Case *c, double t) {
     }
     if (!g) {
         if (free_slots.empty()) return false;
+        // room for the arrived cases of the class, and for later arrivals
+        // to join as members: up to half of the largest free region
         int want = 0;
         for (Case *p : pending)
             if (p->bkey == c->bkey && p->arrival <= t && !p->cancel.load()) want++;
         const size_t fit = arena.largest() / c->fp;
-        const int cap = (int)std::min<size_t>((size_t)std::min(batch_cap, std::max(want, 1)), fit);
+        const size_t spare = std::max<size_t>(fit / 2, 1);
+        const int cap = (int)std::min<size_t>(std::min<size_t>((size_t)batch_cap, fit),
+                                              std::max<size_t>((size_t)want, spare));
         size_t off = 0;
         if (cap < 1 || !arena.alloc((size_t)cap * c->fp, &off)) {
             if (c->id != last_blocked) { n_blocked_mem++; last_blocked = c->id; }
@@ -510,7 +519,8 @@ void Server::scheduler() {
                 Case *c = pending.front();
                 if (c->cancel.load()) { pending.pop_front(); c->state = CANCELLED; c->t_done = t; cv_done.notify_all(); continue; }
                 if (c->arrival > t) break;
-                if (batch_cap > 0 && !c->bkey.empty()) {
+                if (batch_cap > 0 && !c->bkey.empty() &&
+                    (batch_max_cells == 0 || unique_cells(c->host->d) <= batch_max_cells)) {
                     if (!admit_to_group(c, t)) break;
                     pending.pop_front();
                     busy = true;
@@ -649,11 +659,15 @@ int cf_server_create(int device, long long hbm_budget, int max_active, int depth
     return 0;
 }
 
-int cf_server_set_batching(cf_server *sv, int max_members, int chunk) {
-    if (!sv || max_members < 0 || chunk < 1) { set_error("cf_server_set_batching: bad arguments"); return -1; }
+int cf_server_set_batching(cf_server *sv, int max_members, int chunk, long long max_cells) {
+    if (!sv || max_members < 0 || chunk < 1 || max_cells < 0) {
+        set_error("cf_server_set_batching: bad arguments");
+        return -1;
+    }
     if (sv->s.started.load()) { set_error("cf_server_set_batching: server already started"); return -1; }
     sv->s.batch_cap = max_members;
     sv->s.chunk = chunk;
+    sv->s.batch_max_cells = max_cells;
     return 0;
 }
 

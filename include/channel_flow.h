@@ -237,9 +237,10 @@ int cf_server_submit(cf_server *s, const cf_params *p, const double *x, const do
  * but nu, dPdx, dPdz, u_centerline and tau_mean_eqwm shared, the same grid;
  * see cf_batch_create for the classes taken) are stepped together, up to
  * max_members per group, in stretches of at most chunk steps; cases join a
- * running group between stretches.  max_members 0 (the default): one case
- * per stream. */
-int cf_server_set_batching(cf_server *s, int max_members, int chunk);
+ * running group between stretches.  Only cases of at most max_cells unique
+ * cells are batched (0: no limit); larger ones run one per stream, as every
+ * case does with max_members 0 (the default). */
+int cf_server_set_batching(cf_server *s, int max_members, int chunk, long long max_cells);
 /* start the clock, the scheduler and the admission worker (cases may be
  * submitted before and after) */
 int cf_server_start(cf_server *s);

@@ -20,7 +20,7 @@ def _workload():
     rng = np.random.default_rng(21)
     wl, t = [], 0.0
     for q in range(14):
-        t += rng.exponential(0.004)
+        t += rng.exponential(0.0015)
         g = (64, 64, 64) if q % 3 else (96, 64, 96)
         wm = bool(q % 2)
         c = ensemble_case(g, float(rng.uniform(180.0, 2000.0)), wm, name="s%02d" % q)

@@ -3,7 +3,8 @@
 in equal numbers, Re_tau spread over 180..2000, seeded Poisson arrivals.
 An HBM budget of 1.5 times the largest case makes cases queue for memory.
 Every case gives the same bits in the mix as alone through the single-case
-path.  Step counts are cut to 8..24 per case: the identity does not depend
+path, with each case on its own stream and with cases of one size class
+stepped together in groups.  Step counts are cut to 8..24 per case: the identity does not depend
 on the count, and the throughput of the full counts is measured elsewhere."""
 import numpy as np
 import pytest
@@ -40,15 +41,17 @@ def alone():
     return out
 
 
-def test_c5_grid_set_same_alone_and_in_mix_under_budget(alone):
+@pytest.mark.parametrize("batch", [0, 4])
+def test_c5_grid_set_same_alone_and_in_mix_under_budget(alone, batch):
     wl = _workload()
     assert sorted({c.cells for c, _, _ in wl}) == sorted(GRIDS)
     assert sum(c.precision == "mixed" for c, _, _ in wl) == len(wl) // 2
     fps = [footprint(cfg) for cfg, _, _ in wl]
     budget = int(max(fps) * 1.5)
     assert budget < sum(fps)
-    print("c5 mix: %d cases, budget %.2f GB, footprints %s" % (len(wl), budget / 1e9, fps), flush=True)
-    with Server(budget, max_active=4, depth=3) as sv:
+    print("c5 mix: %d cases, batch %d, budget %.2f GB, footprints %s" % (len(wl), batch, budget / 1e9, fps),
+          flush=True)
+    with Server(budget, max_active=4, depth=3, batch=batch, chunk=5) as sv:
         ids = [sv.submit(cfg, n, arrival=t, field=cases.initial_field(cfg)) for cfg, n, t in wl]
         sv.start()
         assert sv.wait(300.0)
