@@ -772,13 +772,15 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
         for (int e = threadIdx.x + MAXL * WG; e < rb * mh; e += WG, ek.next()) {
             const int kx = ek.q, lr = ek.r;
             const int r = row0 + lr;
-            Cx<T> v = {(T)0, (T)0};
+            T vx = (T)0, vy = (T)0;
             if (r < nrows) {
                 int jp, kp;
                 small_divmod(r, d.nzp, inz, jp, kp);
-                v = spec[((int64_t)jp * mh + kx) * d.nzp + kp];
+                const Cx<T> v = spec[((int64_t)jp * mh + kx) * d.nzp + kp];
+                vx = v.x;
+                vy = v.y;
             }
-            lds[lr * ls + fpad<T>(kx)] = v;
+            lds[lr * ls + fpad<T>(kx)] = Cx<T>{vx, vy};
         }
     }
     // row table of the output loop

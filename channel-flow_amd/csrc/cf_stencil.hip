@@ -542,15 +542,14 @@ struct GhostSet {
 };
 template <typename S>
 __global__ void k_ghosts(Dims d, GhostSet gs, double dyg_bot, double dyg_top, int64_t cs) {
-    for (int f = 0; f < gs.n; f++) {
-        gs.F[f] = case_ptr(gs.F[f], cs);
-        gs.alpha[f] = case_ptr(gs.alpha[f], cs);
-    }
+    // the argument set is only read (a copy written with a runtime index
+    // would live in scratch memory); the case offset is applied at use
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t beg = 0;
     for (int f = 0; f < gs.n; f++) {
         if (g < gs.end[f]) {
-            ghost_one<S>(d, (S *)gs.F[f], gs.field[f], gs.mode[f], gs.alpha[f], dyg_bot, dyg_top, (int)(g - beg));
+            ghost_one<S>(d, case_ptr((S *)gs.F[f], cs), gs.field[f], gs.mode[f], case_ptr(gs.alpha[f], cs), dyg_bot,
+                         dyg_top, (int)(g - beg));
             return;
         }
         beg = gs.end[f];
@@ -1112,10 +1111,9 @@ k_smagorinsky(Dims d, Metrics m, const double *__restrict__ y, const double *__r
     const int k = 2 + tz;
     if (i > d.nx - 1 || j > d.nyg - 1) return;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
-    // profile of the deviation storage at j-1, j, j+1
-    const R ub0 = ubar ? (R)ubar[j - 1] : (R)0, ub1 = ubar ? (R)ubar[j] : (R)0,
-            ub2 = ubar ? (R)ubar[j + 1] : (R)0;
-#define u(I, J, K) ((R)U[i3(nx, nyg, I, J, K)] + ((J) == j ? ub1 : ((J) < j ? ub0 : ub2)))
+    // U of the deviation storage plus its profile at row J (a select over
+    // three profile values by J would be kept in scratch memory)
+#define u(I, J, K) ((R)U[i3(nx, nyg, I, J, K)] + (ubar ? (R)ubar[J] : (R)0))
 #define v(I, J, K) ((R)V[i3(nxg, ny, I, J, K)])
 #define w(I, J, K) ((R)W[i3(nxg, nyg, I, J, K)])
     // 1/(x(i)-x(i-1)) = rdxf(i); 1/(xg(I)-xg(I-1)) = rdxc(I-1); likewise y, z
