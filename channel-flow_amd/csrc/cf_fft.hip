@@ -366,26 +366,53 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
         rows[threadIdx.x] = t;
     }
     __syncthreads();
+    // Elements in chunks of XCH per thread: every global load of a chunk is
+    // issued before the first is used (no branch around them: an element
+    // outside the rows reads element 0 and is discarded), so a thread waits
+    // on memory once per chunk instead of once per element.
+    constexpr int XCH = 4;
     StepDiv el(threadIdx.x, WG, N);  // element (row lr = q, ip = r)
-#pragma unroll 4
-    for (int e = threadIdx.x; e < rb * N; e += WG, el.next()) {
-        const int lr = el.q, ip = el.r;
-        const XRow &t = rows[lr];
-        T dv = (T)0;
-        if (t.kr >= 0) {
-            const int i = 2 + ip;
+    for (int e0 = threadIdx.x; e0 < rb * N; e0 += XCH * WG) {
+        S u1[XCH], u0[XCH], v1[XCH], v0[XCH], w1[XCH], w0[XCH];
+        double rx[XCH];
+        int lrs[XCH], ips[XCH];
+#pragma unroll
+        for (int c = 0; c < XCH; c++, el.next()) {
+            const int lr = el.q, ip = el.r;
+            lrs[c] = lr;
+            ips[c] = ip;
+            const bool in = e0 + c * WG < rb * N;
+            const XRow &t = rows[min(lr, rb - 1)];  // read unconditionally (selects below, no branches)
+            const int ta = t.a, tb = t.b, tc = t.c, td = t.d;
+            const bool ok = in && t.kr >= 0;
             // the x and z ghost faces are read at their periodic images, so the
             // fields need no ghost fill between the RK update and this kernel
-            const int qu = t.a + ip, qum = ip == 0 ? t.a + (nx - 3) : qu - 1;  // U(1) = U(nx-1)
-            const int qv = t.b + ip, qw = t.c + ip, qwm = t.d + ip;
-            dv = ((T)U[qu] - (T)U[qum]) * (T)m.rdxf[i] + ((T)V[qv] - (T)V[qv - nxg]) * (T)t.r1 +
-                 ((T)W[qw] - (T)W[qwm]) * (T)t.r2;
-            if (ysrc) {
-                if (t.j == 2) dv += (T)ysrc[t.kr * d.nxp + ip];
-                if (t.j == nyg - 1) dv += (T)ysrc[(d.nzp + t.kr) * d.nxp + ip];
-            }
+            const int qu = ok ? ta + ip : 0, qum = !ok ? 0 : (ip == 0 ? ta + (nx - 3) : ta + ip - 1);  // U(1) = U(nx-1)
+            const int qv = ok ? tb + ip : nxg, qw = ok ? tc + ip : 0, qwm = ok ? td + ip : 0;
+            u1[c] = U[qu];
+            u0[c] = U[qum];
+            v1[c] = V[qv];
+            v0[c] = V[qv - nxg];
+            w1[c] = W[qw];
+            w0[c] = W[qwm];
+            rx[c] = m.rdxf[ok ? 2 + ip : 2];
         }
-        ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
+#pragma unroll
+        for (int c = 0; c < XCH; c++) {
+            if (e0 + c * WG >= rb * N) break;
+            const int lr = lrs[c], ip = ips[c];
+            const XRow &t = rows[lr];
+            T dv = (T)0;
+            if (t.kr >= 0) {
+                dv = ((T)u1[c] - (T)u0[c]) * (T)rx[c] + ((T)v1[c] - (T)v0[c]) * (T)t.r1 +
+                     ((T)w1[c] - (T)w0[c]) * (T)t.r2;
+                if (ysrc) {
+                    if (t.j == 2) dv += (T)ysrc[t.kr * d.nxp + ip];
+                    if (t.j == nyg - 1) dv += (T)ysrc[(d.nzp + t.kr) * d.nxp + ip];
+                }
+            }
+            ldr[2 * (lr * ls + fpad<T>(ip >> 1)) + (ip & 1)] = dv;
+        }
     }
     __syncthreads();
     if constexpr (MC > 0) lds_fft_c<MC, FFT_X_RADIX8, WG, -1>(lds, ls, rb, tw);
@@ -457,9 +484,13 @@ k_fftz(Dims d, Cx<T> *__restrict__ spec, FFTPlan pZ, const Cx<T> *__restrict__ t
 // GPU (the sweep is latency bound, and two lanes per mode double the waves
 // in flight).
 // ---------------------------------------------------------------------------
-constexpr int TH_LB = 16;
+// rows per block: 16 for an FP64 spectrum; TH_LB_F32 for an FP32 one (its
+// block values take half the registers, and a longer block halves the
+// number of load latencies a lane waits through)
+constexpr int TH_LB = 16, TH_LB_F32 = 24;
+template <typename T> constexpr int th_lb() { return sizeof(T) == 4 ? TH_LB_F32 : TH_LB; }
 size_t thomas_scratch_elems(const Dims &d) {
-    return (size_t)4 * ((d.nyp + TH_LB - 1) / TH_LB) * d.mh * d.nzp;
+    return (size_t)4 * ((d.nyp + std::min(TH_LB, TH_LB_F32) - 1) / std::min(TH_LB, TH_LB_F32)) * d.mh * d.nzp;
 }
 
 // pivot reciprocal 1 / (D(j) - dl(j-1) c'(j-1)); c'(j) = du(j) times it
@@ -471,6 +502,7 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
               const double *__restrict__ kzz, const double *__restrict__ dd, const double *__restrict__ dl,
               const double *__restrict__ du, int64_t cs) {
     extern __shared__ double sco[];  // dd | dl | du
+    constexpr int LB = th_lb<T>();
     spec = case_ptr(spec, cs);
     ckpt = case_ptr(ckpt, cs);
     const int nzp = d.nzp, nyp = d.nyp;
@@ -486,18 +518,18 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
     const bool zm = kx == 0 && kz == 0;
     const int64_t js = 2 * (int64_t)nmodes;          // row stride in T
     T *col = reinterpret_cast<T *>(spec) + t;        // this lane's part of mode q, row 0
-    const int nb = (nyp + TH_LB - 1) / TH_LB;
+    const int nb = (nyp + LB - 1) / LB;
     const int64_t cks = 2 * (int64_t)nmodes;         // checkpoint stride per block
     double *ck_c = ckpt, *ck_d = ckpt + (int64_t)nb * cks;
     auto Dj = [&](int j) {
         const double D = (s_dd[j] + kx2) + kz2;
         return (j == 0 && zm) ? 1.5 * D : D;
     };
-    T cur[TH_LB], nxt[TH_LB];
+    T cur[LB], nxt[LB];
     auto loadblk = [&](int b, T *dst) {
 #pragma unroll
-        for (int u = 0; u < TH_LB; u++) {
-            const int j = b * TH_LB + u;
+        for (int u = 0; u < LB; u++) {
+            const int j = b * LB + u;
             dst[u] = j < nyp ? col[j * js] : (T)0;
         }
     };
@@ -507,8 +539,8 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
     for (int b = 0; b < nb; b++) {
         if (b + 1 < nb) loadblk(b + 1, nxt);
 #pragma unroll
-        for (int u = 0; u < TH_LB; u++) {
-            const int j = b * TH_LB + u;
+        for (int u = 0; u < LB; u++) {
+            const int j = b * LB + u;
             if (j < nyp) {
                 const double lo = j > 0 ? s_dl[j - 1] : 0.0;
                 const double r = thomas_pivot(Dj(j), lo, cp);
@@ -519,7 +551,7 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
         ck_c[(int64_t)b * cks + t] = cp;
         ck_d[(int64_t)b * cks + t] = dp;
 #pragma unroll
-        for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
+        for (int u = 0; u < LB; u++) cur[u] = nxt[u];
     }
     // back substitution, block by block from the top
     double x = dp;  // row nyp-1: x = d'
@@ -533,10 +565,10 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
             c0n = b >= 2 ? ck_c[(int64_t)(b - 2) * cks + t] : 0.0;
             d0n = b >= 2 ? ck_d[(int64_t)(b - 2) * cks + t] : 0.0;
         }
-        double c[TH_LB], dv[TH_LB];
+        double c[LB], dv[LB];
 #pragma unroll
-        for (int u = 0; u < TH_LB; u++) {
-            const int j = b * TH_LB + u;
+        for (int u = 0; u < LB; u++) {
+            const int j = b * LB + u;
             c[u] = 0.0;
             dv[u] = 0.0;
             if (j < nyp) {
@@ -549,13 +581,13 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
             }
         }
 #pragma unroll
-        for (int u = TH_LB - 1; u >= 0; u--) {
-            const int j = b * TH_LB + u;
+        for (int u = LB - 1; u >= 0; u--) {
+            const int j = b * LB + u;
             if (j < nyp - 1) x = dv[u] - c[u] * x;
             if (j < nyp) col[j * js] = (T)x;
         }
 #pragma unroll
-        for (int u = 0; u < TH_LB; u++) cur[u] = nxt[u];
+        for (int u = 0; u < LB; u++) cur[u] = nxt[u];
     }
 }
 
