@@ -338,7 +338,10 @@ k_div_fftx(Dims d, Metrics m, const S *__restrict__ U, const S *__restrict__ V,
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, ls = fft_line<T>(M);
     const int nrows = d.nyp * d.nzp;
-    const int row0 = blockIdx.x * rb;
+    // row blocks remapped per XCD: neighbouring blocks share the 128-byte
+    // lines of their runs of consecutive kz (an FP32 run of 8 rows is half a
+    // line), and blocks on one XCD share its L2
+    const int row0 = xcd_remap(blockIdx.x, gridDim.x) * rb;
     const int nx = d.nx, nxg = d.nxg, nyg = d.nyg, ny = d.ny;
     T *ldr = reinterpret_cast<T *>(lds);
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
@@ -775,7 +778,10 @@ k_ifftx_proju(Dims d, Metrics m, const Cx<T> *__restrict__ spec, P *__restrict__
     Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
     const int M = MC ? MC : pM.n, N = 2 * M, mh = M + 1, ls = fft_line<T>(mh);
     const int nrows = d.nyp * d.nzp;
-    const int row0 = blockIdx.x * rb;
+    // row blocks remapped per XCD: neighbouring blocks share the 128-byte
+    // lines of their runs of consecutive kz (an FP32 run of 8 rows is half a
+    // line), and blocks on one XCD share its L2
+    const int row0 = xcd_remap(blockIdx.x, gridDim.x) * rb;
     Cx<T> *tw = lds + rb * ls;  // twiddles of the length-M transform
     for (int e = threadIdx.x; e < M; e += WG) tw[e] = twM[tw_base<T>(M) + e];
     const float inz = 1.0f / (float)d.nzp;
@@ -994,10 +1000,11 @@ int fft_rows_per_wg(int M, int elems_per_row, int esize) {
     if (rb > 16) rb = 16;
     // cap on rows per workgroup (more workgroups per CU); 8 rows = runs of 8
     // consecutive kz in the spectrum, 128 bytes in FP64.  Measured best at 8
-    // in both directions and precisions at 384x256x384 (the inverse took
-    // 0.322 ms with 10 rows, 0.298 ms with 8, in FP64)
+    // in both directions in FP64 at 384x256x384 (the inverse took 0.322 ms
+    // with 10 rows, 0.298 ms with 8) and forward in FP32; the FP32 inverse
+    // reads whole 128-byte lines with 16 rows (0.182 -> 0.179 ms)
     const bool inv = elems_per_row > M;
-    const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", 8);
+    const int cap = env_int(inv ? "CF_FFT_ROWS_INV" : "CF_FFT_ROWS_FWD", (inv && esize == 8) ? 16 : 8);
     if (cap > 0 && rb > cap) rb = cap;
     const int lim = (65536 / esize - M) / line_slots(elems_per_row, esize);  // + M twiddles
     if (rb > lim) rb = lim;
