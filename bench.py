@@ -187,6 +187,51 @@ def other_configs(steps=50, warmup=5):
     return res
 
 
+def batched_steps(members=8, steps=50, warmup=5):
+    """The batched step (cf_batch_*) of one size class against the same cases
+    stepped one after another: `members` DNS cases (FP64, Re_tau spread) on
+    each of the serving ensemble's grids 64^3 and 96x64x96 and `members`
+    wall-modelled LES cases (mixed) on 64^3, graph-replayed, inputs resident."""
+    from channel_flow import cases
+    from channel_flow.serving import ensemble_case
+    from channel_flow.solver import ChannelFlow, ChannelFlowBatch
+    res = {}
+    for name, grid, wm in (("dns64", (64, 64, 64), False), ("dns96x64x96", (96, 64, 96), False),
+                           ("wmles64", (64, 64, 64), True)):
+        cfgs = [ensemble_case(grid, 180.0 + 250.0 * q, wm, name="b%d" % q) for q in range(members)]
+        for q, c in enumerate(cfgs):
+            c.seed = 40 + q
+        b = ChannelFlowBatch(cfgs)
+        for m, c in zip(b.members, cfgs):
+            m.initialise(*cases.initial_field(c))
+        b.run(warmup)
+        t0 = time.perf_counter()
+        b.run(steps)
+        ms_b = (time.perf_counter() - t0) * 1e3 / steps
+        b.close()
+        singles = []
+        for c in cfgs:
+            g = ChannelFlow(c, device=0)
+            g.initialise(*cases.initial_field(c))
+            g.run(warmup, graph=True)
+            singles.append(g)
+        for g in singles:
+            g.sync()
+        t0 = time.perf_counter()
+        for g in singles:
+            g.run(steps, graph=True)
+            g.sync()
+        ms_s = (time.perf_counter() - t0) * 1e3 / steps
+        for g in singles:
+            g.close()
+        n = grid[0] * grid[1] * grid[2] * members
+        res[name] = {"members": members, "cells": list(grid), "batch_ms_per_step": round(ms_b, 4),
+                     "one_after_another_ms_per_step": round(ms_s, 4),
+                     "batch_mcell_per_s": round(n / ms_b / 1e3, 1), "gain": round(ms_s / ms_b, 3)}
+        log("batched %s x%d: %.3f ms per batch step, %.3f ms one after another" % (name, members, ms_b, ms_s))
+    return res
+
+
 def serving_bench(scale, legs, budget_gb, chunk=25, n_cases=64, seed=5):
     """Ensemble of channel cases (seeded Poisson arrivals; grids 64^3,
     96x64x96, 128x96x128, 192x128x192; Re_tau ~ U[180, 2000]; DNS/WMLES
@@ -397,6 +442,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB five times (counter calibration)")
     ap.add_argument("--copy-ceiling", action="store_true", help="only measure a 1 GiB device copy")
+    ap.add_argument("--batch-only", action="store_true", help="only time the batched step of one size class")
     ap.add_argument("--dsm-only", action="store_true",
                     help="only time RK3 steps with the dynamic Smagorinsky model on --cells")
     a = ap.parse_args()
@@ -417,6 +463,9 @@ def main():
         return
     if a.dsm_only:
         print(json.dumps({"dsm": dsm_steps(cells, a.steps, a.warmup)}))
+        return
+    if a.batch_only:
+        print(json.dumps({"batched_steps": batched_steps()}))
         return
     ncell = cells[0] * cells[1] * cells[2]
     mh_ratio = (cells[0] // 2 + 1) * 2.0 / cells[0]
@@ -493,6 +542,7 @@ def main():
     if not a.no_others:
         out["other_configs"] = other_configs()
         out["other_configs"]["dns384_dsm_fp64"] = dsm_steps(cells, 5)
+        out["batched_steps"] = batched_steps()
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cells)
