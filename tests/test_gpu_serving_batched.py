@@ -6,7 +6,8 @@ finished member leaves without stopping the others.  Every case gives the
 same bits as alone through the single-case path.  The workload mixes the
 ensemble's DNS (FP64) and WMLES (mixed) classes on two grids with step
 counts 6..17 and seeded arrivals, so groups form, fill, drain and refill;
-one case is cancelled while its group runs."""
+one case is cancelled while its group runs, and a cell limit keeps the
+larger class out of the groups."""
 import numpy as np
 import pytest
 
@@ -75,3 +76,25 @@ def test_cancel_inside_a_group():
         states = [sv.status(c)[0] for c in ids]
     assert states[1] == "cancelled"
     assert all(st == "done" for q, st in enumerate(states) if q != 1)
+
+
+def test_batching_limited_to_small_classes(alone):
+    """With a cell limit below the 96x64x96 grid, only the 64^3 classes form
+    groups; the 96x64x96 cases run one per stream.  Every case still gives
+    the bits it gives alone."""
+    wl = _workload()
+    n64 = sum(1 for c, _, _ in wl if c.cells == (64, 64, 64))
+    with Server(int(24e9), max_active=6, depth=3, batch=4, chunk=4, batch_max_cells=64 * 64 * 64) as sv:
+        ids = [sv.submit(cfg, n, arrival=t, field=cases.initial_field(cfg)) for cfg, n, t in wl]
+        sv.start()
+        assert sv.wait(300.0)
+        s = sv.summary()
+        res = [sv.result(c) for c in ids]
+    print("limited batching: %s" % s, flush=True)
+    for q, (U, V, W, sc) in enumerate(res):
+        Ua, Va, Wa, sca = alone[q]
+        for a, b in ((U, Ua), (V, Va), (W, Wa)):
+            assert np.array_equal(a, b), wl[q][0].name
+        assert sc["t"] == sca["t"] and sc["dt"] == sca["dt"], wl[q][0].name
+    assert 2 <= s["groups"] <= n64  # groups of the two 64^3 classes only
+    assert s["peak_members"] >= 2
