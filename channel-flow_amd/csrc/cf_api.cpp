@@ -598,6 +598,10 @@ void Ctx<S>::step_head() {
 template <typename S>
 int Ctx<S>::run_steps(int nsteps, bool use_graph) {
     HCK(hipSetDevice(dev));
+    // the launch-error state is per thread: an error left by an earlier,
+    // already reported call on this thread (another context's) is not this
+    // context's, so it is cleared before this call's launches
+    (void)hipGetLastError();
     if (batch_member && !batch_running) {
         set_error("a batch member is stepped through its batch (cf_batch_run_steps)");
         return -1;

@@ -173,6 +173,13 @@ struct Server {
 };
 
 static int prepare_case(Server *sv, Case *c) {
+    // With batching on, the scheduler captures the groups' step graphs: no
+    // HIP call of this thread may overlap such a capture (a synchronous copy
+    // during it is refused and invalidates it), so the whole preparation
+    // holds the scheduler's lock.  Without batching only this thread
+    // captures, and only the capture below takes the lock.
+    std::unique_lock<std::mutex> all(sv->gpu_mu, std::defer_lock);
+    if (sv->batch_cap > 0) all.lock();
     if (hipSetDevice(sv->dev) != hipSuccess) { c->err = "hipSetDevice failed"; return -1; }
     c->slab.base = sv->base + c->off;
     c->slab.cap = c->fp;
@@ -189,7 +196,8 @@ static int prepare_case(Server *sv, Case *c) {
     }
     int rc;
     {   // no HIP call of the scheduler overlaps a stream capture
-        std::lock_guard<std::mutex> g(sv->gpu_mu);
+        std::unique_lock<std::mutex> g(sv->gpu_mu, std::defer_lock);
+        if (!all.owns_lock()) g.lock();
         rc = x->prepare_graphs();
     }
     if (rc || x->sync()) {
