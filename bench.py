@@ -255,6 +255,8 @@ def serving_bench(scale, legs, budget_gb, chunk=25, n_cases=64, seed=5):
     recs1, summ1 = serving.serve(wl, budget, max_active=1, fields=fields)
     seq = serving.latency_report(recs1, summ1)
     log("serving: one-at-a-time run done in %.1f s: %s" % (time.perf_counter() - t0, seq))
+    if seq["done"] != seq["cases"]:
+        raise RuntimeError("one-at-a-time serving run finished %d of %d cases" % (seq["done"], seq["cases"]))
     runs = {}
     for leg in legs:
         t0 = time.perf_counter()
@@ -268,6 +270,8 @@ def serving_bench(scale, legs, budget_gb, chunk=25, n_cases=64, seed=5):
                                        batch_max_cells=mc)
         runs[key] = serving.latency_report(recs, summ)
         log("serving: %s done in %.1f s: %s" % (key, time.perf_counter() - t0, runs[key]))
+        if runs[key]["done"] != runs[key]["cases"]:
+            raise RuntimeError("serving run %s finished %d of %d cases" % (key, runs[key]["done"], runs[key]["cases"]))
     best = max(runs, key=lambda k: runs[k]["mcell_per_s"])
     mix = runs[best]
     return {"workload": "%d cases, Poisson arrivals mean gap %g ms, steps %d-%d per case" % (
