@@ -41,6 +41,7 @@ def algorithmic_bytes(name, n_cells, esz, mh_ratio):
         "fftz_fwd": 2 * spec,
         "fftz_inv": 2 * spec,
         "thomas": 2 * spec,
+        "zsolve": 2 * spec,              # fused z transforms + y solve, both launches
         "ifftx_proju": spec + f + 2 * f,  # read spectrum, write phi, read+write U
         "project_vw": 5 * f,             # read phi,V,W, write V,W
     }

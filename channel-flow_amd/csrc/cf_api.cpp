@@ -273,7 +273,7 @@ int Ctx<S>::init() {
             if (A(&st[b][c], n3[c])) return -1;
     for (int c = 0; c < 3; c++)
         if (A(&G[c], n3[c])) return -1;
-    if (thomas_uses_scratch(d.mh * d.nzp) && A(&cscr, thomas_scratch_elems(d))) return -1;
+    if (ysolve_uses_scratch(d) && A(&cscr, ysolve_scratch_elems(d))) return -1;
     if (A(&mfpart, rhs_rk_tiles(d, true))) return -1;
     if (A(&dtfp, (size_t)ifftx_blocks(d, sizeof(S)) + 2 * (size_t)project_vw_blocks(d, sizeof(S)) +
                      dt_fused_mid_doubles()))
@@ -447,15 +447,21 @@ void Ctx<S>::poisson(S *const *F, S *phi, const Scalars *corr, const DtParts *dt
     tic("div_fftx");
     launch_div_fftx<S, S>(d, met, F[0], F[1], F[2], spec, pM, twM, twN, stream);
     toc();
-    tic("fftz_fwd");
-    launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
-    toc();
-    tic("thomas");
-    launch_thomas<S>(d, spec, cscr, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
-    toc();
-    tic("fftz_inv");
-    launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
-    toc();
+    if (zsolve_fused(d)) {
+        tic("zsolve");
+        launch_zsolve<S>(d, spec, cscr, pZ, twZ, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
+        toc();
+    } else {
+        tic("fftz_fwd");
+        launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
+        toc();
+        tic("thomas");
+        launch_thomas<S>(d, spec, cscr, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
+        toc();
+        tic("fftz_inv");
+        launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
+        toc();
+    }
     tic("ifftx_proju");
     launch_ifftx_proju<S, S, S>(d, met, spec, phi, F[0], pM, twM, twN, (double)(d.nxp * d.nzp), stream, corr, dtp,
                                 gcopy);
@@ -933,9 +939,13 @@ int Ctx<S>::compute_pressure(double *P) {
                               h.prm.omega_z, stream, true);
         launch_pressure_bc<S>(d, F[1], c, Rv, bcs, ysrc, stream);
         launch_div_fftx<double, S>(d, met, Ru, Rv, Rw, spec, pM, twM, twN, stream, ysrc);
-        launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
-        launch_thomas<S>(d, spec, cscr, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
-        launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
+        if (zsolve_fused(d)) {
+            launch_zsolve<S>(d, spec, cscr, pZ, twZ, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
+        } else {
+            launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
+            launch_thomas<S>(d, spec, cscr, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
+            launch_fftz<S>(d, spec, pZ, twZ, +1, stream);
+        }
         launch_ifftx_proju<S, S, double>(d, met, spec, pcell, (S *)nullptr, pM, twM, twN, (double)(d.nxp * d.nzp),
                                          stream);
         toc();

@@ -500,8 +500,19 @@ size_t thomas_scratch_elems(const Dims &d) {
     return (size_t)4 * ((d.nyp + std::min(TH_LB, TH_LB_F32) - 1) / std::min(TH_LB, TH_LB_F32)) * d.mh * d.nzp;
 }
 
-// pivot reciprocal 1 / (D(j) - dl(j-1) c'(j-1)); c'(j) = du(j) times it
-__device__ __forceinline__ double thomas_pivot(double D, double lo, double cprev) { return 1.0 / (D - lo * cprev); }
+// pivot reciprocal 1 / (D(j) - dl(j-1) c'(j-1)); c'(j) = du(j) times it.
+// The hardware reciprocal refined by two Newton steps (within an ulp of the
+// rounded quotient; the pivots are never zero, subnormal or infinite): 7
+// dependent operations from c'(j-1) to c'(j) where an IEEE division needs 12,
+// and this chain is what the row-by-row sweeps wait on.
+__device__ __forceinline__ double thomas_pivot(double D, double lo, double cprev) {
+    const double x = D - lo * cprev;
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -595,6 +606,293 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
         }
 #pragma unroll
         for (int u = 0; u < LB; u++) cur[u] = nxt[u];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z transforms fused with the y solve over kx slabs (projection.f90:84-113:
+// the z part of the 2-D DFT, the zgtsv per mode, the inverse).  A slab is
+// the nyp rows of nzp consecutive kz at one kx; two workgroups share it, one
+// per half of its rows, and nothing outside the slab is read or written.
+// Rows go in batches of ZR, the next batch's loads in flight while the
+// current one is worked on.
+//   Forward: a batch's rows are staged in LDS (thread kz holds element kz),
+//   transformed along z with the same Stockham passes as k_fftz, and thread
+//   kz advances the elimination of mode (kx, kz) through them, real and
+//   imaginary parts sharing the pivots.  The eliminated right-hand side is
+//   stored in place of the row, and the pivot ratio of the batch's last row
+//   goes to an FP64 checkpoint.
+//   Backward: batches from the middle of the slab to the wall.  A batch's
+//   stored values are loaded, the pivot ratios recomputed from the
+//   checkpoint of the batch nearer the wall (the same operations on the same
+//   values, so the same ratios), and the substitution writes the solution
+//   into the LDS lines, which are transformed back along z and stored.
+// The spectrum moves four times (read, write; read, write) instead of seven
+// (z transform 2, lane solve 3, inverse z transform 2).  Arithmetic is FP64;
+// with FP32 storage the eliminated values are rounded to FP32 like the
+// right-hand side and the solution around them.
+// ---------------------------------------------------------------------------
+// Rows per batch ZR and row groups G: with G = 2 (fixed-plan lengths, 2 n
+// threads) thread t loads and stores the rows u = g, g + 2, ... of column
+// t mod n (g = t / n), so a batch of 16 rows costs each thread as many
+// registers as 8 rows with one thread per column; the sweeps run on the
+// first n threads.  G = 1: one thread per column (run-time plans, n <= WG),
+// 8 rows.
+//
+// halves = 1: one workgroup per slab eliminates rows 0..nyp-1 from the
+// bottom wall (c', d' in projection.f90's zgtsv order, the operations of
+// k_thomas_lane, so in FP64 storage the same bits as the unfused chain).
+// halves = 2: the slab is solved from both walls (a twisted factorisation):
+// half 0 eliminates rows 0..m-1 as above, half 1 rows nyp-1..m upward from
+// the top wall, x_j = e'_j - b'_j x_(j-1) with
+//   q = 1 / (D_j - du_j b'_(j+1)),  b'_j = dl_(j-1) q,  e'_j = (r_j - du_j e'_(j+1)) q,
+// m = ceil(nyp/2).  The backward kernel joins the halves at rows m-1 and m,
+//   x_(m-1) = (d'_(m-1) - c'_(m-1) e'_m) / (1 - c'_(m-1) b'_m),  x_m = e'_m - b'_m x_(m-1),
+// which both halves compute from the same values, so both get the same
+// bits, and substitutes outward: 2 mh workgroups of half the rows.  The
+// forward kernel also keeps each half's last d' (e') at FP64 in the scratch
+// after the checkpoints: the backward kernel reads the other half's from
+// there, since the other workgroup overwrites that row of the spectrum with
+// its solution.
+// twiddles, the y operator and the z wavenumbers into LDS: the sweeps then
+// read no global memory, so their first row waits for no load in flight
+template <typename T, int WG>
+__device__ __forceinline__ void zs_stage_consts(Dims d, int n, Cx<T> *tw, const Cx<T> *__restrict__ twZ,
+                                                double *s_dd, double *s_dl, double *s_du, double *s_kz,
+                                                const double *__restrict__ dd, const double *__restrict__ dl,
+                                                const double *__restrict__ du, const double *__restrict__ kzz) {
+    for (int e = threadIdx.x; e < n; e += WG) { tw[e] = twZ[tw_base<T>(n) + e]; s_kz[e] = kzz[e]; }
+    for (int e = threadIdx.x; e < d.nyp; e += WG) { s_dd[e] = dd[e]; s_dl[e] = dl[e]; s_du[e] = du[e]; }
+}
+
+// rows of a half, batches per half, and the row of batch b, line u
+struct ZsHalf {
+    int h, nyp, m, nh, nbh, zr;
+    __device__ ZsHalf(int h_, int nyp_, int zr_, int halves) : h(h_), nyp(nyp_), zr(zr_) {
+        m = halves == 2 ? (nyp + 1) / 2 : nyp;
+        nh = h == 0 ? m : nyp - m;
+        nbh = (nh + zr - 1) / zr;
+    }
+    __device__ int nl(int b) const { return min(zr, nh - b * zr); }
+    __device__ int row(int b, int u) const { return h == 0 ? b * zr + u : nyp - 1 - (b * zr + u); }
+};
+// checkpoint batches per half (half 0 has the more rows)
+__host__ __device__ inline int zs_ck_batches(int nyp, int zr, int halves) {
+    return ((halves == 2 ? (nyp + 1) / 2 : nyp) + zr - 1) / zr;
+}
+
+template <typename T, int WG, int NC, int ZR, int G>
+__global__ void __launch_bounds__(WG)
+k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPlan pZ,
+             const Cx<T> *__restrict__ twZ, const double *__restrict__ kxx, const double *__restrict__ kzz,
+             const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
+             int halves, int64_t cs) {
+    static_assert(G == 1 || (NC > 0 && WG == G * NC), "row groups need a fixed length");
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int RPT = ZR / G;  // rows per thread per batch
+    spec = case_ptr(spec, cs);
+    ckpt = case_ptr(ckpt, cs);
+    const int n = NC ? NC : pZ.n, ls = fft_line<T>(n) + 1, nyp = d.nyp;
+    Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
+    Cx<T> *tw = lds + ZR * ls;
+    double *s_dd = reinterpret_cast<double *>(tw + n), *s_dl = s_dd + nyp, *s_du = s_dl + nyp, *s_kz = s_du + nyp;
+    zs_stage_consts<T, WG>(d, n, tw, twZ, s_dd, s_dl, s_du, s_kz, dd, dl, du, kzz);
+    const int kx = halves == 2 ? (int)blockIdx.x >> 1 : (int)blockIdx.x;
+    const ZsHalf hf(halves == 2 ? (int)blockIdx.x & 1 : 0, nyp, ZR, halves);
+    const int g = G == 1 ? 0 : (int)threadIdx.x / n, kz = (int)threadIdx.x - g * n;
+    const bool act = kz < n;           // a column (all threads when G = 2)
+    const bool sweep = act && g == 0;  // runs the elimination of mode (kx, kz)
+    const double kx2 = kxx[kx];
+    const bool zm = kx == 0 && kz == 0;
+    const int64_t rs = (int64_t)d.mh * n;  // row stride in elements
+    Cx<T> *col = spec + (int64_t)kx * n + kz;
+    double *ck = ckpt + (int64_t)hf.h * zs_ck_batches(nyp, ZR, halves) * rs + (int64_t)kx * n + kz;
+    auto Dj = [&](int j) {
+        const double D = (s_dd[j] + kx2) + s_kz[kz];
+        return (j == 0 && zm) ? 1.5 * D : D;
+    };
+    Cx<T> cur[RPT], nxt[RPT];
+    auto loadblk = [&](int b, Cx<T> *dst) {
+        const int nl = hf.nl(b);
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const int u = g + G * i;
+            dst[i] = (act && u < nl) ? col[hf.row(b, u) * rs] : Cx<T>{(T)0, (T)0};
+        }
+    };
+    // half 0: p = c', (dr, di) = d'; half 1: p = b', (dr, di) = e'
+    double p = 0.0, dr = 0.0, di = 0.0;
+    loadblk(0, cur);
+    for (int b = 0; b < hf.nbh; b++) {
+        const int nl = hf.nl(b);
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                if (g + G * i < nl) lds[(g + G * i) * ls + fpad<T>(kz)] = cur[i];
+        }
+        if (b + 1 < hf.nbh) loadblk(b + 1, nxt);
+        __syncthreads();  // staged rows (and, at b = 0, the constants) visible
+        if constexpr (NC > 0) lds_fft_c<NC, true, WG, -1>(lds, ls, nl, tw);
+        else lds_fft<WG>(lds, ls, nl, pZ, tw, -1);
+        if (sweep) {
+#pragma unroll
+            for (int u = 0; u < ZR; u++) {
+                if (u < nl) {
+                    const int j = hf.row(b, u);
+                    const Cx<T> v = lds[u * ls + fpad<T>(kz)];
+                    // coupling to the row already eliminated, and to the next
+                    const double cin = hf.h == 0 ? (j > 0 ? s_dl[j - 1] : 0.0) : (j < nyp - 1 ? s_du[j] : 0.0);
+                    const double cout = hf.h == 0 ? s_du[j] : (j > 0 ? s_dl[j - 1] : 0.0);
+                    const double r = thomas_pivot(Dj(j), cin, p);
+                    dr = ((double)v.x - cin * dr) * r;
+                    di = ((double)v.y - cin * di) * r;
+                    p = cout * r;
+                    col[j * rs] = Cx<T>{(T)dr, (T)di};
+                }
+            }
+            ck[(int64_t)b * rs] = p;
+            if (b == hf.nbh - 1) {  // the half's middle row: the value the join needs
+                double *jn = ckpt + (int64_t)halves * zs_ck_batches(nyp, ZR, halves) * rs + (int64_t)kx * n + kz;
+                jn[(int64_t)(2 * hf.h) * rs] = dr;
+                jn[(int64_t)(2 * hf.h + 1) * rs] = di;
+            }
+        }
+        // G = 1: thread kz reads and next writes only its own LDS column, and
+        // the transform's last pass ends with a barrier; G = 2: the other
+        // group stages rows of the same columns next
+        if constexpr (G > 1) __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RPT; i++) cur[i] = nxt[i];
+    }
+}
+
+template <typename T, int WG, int NC, int ZR, int G>
+__global__ void __launch_bounds__(WG)
+k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, FFTPlan pZ,
+             const Cx<T> *__restrict__ twZ, const double *__restrict__ kxx, const double *__restrict__ kzz,
+             const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
+             int halves, int64_t cs) {
+    static_assert(G == 1 || (NC > 0 && WG == G * NC), "row groups need a fixed length");
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int RPT = ZR / G;
+    spec = case_ptr(spec, cs);
+    ckpt = case_ptr(ckpt, cs);
+    const int n = NC ? NC : pZ.n, ls = fft_line<T>(n) + 1, nyp = d.nyp;
+    Cx<T> *lds = reinterpret_cast<Cx<T> *>(smem_raw);
+    Cx<T> *tw = lds + ZR * ls;
+    double *s_dd = reinterpret_cast<double *>(tw + n), *s_dl = s_dd + nyp, *s_du = s_dl + nyp, *s_kz = s_du + nyp;
+    zs_stage_consts<T, WG>(d, n, tw, twZ, s_dd, s_dl, s_du, s_kz, dd, dl, du, kzz);
+    const int kx = halves == 2 ? (int)blockIdx.x >> 1 : (int)blockIdx.x;
+    const ZsHalf hf(halves == 2 ? (int)blockIdx.x & 1 : 0, nyp, ZR, halves);
+    const int g = G == 1 ? 0 : (int)threadIdx.x / n, kz = (int)threadIdx.x - g * n;
+    const bool act = kz < n;
+    const bool sweep = act && g == 0;
+    const double kx2 = kxx[kx];
+    const bool zm = kx == 0 && kz == 0;
+    const int64_t rs = (int64_t)d.mh * n;
+    Cx<T> *col = spec + (int64_t)kx * n + kz;
+    const int nbc = zs_ck_batches(nyp, ZR, halves);
+    const double *ck0 = ckpt + (int64_t)kx * n + kz;  // half 0's checkpoints
+    const double *ck = ck0 + (int64_t)hf.h * nbc * rs;
+    auto Dj = [&](int j) {
+        const double D = (s_dd[j] + kx2) + s_kz[kz];
+        return (j == 0 && zm) ? 1.5 * D : D;
+    };
+    Cx<T> cur[RPT], nxt[RPT];
+    auto loadblk = [&](int b, Cx<T> *dst) {
+        const int nl = hf.nl(b);
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const int u = g + G * i;
+            dst[i] = (act && u < nl) ? col[hf.row(b, u) * rs] : Cx<T>{(T)0, (T)0};
+        }
+    };
+    const int nb = hf.nbh;
+    loadblk(nb - 1, cur);
+    // the solution at the half's middle row: halves = 1, x(nyp-1) = d'(nyp-1);
+    // halves = 2, the join of x(m-1) and x(m) from the last c', d' of half 0
+    // and b', e' of half 1
+    const int m = hf.m, jmid = hf.h == 0 ? m - 1 : m;
+    double xr = 0.0, xi = 0.0, c0n = 0.0;
+    if (sweep) {
+        const double *jn = ck0 + (int64_t)halves * nbc * rs;
+        const double dmr = jn[0], dmi = jn[rs];
+        if (halves == 2) {
+            const ZsHalf h1(1, nyp, ZR, halves);
+            const double cm = ck0[(int64_t)(nbc - 1) * rs];  // half 0 has nbc batches
+            const double bm = ck0[((int64_t)nbc + h1.nbh - 1) * rs];
+            const double emr = jn[2 * rs], emi = jn[3 * rs];
+            const double den = 1.0 - cm * bm;
+            const double x1r = (dmr - cm * emr) / den;
+            const double x1i = (dmi - cm * emi) / den;
+            if (hf.h == 0) { xr = x1r; xi = x1i; }
+            else { xr = emr - bm * x1r; xi = emi - bm * x1i; }
+        } else {
+            xr = dmr;
+            xi = dmi;
+        }
+        c0n = nb >= 2 ? ck[(int64_t)(nb - 2) * rs] : 0.0;
+    }
+    const double xmr = xr, xmi = xi;
+    for (int b = nb - 1; b >= 0; b--) {
+        const int nl = hf.nl(b);
+        double pprev = c0n;  // c' (b') at the row before the batch, wall side
+        // e' / d' of the batch into the LDS lines (G = 1: each thread its own
+        // column; G = 2: both groups, read by the first after the barrier)
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                if (g + G * i < nl) lds[(g + G * i) * ls + fpad<T>(kz)] = cur[i];
+        }
+        if (b > 0) {
+            loadblk(b - 1, nxt);
+            c0n = (sweep && b >= 2) ? ck[(int64_t)(b - 2) * rs] : 0.0;
+        }
+        if constexpr (G > 1) __syncthreads();
+        if (sweep) {
+            double c[ZR];
+#pragma unroll
+            for (int u = 0; u < ZR; u++) {
+                c[u] = 0.0;
+                if (u < nl) {
+                    const int j = hf.row(b, u);
+                    const double cin = hf.h == 0 ? (j > 0 ? s_dl[j - 1] : 0.0) : (j < nyp - 1 ? s_du[j] : 0.0);
+                    const double cout = hf.h == 0 ? s_du[j] : (j > 0 ? s_dl[j - 1] : 0.0);
+                    const double r = thomas_pivot(Dj(j), cin, pprev);
+                    c[u] = cout * r;
+                    pprev = c[u];
+                }
+            }
+#pragma unroll
+            for (int u = ZR - 1; u >= 0; u--) {
+                if (u < nl) {
+                    const int j = hf.row(b, u);
+                    const Cx<T> dv = lds[u * ls + fpad<T>(kz)];
+                    if (j == jmid) {
+                        xr = xmr;
+                        xi = xmi;
+                    } else {
+                        xr = (double)dv.x - c[u] * xr;
+                        xi = (double)dv.y - c[u] * xi;
+                    }
+                    lds[u * ls + fpad<T>(kz)] = Cx<T>{(T)xr, (T)xi};
+                }
+            }
+        }
+        __syncthreads();
+        if constexpr (NC > 0) lds_fft_c<NC, true, WG, 1>(lds, ls, nl, tw);
+        else lds_fft<WG>(lds, ls, nl, pZ, tw, 1);
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const int u = g + G * i;
+                if (u < nl) col[hf.row(b, u) * rs] = lds[u * ls + fpad<T>(kz)];
+            }
+        }
+        // the next batch's staging writes only the rows this thread has just
+        // stored from, and the sweep reads them after the barrier above
+#pragma unroll
+        for (int i = 0; i < RPT; i++) cur[i] = nxt[i];
     }
 }
 
@@ -1128,6 +1426,65 @@ void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, 
                            du, KZ, P, launch_batch().cs);
 }
 
+// fused z transforms and y solve: one workgroup per kx slab, so only where
+// the slabs fill the GPU and a workgroup covers a row (env CF_ZSOLVE: 1 on,
+// 0 off)
+// (and where a workgroup's LDS, FP64 rows of the batch, twiddles and y
+// operator, fits in the 160 KiB of a CU)
+static int zsolve_rows(int n) { return (n == 384 || n == 256 || n == 192 || n == 128) ? 16 : 8; }
+bool zsolve_fused(const Dims &d) {
+    const size_t lds = ((size_t)zsolve_rows(d.nzp) * (fft_line<double>(d.nzp) + 1) + d.nzp) * 16 + 8 * (3 * (size_t)d.nyp + d.nzp);
+    if (d.nzp > 512 || d.nyp < 4 || lds > 160 * 1024) return false;
+    return env_int("CF_ZSOLVE", d.mh >= 128 ? 1 : 0) != 0;
+}
+// checkpoints: one c' per batch and mode, batches of at least 8 rows
+// FP64 checkpoints (one per batch, half and mode) and the middle values of
+// the halves (complex, one per half and mode), for batches of 8 rows or more
+static size_t zsolve_scratch_elems(const Dims &d) {
+    const size_t a = (size_t)2 * zs_ck_batches(d.nyp, 8, 2) + 4, b = (size_t)zs_ck_batches(d.nyp, 8, 1) + 2;
+    return std::max(a, b) * d.mh * d.nzp;
+}
+bool ysolve_uses_scratch(const Dims &d) { return zsolve_fused(d) || thomas_uses_scratch(d.mh * d.nzp); }
+size_t ysolve_scratch_elems(const Dims &d) { return std::max(thomas_scratch_elems(d), zsolve_scratch_elems(d)); }
+
+template <typename T, int WG, int NC, int ZR, int G>
+static void launch_zsolve_v(const Dims &d, Cx<T> *spec, double *cscr, const FFTPlan &pZ, const Cx<T> *twZ,
+                            const double *kxx, const double *kzz, const double *dd, const double *dl,
+                            const double *du, int halves, hipStream_t st) {
+    const int n = pZ.n;
+    const size_t shm = ((size_t)ZR * (fft_line<T>(n) + 1) + n) * sizeof(Cx<T>) + (3 * (size_t)d.nyp + n) * sizeof(double);
+    const dim3 grid(halves * d.mh, launch_batch().n);
+    const int64_t cs = launch_batch().cs;
+    hipLaunchKernelGGL((k_zsolve_fwd<T, WG, NC, ZR, G>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ, kxx, kzz,
+                       dd, dl, du, halves, cs);
+    hipLaunchKernelGGL((k_zsolve_bwd<T, WG, NC, ZR, G>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ, kxx, kzz,
+                       dd, dl, du, halves, cs);
+}
+
+// Forms measured at 384x256x384 (forward + backward): FP64, one workgroup per
+// slab with 16-row batches on 768 threads 0.397 ms, both walls with 8-row
+// batches on 384 threads 0.452 ms (386 half slabs on 256 CUs: 130 CUs take
+// two, so the slowest CU still does 256 rows); FP32 0.283 and 0.261 ms.
+template <typename T>
+void launch_zsolve(const Dims &d, Cx<T> *spec, double *cscr, const FFTPlan &pZ, const Cx<T> *twZ,
+                   const double *kxx, const double *kzz, const double *dd, const double *dl, const double *du,
+                   hipStream_t st) {
+    const int n = pZ.n;
+    const int halves = env_int("CF_ZS_HALVES", sizeof(T) == 4 ? 2 : 1) == 2 ? 2 : 1;
+    const bool groups = env_int("CF_ZS_G", sizeof(T) == 4 ? 1 : 2) == 2;
+#define CF_ZSV(WG, NC, ZR, G) launch_zsolve_v<T, WG, NC, ZR, G>(d, spec, cscr, pZ, twZ, kxx, kzz, dd, dl, du, halves, st)
+    switch (fft_static_sizes() ? n : 0) {
+        case 384: if (groups) CF_ZSV(768, 384, 16, 2); else CF_ZSV(384, 384, 8, 1); return;
+        case 256: if (groups) CF_ZSV(512, 256, 16, 2); else CF_ZSV(256, 256, 8, 1); return;
+        case 192: if (groups) CF_ZSV(384, 192, 16, 2); else CF_ZSV(256, 192, 8, 1); return;
+        case 128: if (groups) CF_ZSV(256, 128, 16, 2); else CF_ZSV(256, 128, 8, 1); return;
+        default:
+            if (n <= 256) CF_ZSV(256, 0, 8, 1);
+            else CF_ZSV(512, 0, 8, 1);
+    }
+#undef CF_ZSV
+}
+
 int ifftx_blocks(const Dims &d, int esize) {
     const int M = d.nxp / 2;
     const int rb = fft_rows_per_wg(M, M + 1, 2 * esize);
@@ -1181,6 +1538,12 @@ template void launch_thomas<double>(const Dims &, Cx<double> *, double *, const 
                                     const double *, const double *, const double *, hipStream_t);
 template void launch_thomas<float>(const Dims &, Cx<float> *, double *, const double *, const double *,
                                    const double *, const double *, const double *, hipStream_t);
+template void launch_zsolve<double>(const Dims &, Cx<double> *, double *, const FFTPlan &, const Cx<double> *,
+                                    const double *, const double *, const double *, const double *,
+                                    const double *, hipStream_t);
+template void launch_zsolve<float>(const Dims &, Cx<float> *, double *, const FFTPlan &, const Cx<float> *,
+                                   const double *, const double *, const double *, const double *,
+                                   const double *, hipStream_t);
 template void launch_ifftx_proju<double, double, double>(const Dims &, const Metrics &,
                                                          const Cx<double> *, double *, double *,
                                                          const FFTPlan &, const Cx<double> *,

@@ -152,6 +152,18 @@ size_t thomas_scratch_elems(const Dims &d);  // FP64 checkpoints of the lane-for
 template <typename T>
 void launch_thomas(const Dims &d, Cx<T> *spec, double *cscr, const double *kxx, const double *kzz,
                    const double *dd, const double *dl, const double *du, hipStream_t st);
+// forward z transform, y solve and inverse z transform in two launches (one
+// workgroup per kx slab) in place of launch_fftz, launch_thomas, launch_fftz
+// where zsolve_fused(d); needs the FP64 checkpoint scratch
+bool zsolve_fused(const Dims &d);
+template <typename T>
+void launch_zsolve(const Dims &d, Cx<T> *spec, double *cscr, const FFTPlan &pZ, const Cx<T> *twZ,
+                   const double *kxx, const double *kzz, const double *dd, const double *dl, const double *du,
+                   hipStream_t st);
+// the y solve's FP64 checkpoint scratch (lane form or fused form): whether
+// it is needed and its size in doubles
+bool ysolve_uses_scratch(const Dims &d);
+size_t ysolve_scratch_elems(const Dims &d);
 // U null: phi only, no projection of U; corr (optional): the U projection
 // also adds corr->corr (the mass-flow correction, time_integration.f90:155)
 // dtp (optional, dtp.pu non-null): per-workgroup maxima of compute_dt's

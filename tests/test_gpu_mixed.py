@@ -54,9 +54,11 @@ def test_mixed_tracks_fp64(nsteps):
 
 def test_mixed_lane_solve_and_fixed_plans_track_fp64(monkeypatch):
     """The mixed path with the kernels of the 384x256x384 workload (half
-    length 192, z length 64 fixed-plan transforms, lane y solve forced):
-    ten steps within the mixed tolerances of the FP64 path."""
+    length 192, z length 64 fixed-plan transforms, lane y solve forced and
+    the z transforms as launches of their own): ten steps within the mixed
+    tolerances of the FP64 path."""
     monkeypatch.setenv("CF_TH_LANE", "1")
+    monkeypatch.setenv("CF_ZSOLVE", "0")
     CF = solver()
     g64 = CF(cfg_of("fp64", cells=(384, 24, 64)))
     g32 = CF(cfg_of("mixed", cells=(384, 24, 64)))

@@ -119,8 +119,11 @@ def test_ten_rk3_steps_large_grid_kernels(cells, monkeypatch):
     """The kernels the 384x256x384 workload runs, on grids the oracle steps
     in seconds: x and z transform lengths compiled with fixed plans (half
     length 64, 32 or 192; z length 96, 384 or 64) and the lane form of the
-    y solve, forced here (it is chosen by default from 65,536 modes up)."""
+    y solve, forced here (it is chosen by default from 65,536 modes up), with
+    the z transforms and the y solve as separate launches (the fused form is
+    the default from 128 x wavenumbers up)."""
     monkeypatch.setenv("CF_TH_LANE", "1")
+    monkeypatch.setenv("CF_ZSOLVE", "0")
     cfg = small_cfg(cells=cells)
     g, o = pair(cfg)
     U, V, W = cases.initial_field(cfg)
