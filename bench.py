@@ -42,6 +42,8 @@ def algorithmic_bytes(name, n_cells, esz, mh_ratio):
         "fftz_inv": 2 * spec,
         "thomas": 2 * spec,
         "zsolve": 2 * spec,              # fused z transforms + y solve, both launches
+        "zsolve_fwd": 2 * spec,          # its forward launch: read spectrum, write eliminated values
+        "zsolve_bwd": 2 * spec,          # its backward launch: read them, write the solved spectrum
         "ifftx_proju": spec + f + 2 * f,  # read spectrum, write phi, read+write U
         "project_vw": 5 * f,             # read phi,V,W, write V,W
     }
