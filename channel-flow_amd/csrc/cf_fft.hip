@@ -669,25 +669,31 @@ __device__ __forceinline__ void zs_stage_consts(Dims d, int n, Cx<T> *tw, const 
 struct ZsHalf {
     int h, nyp, m, nh, nbh, zr;
     __device__ ZsHalf(int h_, int nyp_, int zr_, int halves) : h(h_), nyp(nyp_), zr(zr_) {
-        m = halves == 2 ? (nyp + 1) / 2 : nyp;
+        m = halves >= 2 ? (nyp + 1) / 2 : nyp;
         nh = h == 0 ? m : nyp - m;
         nbh = (nh + zr - 1) / zr;
     }
-    __device__ int nl(int b) const { return min(zr, nh - b * zr); }
+    __device__ int nl(int b) const { return max(0, min(zr, nh - b * zr)); }
     __device__ int row(int b, int u) const { return h == 0 ? b * zr + u : nyp - 1 - (b * zr + u); }
 };
 // checkpoint batches per half (half 0 has the more rows)
 __host__ __device__ inline int zs_ck_batches(int nyp, int zr, int halves) {
-    return ((halves == 2 ? (nyp + 1) / 2 : nyp) + zr - 1) / zr;
+    return ((halves >= 2 ? (nyp + 1) / 2 : nyp) + zr - 1) / zr;
 }
 
-template <typename T, int WG, int NC, int ZR, int G>
+// halves: 1, one workgroup per slab sweeping all rows (with G = 2 on the
+// first thread group); 2, one workgroup per half slab; 3 (G = 2 only), both
+// halves in one workgroup, thread group g sweeping half g, each batch ZR/2
+// rows of either half (lines g ZR/2 ..), so the sweep per batch is half as
+// long and the grid stays at one workgroup per slab.
+template <typename T, int WG, int NC, int ZR, int G, bool TW>
 __global__ void __launch_bounds__(WG)
 k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPlan pZ,
              const Cx<T> *__restrict__ twZ, const double *__restrict__ kxx, const double *__restrict__ kzz,
              const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
              int halves, int64_t cs) {
     static_assert(G == 1 || (NC > 0 && WG == G * NC), "row groups need a fixed length");
+    static_assert(!TW || G == 2, "both halves in one workgroup need two thread groups");
     extern __shared__ unsigned char smem_raw[];
     constexpr int RPT = ZR / G;  // rows per thread per batch
     spec = case_ptr(spec, cs);
@@ -697,49 +703,57 @@ k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPla
     Cx<T> *tw = lds + ZR * ls;
     double *s_dd = reinterpret_cast<double *>(tw + n), *s_dl = s_dd + nyp, *s_du = s_dl + nyp, *s_kz = s_du + nyp;
     zs_stage_consts<T, WG>(d, n, tw, twZ, s_dd, s_dl, s_du, s_kz, dd, dl, du, kzz);
-    const int kx = halves == 2 ? (int)blockIdx.x >> 1 : (int)blockIdx.x;
-    const ZsHalf hf(halves == 2 ? (int)blockIdx.x & 1 : 0, nyp, ZR, halves);
+    constexpr bool twin = TW;  // halves = 3
     const int g = G == 1 ? 0 : (int)threadIdx.x / n, kz = (int)threadIdx.x - g * n;
-    const bool act = kz < n;           // a column (all threads when G = 2)
-    const bool sweep = act && g == 0;  // runs the elimination of mode (kx, kz)
+    const int kx = halves == 2 ? (int)blockIdx.x >> 1 : (int)blockIdx.x;
+    const int zr = twin ? ZR / 2 : ZR;  // rows of one half per batch
+    const ZsHalf hf(halves == 2 ? (int)blockIdx.x & 1 : (twin ? g : 0), nyp, zr, halves);
+    const int nbl = twin ? ZsHalf(0, nyp, zr, halves).nbh : hf.nbh;  // batches of the loop
+    const int l0 = twin ? g * zr : 0;                                 // the half's first LDS line
+    const bool act = kz < n;                    // a column (all threads when G = 2)
+    const bool sweep = act && (twin || g == 0);  // runs an elimination of mode (kx, kz)
     const double kx2 = kxx[kx];
     const bool zm = kx == 0 && kz == 0;
     const int64_t rs = (int64_t)d.mh * n;  // row stride in elements
     Cx<T> *col = spec + (int64_t)kx * n + kz;
-    double *ck = ckpt + (int64_t)hf.h * zs_ck_batches(nyp, ZR, halves) * rs + (int64_t)kx * n + kz;
+    const int nbc = zs_ck_batches(nyp, zr, halves);
+    double *ck = ckpt + (int64_t)hf.h * nbc * rs + (int64_t)kx * n + kz;
     auto Dj = [&](int j) {
         const double D = (s_dd[j] + kx2) + s_kz[kz];
         return (j == 0 && zm) ? 1.5 * D : D;
     };
+    // row of load slot i: twin, the half's row i; otherwise row g + G i
+    auto slot = [&](int i) { return twin ? i : g + G * i; };
     Cx<T> cur[RPT], nxt[RPT];
     auto loadblk = [&](int b, Cx<T> *dst) {
         const int nl = hf.nl(b);
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
-            const int u = g + G * i;
+            const int u = slot(i);
             dst[i] = (act && u < nl) ? col[hf.row(b, u) * rs] : Cx<T>{(T)0, (T)0};
         }
     };
     // half 0: p = c', (dr, di) = d'; half 1: p = b', (dr, di) = e'
     double p = 0.0, dr = 0.0, di = 0.0;
     loadblk(0, cur);
-    for (int b = 0; b < hf.nbh; b++) {
+    for (int b = 0; b < nbl; b++) {
         const int nl = hf.nl(b);
         if (act) {
 #pragma unroll
             for (int i = 0; i < RPT; i++)
-                if (g + G * i < nl) lds[(g + G * i) * ls + fpad<T>(kz)] = cur[i];
+                if (slot(i) < nl) lds[(l0 + slot(i)) * ls + fpad<T>(kz)] = cur[i];
         }
-        if (b + 1 < hf.nbh) loadblk(b + 1, nxt);
+        if (b + 1 < nbl) loadblk(b + 1, nxt);
         __syncthreads();  // staged rows (and, at b = 0, the constants) visible
-        if constexpr (NC > 0) lds_fft_c<NC, true, WG, -1>(lds, ls, nl, tw);
-        else lds_fft<WG>(lds, ls, nl, pZ, tw, -1);
-        if (sweep) {
+        const int nlf = twin ? ZR : nl;  // twin: both halves' lines (a short half's spare lines unused)
+        if constexpr (NC > 0) lds_fft_c<NC, true, WG, -1>(lds, ls, nlf, tw);
+        else lds_fft<WG>(lds, ls, nlf, pZ, tw, -1);
+        if (sweep && nl > 0) {
 #pragma unroll
             for (int u = 0; u < ZR; u++) {
                 if (u < nl) {
                     const int j = hf.row(b, u);
-                    const Cx<T> v = lds[u * ls + fpad<T>(kz)];
+                    const Cx<T> v = lds[(l0 + u) * ls + fpad<T>(kz)];
                     // coupling to the row already eliminated, and to the next
                     const double cin = hf.h == 0 ? (j > 0 ? s_dl[j - 1] : 0.0) : (j < nyp - 1 ? s_du[j] : 0.0);
                     const double cout = hf.h == 0 ? s_du[j] : (j > 0 ? s_dl[j - 1] : 0.0);
@@ -752,27 +766,29 @@ k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPla
             }
             ck[(int64_t)b * rs] = p;
             if (b == hf.nbh - 1) {  // the half's middle row: the value the join needs
-                double *jn = ckpt + (int64_t)halves * zs_ck_batches(nyp, ZR, halves) * rs + (int64_t)kx * n + kz;
+                double *jn = ckpt + (int64_t)(halves >= 2 ? 2 : 1) * nbc * rs + (int64_t)kx * n + kz;
                 jn[(int64_t)(2 * hf.h) * rs] = dr;
                 jn[(int64_t)(2 * hf.h + 1) * rs] = di;
             }
         }
-        // G = 1: thread kz reads and next writes only its own LDS column, and
-        // the transform's last pass ends with a barrier; G = 2: the other
-        // group stages rows of the same columns next
-        if constexpr (G > 1) __syncthreads();
+        // G = 1 or twin: a thread reads and next writes only its own LDS
+        // column of its own lines, and the transform's last pass ends with a
+        // barrier; G = 2 with one half: the other group stages rows of the
+        // same columns next
+        if constexpr (G > 1 && !twin) __syncthreads();
 #pragma unroll
         for (int i = 0; i < RPT; i++) cur[i] = nxt[i];
     }
 }
 
-template <typename T, int WG, int NC, int ZR, int G>
+template <typename T, int WG, int NC, int ZR, int G, bool TW>
 __global__ void __launch_bounds__(WG)
 k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, FFTPlan pZ,
              const Cx<T> *__restrict__ twZ, const double *__restrict__ kxx, const double *__restrict__ kzz,
              const double *__restrict__ dd, const double *__restrict__ dl, const double *__restrict__ du,
              int halves, int64_t cs) {
     static_assert(G == 1 || (NC > 0 && WG == G * NC), "row groups need a fixed length");
+    static_assert(!TW || G == 2, "both halves in one workgroup need two thread groups");
     extern __shared__ unsigned char smem_raw[];
     constexpr int RPT = ZR / G;
     spec = case_ptr(spec, cs);
@@ -782,43 +798,48 @@ k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, 
     Cx<T> *tw = lds + ZR * ls;
     double *s_dd = reinterpret_cast<double *>(tw + n), *s_dl = s_dd + nyp, *s_du = s_dl + nyp, *s_kz = s_du + nyp;
     zs_stage_consts<T, WG>(d, n, tw, twZ, s_dd, s_dl, s_du, s_kz, dd, dl, du, kzz);
-    const int kx = halves == 2 ? (int)blockIdx.x >> 1 : (int)blockIdx.x;
-    const ZsHalf hf(halves == 2 ? (int)blockIdx.x & 1 : 0, nyp, ZR, halves);
+    constexpr bool twin = TW;  // halves = 3
     const int g = G == 1 ? 0 : (int)threadIdx.x / n, kz = (int)threadIdx.x - g * n;
+    const int kx = halves == 2 ? (int)blockIdx.x >> 1 : (int)blockIdx.x;
+    const int zr = twin ? ZR / 2 : ZR;
+    const ZsHalf hf(halves == 2 ? (int)blockIdx.x & 1 : (twin ? g : 0), nyp, zr, halves);
+    const int nbl = twin ? ZsHalf(0, nyp, zr, halves).nbh : hf.nbh;
+    const int l0 = twin ? g * zr : 0;
     const bool act = kz < n;
-    const bool sweep = act && g == 0;
+    const bool sweep = act && (twin || g == 0);
     const double kx2 = kxx[kx];
     const bool zm = kx == 0 && kz == 0;
     const int64_t rs = (int64_t)d.mh * n;
     Cx<T> *col = spec + (int64_t)kx * n + kz;
-    const int nbc = zs_ck_batches(nyp, ZR, halves);
+    const int nbc = zs_ck_batches(nyp, zr, halves);
     const double *ck0 = ckpt + (int64_t)kx * n + kz;  // half 0's checkpoints
     const double *ck = ck0 + (int64_t)hf.h * nbc * rs;
     auto Dj = [&](int j) {
         const double D = (s_dd[j] + kx2) + s_kz[kz];
         return (j == 0 && zm) ? 1.5 * D : D;
     };
+    auto slot = [&](int i) { return twin ? i : g + G * i; };
     Cx<T> cur[RPT], nxt[RPT];
     auto loadblk = [&](int b, Cx<T> *dst) {
         const int nl = hf.nl(b);
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
-            const int u = g + G * i;
+            const int u = slot(i);
             dst[i] = (act && u < nl) ? col[hf.row(b, u) * rs] : Cx<T>{(T)0, (T)0};
         }
     };
-    const int nb = hf.nbh;
+    const int nb = nbl;
     loadblk(nb - 1, cur);
     // the solution at the half's middle row: halves = 1, x(nyp-1) = d'(nyp-1);
-    // halves = 2, the join of x(m-1) and x(m) from the last c', d' of half 0
+    // two halves, the join of x(m-1) and x(m) from the last c', d' of half 0
     // and b', e' of half 1
     const int m = hf.m, jmid = hf.h == 0 ? m - 1 : m;
     double xr = 0.0, xi = 0.0, c0n = 0.0;
     if (sweep) {
-        const double *jn = ck0 + (int64_t)halves * nbc * rs;
+        const double *jn = ck0 + (int64_t)(halves >= 2 ? 2 : 1) * nbc * rs;
         const double dmr = jn[0], dmi = jn[rs];
-        if (halves == 2) {
-            const ZsHalf h1(1, nyp, ZR, halves);
+        if (halves >= 2) {
+            const ZsHalf h1(1, nyp, zr, halves);
             const double cm = ck0[(int64_t)(nbc - 1) * rs];  // half 0 has nbc batches
             const double bm = ck0[((int64_t)nbc + h1.nbh - 1) * rs];
             const double emr = jn[2 * rs], emi = jn[3 * rs];
@@ -837,19 +858,20 @@ k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, 
     for (int b = nb - 1; b >= 0; b--) {
         const int nl = hf.nl(b);
         double pprev = c0n;  // c' (b') at the row before the batch, wall side
-        // e' / d' of the batch into the LDS lines (G = 1: each thread its own
-        // column; G = 2: both groups, read by the first after the barrier)
+        // e' / d' of the batch into the LDS lines (G = 1 or twin: each thread
+        // its own column of its own lines; G = 2 with one half: both groups,
+        // read by the first after the barrier)
         if (act) {
 #pragma unroll
             for (int i = 0; i < RPT; i++)
-                if (g + G * i < nl) lds[(g + G * i) * ls + fpad<T>(kz)] = cur[i];
+                if (slot(i) < nl) lds[(l0 + slot(i)) * ls + fpad<T>(kz)] = cur[i];
         }
         if (b > 0) {
             loadblk(b - 1, nxt);
             c0n = (sweep && b >= 2) ? ck[(int64_t)(b - 2) * rs] : 0.0;
         }
-        if constexpr (G > 1) __syncthreads();
-        if (sweep) {
+        if constexpr (G > 1 && !twin) __syncthreads();
+        if (sweep && nl > 0) {
             double c[ZR];
 #pragma unroll
             for (int u = 0; u < ZR; u++) {
@@ -867,7 +889,7 @@ k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, 
             for (int u = ZR - 1; u >= 0; u--) {
                 if (u < nl) {
                     const int j = hf.row(b, u);
-                    const Cx<T> dv = lds[u * ls + fpad<T>(kz)];
+                    const Cx<T> dv = lds[(l0 + u) * ls + fpad<T>(kz)];
                     if (j == jmid) {
                         xr = xmr;
                         xi = xmi;
@@ -875,18 +897,19 @@ k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, 
                         xr = (double)dv.x - c[u] * xr;
                         xi = (double)dv.y - c[u] * xi;
                     }
-                    lds[u * ls + fpad<T>(kz)] = Cx<T>{(T)xr, (T)xi};
+                    lds[(l0 + u) * ls + fpad<T>(kz)] = Cx<T>{(T)xr, (T)xi};
                 }
             }
         }
         __syncthreads();
-        if constexpr (NC > 0) lds_fft_c<NC, true, WG, 1>(lds, ls, nl, tw);
-        else lds_fft<WG>(lds, ls, nl, pZ, tw, 1);
+        const int nlf = twin ? ZR : nl;
+        if constexpr (NC > 0) lds_fft_c<NC, true, WG, 1>(lds, ls, nlf, tw);
+        else lds_fft<WG>(lds, ls, nlf, pZ, tw, 1);
         if (act) {
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
-                const int u = g + G * i;
-                if (u < nl) col[hf.row(b, u) * rs] = lds[u * ls + fpad<T>(kz)];
+                const int u = slot(i);
+                if (u < nl) col[hf.row(b, u) * rs] = lds[(l0 + u) * ls + fpad<T>(kz)];
             }
         }
         // the next batch's staging writes only the rows this thread has just
@@ -1453,24 +1476,36 @@ static void launch_zsolve_v(const Dims &d, Cx<T> *spec, double *cscr, const FFTP
                             const double *du, int halves, hipStream_t st) {
     const int n = pZ.n;
     const size_t shm = ((size_t)ZR * (fft_line<T>(n) + 1) + n) * sizeof(Cx<T>) + (3 * (size_t)d.nyp + n) * sizeof(double);
-    const dim3 grid(halves * d.mh, launch_batch().n);
+    if (G == 1 && halves == 3) halves = 2;  // both halves in one workgroup needs two thread groups
+    const dim3 grid((halves == 2 ? 2 : 1) * d.mh, launch_batch().n);
     const int64_t cs = launch_batch().cs;
-    hipLaunchKernelGGL((k_zsolve_fwd<T, WG, NC, ZR, G>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ, kxx, kzz,
-                       dd, dl, du, halves, cs);
-    hipLaunchKernelGGL((k_zsolve_bwd<T, WG, NC, ZR, G>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ, kxx, kzz,
-                       dd, dl, du, halves, cs);
+    if constexpr (G == 2) {
+        if (halves == 3) {
+            hipLaunchKernelGGL((k_zsolve_fwd<T, WG, NC, ZR, G, true>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ,
+                               kxx, kzz, dd, dl, du, halves, cs);
+            hipLaunchKernelGGL((k_zsolve_bwd<T, WG, NC, ZR, G, true>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ,
+                               kxx, kzz, dd, dl, du, halves, cs);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_zsolve_fwd<T, WG, NC, ZR, G, false>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ, kxx,
+                       kzz, dd, dl, du, halves, cs);
+    hipLaunchKernelGGL((k_zsolve_bwd<T, WG, NC, ZR, G, false>), grid, dim3(WG), shm, st, d, spec, cscr, pZ, twZ, kxx,
+                       kzz, dd, dl, du, halves, cs);
 }
 
-// Forms measured at 384x256x384 (forward + backward): FP64, one workgroup per
-// slab with 16-row batches on 768 threads 0.397 ms, both walls with 8-row
-// batches on 384 threads 0.452 ms (386 half slabs on 256 CUs: 130 CUs take
-// two, so the slowest CU still does 256 rows); FP32 0.283 and 0.261 ms.
+// Forms measured at 384x256x384 in one run (forward + backward): FP64, both
+// halves in one 768-thread workgroup 0.391-0.397 ms, one half sweeping all
+// rows 0.429 ms, a workgroup per half on 384 threads 0.469 ms (386 half slabs
+// on 256 CUs: 130 CUs take two, so the slowest CU still does 256 rows);
+// FP32 0.260, 0.294 and 0.251 ms.
 template <typename T>
 void launch_zsolve(const Dims &d, Cx<T> *spec, double *cscr, const FFTPlan &pZ, const Cx<T> *twZ,
                    const double *kxx, const double *kzz, const double *dd, const double *dl, const double *du,
                    hipStream_t st) {
     const int n = pZ.n;
-    const int halves = env_int("CF_ZS_HALVES", sizeof(T) == 4 ? 2 : 1) == 2 ? 2 : 1;
+    int halves = env_int("CF_ZS_HALVES", sizeof(T) == 4 ? 2 : 3);
+    halves = halves == 2 ? 2 : (halves == 3 ? 3 : 1);
     const bool groups = env_int("CF_ZS_G", sizeof(T) == 4 ? 1 : 2) == 2;
 #define CF_ZSV(WG, NC, ZR, G) launch_zsolve_v<T, WG, NC, ZR, G>(d, spec, cscr, pZ, twZ, kxx, kzz, dd, dl, du, halves, st)
     switch (fft_static_sizes() ? n : 0) {

@@ -1,17 +1,18 @@
 """The z transforms fused with the y solve (k_zsolve_fwd / k_zsolve_bwd over
 kx slabs, projection.f90:84-113) against the unfused chain (z transform,
-lane-form y solve, inverse z transform).  Two forms: one workgroup per slab
-eliminating from the bottom wall (the FP64 default), which repeats the lane
-solve's operations in the same order, so in FP64 the state after several
-steps is bit-identical; and two workgroups per slab eliminating from both
-walls and joining in the middle (the FP32 default), which rounds differently
-and agrees to 1e-12 in FP64.  In the mixed path, where the eliminated values
-are stored in FP32, the state stays within FP32 round-off of the unfused
-chain.  The grids cover the z lengths compiled with fixed plans (384, 256,
-192, 128) and run-time plans (96; 300 = 4*3*25 on 512 threads), y row
-counts that leave the last batches part-filled (down to 5 rows: halves of 3
-and 2), the actual-pressure solve, the oracle at 1e-10, and the batched
-step."""
+lane-form y solve, inverse z transform).  Three forms: one workgroup per slab
+eliminating from the bottom wall, which repeats the lane solve's operations
+in the same order, so in FP64 the state after several steps is
+bit-identical; two workgroups per slab eliminating from both walls and
+joining in the middle (the FP32 default); and both halves in one workgroup,
+one thread group each (the FP64 default).  The two-ended forms round
+differently and agree to 1e-12 in FP64.  In the mixed path, where the
+eliminated values are stored in FP32, the state stays within FP32 round-off
+of the unfused chain.  The grids cover the z lengths compiled with fixed
+plans (384, 256, 192, 128) and run-time plans (96; 300 = 4*3*25 on 512
+threads), y row counts that leave the last batches part-filled (down to 5
+rows: halves of 3 and 2), the actual-pressure solve, the oracle at 1e-10,
+and the batched step."""
 import numpy as np
 import pytest
 
@@ -30,7 +31,8 @@ def _cfg(cells, **kw):
 
 
 FORMS = {"one_half": {"CF_ZS_HALVES": "1", "CF_ZS_G": "2"},
-         "two_halves": {"CF_ZS_HALVES": "2", "CF_ZS_G": "1"}}
+         "two_halves": {"CF_ZS_HALVES": "2", "CF_ZS_G": "1"},
+         "twin_halves": {"CF_ZS_HALVES": "3", "CF_ZS_G": "2"}}
 
 
 def _run(cfg, monkeypatch, fused, nsteps=6, graph=True, form=None):
@@ -51,7 +53,7 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("form", ["one_half", "two_halves"])
+@pytest.mark.parametrize("form", ["one_half", "two_halves", "twin_halves"])
 @pytest.mark.parametrize("cells", [(64, 20, 384), (48, 33, 256), (40, 16, 192), (32, 24, 128),
                                    (32, 17, 96), (24, 12, 300), (32, 5, 128)])
 def test_fused_matches_the_unfused_chain_fp64(cells, form, monkeypatch):
@@ -71,7 +73,7 @@ def test_fused_matches_the_unfused_chain_fp64(cells, form, monkeypatch):
         assert relerr(pb, pa) < 1e-10, relerr(pb, pa)
 
 
-@pytest.mark.parametrize("form", ["one_half", "two_halves"])
+@pytest.mark.parametrize("form", ["one_half", "two_halves", "twin_halves"])
 @pytest.mark.parametrize("cells", [(64, 20, 384), (32, 17, 96)])
 def test_fused_mixed_tracks_unfused(cells, form, monkeypatch):
     print("cells %s mixed %s" % (cells, form), flush=True)
@@ -83,7 +85,7 @@ def test_fused_mixed_tracks_unfused(cells, form, monkeypatch):
     assert relerr(pb, pa) < 1e-4, relerr(pb, pa)
 
 
-@pytest.mark.parametrize("form", ["one_half", "two_halves"])
+@pytest.mark.parametrize("form", ["one_half", "two_halves", "twin_halves"])
 def test_fused_against_oracle(form, monkeypatch):
     """Ten steps with the fused solve at a grid the oracle steps in seconds
     (z length 384 on its fixed plan, 20 rows)."""
@@ -106,7 +108,7 @@ def test_fused_against_oracle(form, monkeypatch):
     g.close()
 
 
-@pytest.mark.parametrize("form", ["one_half", "two_halves"])
+@pytest.mark.parametrize("form", ["one_half", "two_halves", "twin_halves"])
 def test_fused_in_the_batched_step(form, monkeypatch):
     """Three cases of one size class through the batched step with the fused
     solve (the checkpoint scratch at the case stride): each member gives the
