@@ -305,6 +305,40 @@ __device__ __forceinline__ void lds_fft_c(Cx<T> *buf, int ls, int nlines, const 
     }
 }
 
+// passes PASS .. END-1 of lds_fft_c, for a caller that runs the first pass
+// from registers or the last one into global memory itself
+template <int N, bool R8, int WG, int SIGN, int PASS, int END, int NS, typename T>
+__device__ __forceinline__ void lds_fft_range(Cx<T> *buf, int ls, int nlines, const Cx<T> *tw) {
+    constexpr CPlan p = cplan(N, R8);
+    if constexpr (PASS < END) {
+        constexpr int R = p.radix[PASS];
+        fft_pass<R, WG, T, N, NS, SIGN>(buf, ls, nlines, N, NS, tw, SIGN);
+        lds_fft_range<N, R8, WG, SIGN, PASS + 1, END, NS * R>(buf, ls, nlines, tw);
+    }
+}
+
+// The twiddled inputs of butterfly (jj, k) of a Stockham pass (Ns, R) of a
+// length-n line, as fft_pass forms them: a[r] times the pass's twiddle r k
+template <int R, typename T>
+__device__ __forceinline__ void fft_twiddle(Cx<T> (&a)[R], int n, int Ns, int k, const Cx<T> *tw, int sign) {
+    const int stw = n / (Ns * R);
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+        Cx<T> w = sizeof(T) == 8 ? tw[Ns - 1 + k * (R - 1) + (r - 1)] : tw[r * k * stw];
+        if (sign > 0) w.y = -w.y;
+        a[r] = cmul(a[r], w);
+    }
+}
+template <int R, typename T>
+__device__ __forceinline__ void fft_dft(Cx<T> (&a)[R], int sign) {
+    if constexpr (R == 2) dft2(a);
+    else if constexpr (R == 4) dft4(a, sign);
+    else if constexpr (R == 8) dft8(a, sign);
+    else if constexpr (R == 3) dft3(a, sign);
+    else if constexpr (R == 6) dft6(a, sign);
+    else dft_odd<R>(a, sign);
+}
+
 // Per-row offsets of the x transforms, written to LDS once per workgroup so
 // that the element loops add the x index instead of re-deriving (j, k) and
 // three-dimensional offsets per element.
@@ -724,13 +758,36 @@ k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPla
     };
     // row of load slot i: twin, the half's row i; otherwise row g + G i
     auto slot = [&](int i) { return twin ? i : g + G * i; };
+    // REG: fixed lengths whose plan starts with radix 8 and whose first-pass
+    // butterflies of a batch number at most one per thread: a thread loads
+    // the eight inputs of its butterfly (line L, jj) from global memory and
+    // runs the first Stockham pass in registers, so the rows are not staged
+    // in LDS first
+    constexpr bool REG = NC > 0 && RPT == 8 && cplan(NC > 0 ? NC : 8, true).radix[0] == 8 && (NC / 8) * ZR <= WG;
+    constexpr int NB1 = NC > 0 ? NC / 8 : 1;  // first-pass butterflies per line
+    const int bt = (int)threadIdx.x, bL = bt / NB1, bj = bt - bL * NB1;
+    const bool bact = bt < NB1 * ZR;
+    Cx<T> *slab = spec + (int64_t)kx * n;
+    // the row of LDS line L of batch b, or -1
+    auto line_row = [&](int b, int L) {
+        if (!twin) return L < hf.nl(b) ? hf.row(b, L) : -1;
+        const ZsHalf hh(L / zr, nyp, zr, halves);
+        const int u = L - hh.h * zr;
+        return u < hh.nl(b) ? hh.row(b, u) : -1;
+    };
     Cx<T> cur[RPT], nxt[RPT];
     auto loadblk = [&](int b, Cx<T> *dst) {
-        const int nl = hf.nl(b);
+        if constexpr (REG) {
+            const int j = bact ? line_row(b, bL) : -1;
 #pragma unroll
-        for (int i = 0; i < RPT; i++) {
-            const int u = slot(i);
-            dst[i] = (act && u < nl) ? col[hf.row(b, u) * rs] : Cx<T>{(T)0, (T)0};
+            for (int r = 0; r < 8; r++) dst[r] = j >= 0 ? slab[j * rs + bj + r * NB1] : Cx<T>{(T)0, (T)0};
+        } else {
+            const int nl = hf.nl(b);
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const int u = slot(i);
+                dst[i] = (act && u < nl) ? col[hf.row(b, u) * rs] : Cx<T>{(T)0, (T)0};
+            }
         }
     };
     // half 0: p = c', (dr, di) = d'; half 1: p = b', (dr, di) = e'
@@ -738,16 +795,35 @@ k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPla
     loadblk(0, cur);
     for (int b = 0; b < nbl; b++) {
         const int nl = hf.nl(b);
-        if (act) {
-#pragma unroll
-            for (int i = 0; i < RPT; i++)
-                if (slot(i) < nl) lds[(l0 + slot(i)) * ls + fpad<T>(kz)] = cur[i];
-        }
-        if (b + 1 < nbl) loadblk(b + 1, nxt);
-        __syncthreads();  // staged rows (and, at b = 0, the constants) visible
         const int nlf = twin ? ZR : nl;  // twin: both halves' lines (a short half's spare lines unused)
-        if constexpr (NC > 0) lds_fft_c<NC, true, WG, -1>(lds, ls, nlf, tw);
-        else lds_fft<WG>(lds, ls, nlf, pZ, tw, -1);
+        if constexpr (REG) {
+            if (b + 1 < nbl) loadblk(b + 1, nxt);
+            // the previous batch's sweep has read its lines (and, at b = 0,
+            // the twiddles are staged)
+            __syncthreads();
+            if (bact && bL < nlf) {
+                Cx<T> a[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) a[r] = cur[r];
+                fft_twiddle<8>(a, NC, 1, 0, tw, -1);
+                fft_dft<8>(a, -1);
+                Cx<T> *dst = lds + bL * ls;
+#pragma unroll
+                for (int r = 0; r < 8; r++) dst[fpad<T>(bj * 8 + r)] = a[r];
+            }
+            __syncthreads();
+            lds_fft_range<(NC > 0 ? NC : 8), true, WG, -1, 1, cplan(NC > 0 ? NC : 8, true).nr, 8>(lds, ls, nlf, tw);
+        } else {
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < RPT; i++)
+                    if (slot(i) < nl) lds[(l0 + slot(i)) * ls + fpad<T>(kz)] = cur[i];
+            }
+            if (b + 1 < nbl) loadblk(b + 1, nxt);
+            __syncthreads();  // staged rows (and, at b = 0, the constants) visible
+            if constexpr (NC > 0) lds_fft_c<NC, true, WG, -1>(lds, ls, nlf, tw);
+            else lds_fft<WG>(lds, ls, nlf, pZ, tw, -1);
+        }
         if (sweep && nl > 0) {
 #pragma unroll
             for (int u = 0; u < ZR; u++) {
@@ -774,8 +850,8 @@ k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPla
         // G = 1 or twin: a thread reads and next writes only its own LDS
         // column of its own lines, and the transform's last pass ends with a
         // barrier; G = 2 with one half: the other group stages rows of the
-        // same columns next
-        if constexpr (G > 1 && !twin) __syncthreads();
+        // same columns next (REG: the barrier at the top of the next batch)
+        if constexpr (G > 1 && !twin && !REG) __syncthreads();
 #pragma unroll
         for (int i = 0; i < RPT; i++) cur[i] = nxt[i];
     }
@@ -819,6 +895,17 @@ k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, 
         return (j == 0 && zm) ? 1.5 * D : D;
     };
     auto slot = [&](int i) { return twin ? i : g + G * i; };
+    // REGL (fixed lengths): the last Stockham pass reads the LDS lines and
+    // stores its outputs straight to the spectrum, a run of consecutive kz
+    // per output index, instead of writing them to LDS for a store pass
+    constexpr bool REGL = NC > 0;
+    Cx<T> *slab = spec + (int64_t)kx * n;
+    auto line_row = [&](int b, int L) {
+        if (!twin) return L < hf.nl(b) ? hf.row(b, L) : -1;
+        const ZsHalf hh(L / zr, nyp, zr, halves);
+        const int u = L - hh.h * zr;
+        return u < hh.nl(b) ? hh.row(b, u) : -1;
+    };
     Cx<T> cur[RPT], nxt[RPT];
     auto loadblk = [&](int b, Cx<T> *dst) {
         const int nl = hf.nl(b);
@@ -903,17 +990,44 @@ k_zsolve_bwd(Dims d, Cx<T> *__restrict__ spec, const double *__restrict__ ckpt, 
         }
         __syncthreads();
         const int nlf = twin ? ZR : nl;
-        if constexpr (NC > 0) lds_fft_c<NC, true, WG, 1>(lds, ls, nlf, tw);
-        else lds_fft<WG>(lds, ls, nlf, pZ, tw, 1);
-        if (act) {
+        if constexpr (REGL) {
+            constexpr int NCL = NC > 0 ? NC : 8;
+            constexpr CPlan pl = cplan(NCL, true);
+            constexpr int RL = pl.radix[pl.nr - 1], NSL = NCL / RL, MAXL = (8 + RL - 1) / RL;
+            lds_fft_range<NCL, true, WG, 1, 0, pl.nr - 1, 1>(lds, ls, nlf, tw);
+            // the last pass (Ns = NSL, so q = 0 and k = jj): butterfly jj of
+            // a line writes outputs jj + r NSL
 #pragma unroll
-            for (int i = 0; i < RPT; i++) {
-                const int u = slot(i);
-                if (u < nl) col[hf.row(b, u) * rs] = lds[(l0 + u) * ls + fpad<T>(kz)];
+            for (int c = 0; c < MAXL; c++) {
+                const int it = (int)threadIdx.x + c * WG;
+                if (it < NSL * nlf) {
+                    const int L = it / NSL, jj = it - L * NSL;
+                    const Cx<T> *src = lds + L * ls;
+                    Cx<T> a[RL];
+#pragma unroll
+                    for (int r = 0; r < RL; r++) a[r] = src[fpad<T>(jj + r * NSL)];
+                    fft_twiddle<RL>(a, NCL, NSL, jj, tw, 1);
+                    fft_dft<RL>(a, 1);
+                    const int j = line_row(b, L);
+                    if (j >= 0) {
+#pragma unroll
+                        for (int r = 0; r < RL; r++) slab[j * rs + jj + r * NSL] = a[r];
+                    }
+                }
             }
+            __syncthreads();  // the next batch writes the lines this pass read
+        } else {
+            lds_fft<WG>(lds, ls, nlf, pZ, tw, 1);
+            if (act) {
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    const int u = slot(i);
+                    if (u < nl) col[hf.row(b, u) * rs] = lds[(l0 + u) * ls + fpad<T>(kz)];
+                }
+            }
+            // the next batch's staging writes only the rows this thread has
+            // just stored from, and the sweep reads them after the barrier above
         }
-        // the next batch's staging writes only the rows this thread has just
-        // stored from, and the sweep reads them after the barrier above
 #pragma unroll
         for (int i = 0; i < RPT; i++) cur[i] = nxt[i];
     }
