@@ -768,17 +768,13 @@ k_zsolve_fwd(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, FFTPla
     const int bt = (int)threadIdx.x, bL = bt / NB1, bj = bt - bL * NB1;
     const bool bact = bt < NB1 * ZR;
     Cx<T> *slab = spec + (int64_t)kx * n;
-    // the row of LDS line L of batch b, or -1
-    auto line_row = [&](int b, int L) {
-        if (!twin) return L < hf.nl(b) ? hf.row(b, L) : -1;
-        const ZsHalf hh(L / zr, nyp, zr, halves);
-        const int u = L - hh.h * zr;
-        return u < hh.nl(b) ? hh.row(b, u) : -1;
-    };
+    // the half and batch row of this thread's line, fixed along the march
+    const ZsHalf bh(twin ? bL / zr : hf.h, nyp, zr, halves);
+    const int bu = twin ? bL - bh.h * zr : bL;
     Cx<T> cur[RPT], nxt[RPT];
     auto loadblk = [&](int b, Cx<T> *dst) {
         if constexpr (REG) {
-            const int j = bact ? line_row(b, bL) : -1;
+            const int j = (bact && bu < bh.nl(b)) ? bh.row(b, bu) : -1;
 #pragma unroll
             for (int r = 0; r < 8; r++) dst[r] = j >= 0 ? slab[j * rs + bj + r * NB1] : Cx<T>{(T)0, (T)0};
         } else {
