@@ -646,13 +646,14 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
 // ---------------------------------------------------------------------------
 // z transforms fused with the y solve over kx slabs (projection.f90:84-113:
 // the z part of the 2-D DFT, the zgtsv per mode, the inverse).  A slab is
-// the nyp rows of nzp consecutive kz at one kx; two workgroups share it, one
-// per half of its rows, and nothing outside the slab is read or written.
-// Rows go in batches of ZR, the next batch's loads in flight while the
-// current one is worked on.
-//   Forward: a batch's rows are staged in LDS (thread kz holds element kz),
-//   transformed along z with the same Stockham passes as k_fftz, and thread
-//   kz advances the elimination of mode (kx, kz) through them, real and
+// the nyp rows of nzp consecutive kz at one kx, owned by one workgroup or by
+// two (one per half of its rows), and nothing outside the slab is read or
+// written.  Rows go in batches of ZR, the next batch's loads in flight while
+// the current one is worked on.
+//   Forward: a batch's rows are transformed along z with the same Stockham
+//   passes as k_fftz (fixed lengths: the first pass in registers from the
+//   loads; run-time plans: the rows staged in LDS first), and thread kz
+//   advances the elimination of mode (kx, kz) through them, real and
 //   imaginary parts sharing the pivots.  The eliminated right-hand side is
 //   stored in place of the row, and the pivot ratio of the batch's last row
 //   goes to an FP64 checkpoint.
@@ -660,7 +661,8 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
 //   stored values are loaded, the pivot ratios recomputed from the
 //   checkpoint of the batch nearer the wall (the same operations on the same
 //   values, so the same ratios), and the substitution writes the solution
-//   into the LDS lines, which are transformed back along z and stored.
+//   into the LDS lines, which are transformed back along z (fixed lengths:
+//   the last pass stores to the spectrum).
 // The spectrum moves four times (read, write; read, write) instead of seven
 // (z transform 2, lane solve 3, inverse z transform 2).  Arithmetic is FP64;
 // with FP32 storage the eliminated values are rounded to FP32 like the
@@ -688,6 +690,7 @@ k_thomas_lane(Dims d, Cx<T> *__restrict__ spec, double *__restrict__ ckpt, const
 // after the checkpoints: the backward kernel reads the other half's from
 // there, since the other workgroup overwrites that row of the spectrum with
 // its solution.
+
 // twiddles, the y operator and the z wavenumbers into LDS: the sweeps then
 // read no global memory, so their first row waits for no load in flight
 template <typename T, int WG>
