@@ -447,7 +447,7 @@ void Ctx<S>::poisson(S *const *F, S *phi, const Scalars *corr, const DtParts *dt
     tic("div_fftx");
     launch_div_fftx<S, S>(d, met, F[0], F[1], F[2], spec, pM, twM, twN, stream);
     toc();
-    if (zsolve_fused(d)) {
+    if (cscr && zsolve_fused(d)) {  // the scratch is sized for either form when allocated
         tic("zsolve");
         launch_zsolve<S>(d, spec, cscr, pZ, twZ, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
         toc();
@@ -939,7 +939,7 @@ int Ctx<S>::compute_pressure(double *P) {
                               h.prm.omega_z, stream, true);
         launch_pressure_bc<S>(d, F[1], c, Rv, bcs, ysrc, stream);
         launch_div_fftx<double, S>(d, met, Ru, Rv, Rw, spec, pM, twM, twN, stream, ysrc);
-        if (zsolve_fused(d)) {
+        if (cscr && zsolve_fused(d)) {  // the scratch is sized for either form when allocated
             launch_zsolve<S>(d, spec, cscr, pZ, twZ, m_kxx, m_kzz, m_dd, m_dl, m_du, stream);
         } else {
             launch_fftz<S>(d, spec, pZ, twZ, -1, stream);
