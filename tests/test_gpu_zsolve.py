@@ -3,15 +3,18 @@ kx slabs, projection.f90:84-113) against the unfused chain (z transform,
 lane-form y solve, inverse z transform).  Three forms: one workgroup per slab
 eliminating from the bottom wall, which repeats the lane solve's operations
 in the same order, so in FP64 the state after several steps is
-bit-identical; two workgroups per slab eliminating from both walls and
+bit-identical at the fixed-plan z lengths (within 1e-14 with run-time plans,
+where the compiler may contract the same operations differently in the two
+kernels); two workgroups per slab eliminating from both walls and
 joining in the middle (the FP32 default); and both halves in one workgroup,
 one thread group each (the FP64 default).  The two-ended forms round
 differently and agree to 1e-12 in FP64.  In the mixed path, where the
 eliminated values are stored in FP32, the state stays within FP32 round-off
 of the unfused chain.  The grids cover the z lengths compiled with fixed
-plans (384, 256, 192, 128) and run-time plans (96; 300 = 4*3*25 on 512
-threads), y row counts that leave the last batches part-filled (down to 5
-rows: halves of 3 and 2), the actual-pressure solve, the oracle at 1e-10,
+plans (384, 256, 192, 128) and run-time plans (96; 300 = 4*3*25 and the
+largest accepted length, 512, on 512 threads), y row counts that leave the
+last batches part-filled (down to 5 rows, halves of 3 and 2, and the
+smallest accepted, 4), the actual-pressure solve, the oracle at 1e-10,
 and the batched step."""
 import numpy as np
 import pytest
@@ -55,17 +58,25 @@ def relerr(a, b):
 
 @pytest.mark.parametrize("form", ["one_half", "two_halves", "twin_halves"])
 @pytest.mark.parametrize("cells", [(64, 20, 384), (48, 33, 256), (40, 16, 192), (32, 24, 128),
-                                   (32, 17, 96), (24, 12, 300), (32, 5, 128)])
+                                   (32, 17, 96), (24, 12, 300), (32, 5, 128), (32, 4, 128), (16, 8, 512)])
 def test_fused_matches_the_unfused_chain_fp64(cells, form, monkeypatch):
     print("cells %s fp64 %s" % (cells, form), flush=True)
     cfg = _cfg(cells)
     (a, sa, pa) = _run(cfg, monkeypatch, False)
     (b, sb, pb) = _run(cfg, monkeypatch, True, form=form)
-    if form == "one_half":
+    if form == "one_half" and cells[2] in (384, 256, 192, 128):
+        # fixed-plan lengths: the transforms and sweeps of both chains round alike
         for x, y in zip(a, b):
             assert np.array_equal(x, y), relerr(y, x)
         assert sa["dt"] == sb["dt"] and sa["dPdx"] == sb["dPdx"]
         assert np.array_equal(pa, pb), relerr(pb, pa)
+    elif form == "one_half":
+        # run-time plans: the same operations compiled into two kernels may
+        # contract multiply-adds differently (a length-512 case differs by
+        # one ulp)
+        for x, y in zip(a, b):
+            assert relerr(y, x) < 1e-14, relerr(y, x)
+        assert relerr(pb, pa) < 1e-12, relerr(pb, pa)
     else:
         for x, y in zip(a, b):
             assert relerr(y, x) < 1e-12, relerr(y, x)
